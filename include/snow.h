@@ -1,0 +1,191 @@
+/*
+ * snow.h - C ABI of the MI355X snow MPM timestep.
+ *
+ * This library replaces the GLSL compute-shader dispatch chain of the explicit
+ * snow MPM step (reference: src/snow/simulation/timeUpdate/explicittimeupdate.cpp:4-63)
+ * and the SSBO buffer interface it reads and writes (src/snow/object/particlesystem.cpp:101-228,
+ * src/snow/object/grid.cpp:181-255, src/snow/object/collisionObjects.cpp:25-88).
+ *
+ * All functions return 0 on success or a negative SNOW_E* code; snow_last_error()
+ * returns the message of the last failure on a context (or of a failed snow_create*).
+ * Nothing here calls exit(). One context = one HIP stream; use a context from one
+ * host thread at a time. Calls are asynchronous on the context's stream except
+ * snow_read_*, snow_sync and snow_create*.
+ *
+ * Matrices are 3x3, column-major (element (row r, col c) at index c*3+r), the
+ * order GLSL uses for mat3/mat4 (particle.hpp:8-11, particleAdvance.glsl:230).
+ */
+#ifndef SNOW_H
+#define SNOW_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNOW_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+#define SNOW_OK 0
+#define SNOW_E_ARG (-1)        /* invalid argument or descriptor */
+#define SNOW_E_HIP (-2)        /* HIP runtime error (message has the HIP string) */
+#define SNOW_E_NONFINITE (-3)  /* snow_check_finite found NaN/Inf in particle state */
+#define SNOW_E_CAPACITY (-4)   /* size beyond what the context was created for */
+#define SNOW_E_NODEVICE (-5)   /* no usable gfx950 device / code object */
+#define SNOW_E_STATE (-6)      /* call not valid in the context's current state */
+
+/* ---- parameter blocks ------------------------------------------------------ */
+
+/* Material constants. Defaults (snow_material_defaults) are the values of
+ * src/snow/utils/defines.hpp:50-54 plus the shader literals alpha=0.95
+ * (g2pTransfer.glsl:2) and g=(0,-9.81,0) (gForceCompute.glsl:39). */
+typedef struct snow_material {
+  float young;        /* E0, YOUNG_MODULUS (defines.hpp:50)            */
+  float poisson;      /* nu, POISSON (defines.hpp:51)                  */
+  float hardening;    /* xi, HARDENING (defines.hpp:52)                */
+  float crit_comp;    /* theta_c, CRIT_COMPRESSION (defines.hpp:53)    */
+  float crit_stretch; /* theta_s, CRIT_STRETCH (defines.hpp:54)        */
+  float flip_alpha;   /* FLIP blend alpha (g2pTransfer.glsl:2)         */
+  float gravity[3];   /* body acceleration (gForceCompute.glsl:39)     */
+  /* Scale applied to the cubic B-spline weight gradient. The reference omits
+   * the 1/h factor (cubic.include.glsl:25-40), i.e. grad_scale = 1. A value of
+   * 0 selects the physical gradient 1/h. */
+  float grad_scale;
+} snow_material;
+
+/* Dense node grid: node (i,j,k) sits at origin + spacing*(i,j,k), index
+ * i + j*dim[0] + k*dim[0]*dim[1] (grid.cpp:205-215, gridIndex.include.glsl:7-9). */
+typedef struct snow_grid {
+  int32_t dim[3];
+  float origin[3];
+  float spacing;
+} snow_grid;
+
+/* One collision object, laid out as the reference's three vec4 SSBO entries
+ * cx/cv/cn (collisionObjects.cpp:25-66, gForceCompute.glsl:40-53):
+ *   pos,type      = cx  (type 0 half-plane, 1 sphere, 2 axis-aligned box [extension])
+ *   vel,radius    = cv  (radius used by spheres)
+ *   normal,friction = cn (half-plane normal as stored; box half-extents for type 2)
+ * Any other type never collides (gForceCompute.glsl:49-53). */
+typedef struct snow_collider {
+  float pos[3];
+  float type;
+  float vel[3];
+  float radius;
+  float normal[3];
+  float friction;
+} snow_collider;
+
+/* One scene: particles + grid + material + colliders. Host arrays are copied at
+ * snow_create* and never retained. fe/fp may be NULL for identity. */
+typedef struct snow_scene_desc {
+  int64_t num_particles;
+  const float* x;     /* [N][3] positions                                   */
+  const float* v;     /* [N][3] velocities in m/s                           */
+  const float* mass;  /* [N]                                                */
+  const float* fe;    /* [N][9] elastic deformation gradient or NULL (=I)   */
+  const float* fp;    /* [N][9] plastic deformation gradient or NULL (=I)   */
+  snow_grid grid;
+  snow_material material;
+  int32_t num_colliders;
+  const snow_collider* colliders;
+  float dt;           /* step used by snow_step_batch (PHYSIC_DT, defines.hpp:23) */
+} snow_scene_desc;
+
+/* Context options (flags for snow_create_batch). */
+#define SNOW_FLAG_COMPENSATED_G2P (1u << 0) /* compensated sums in the G2P gather     */
+#define SNOW_FLAG_MIXED (1u << 1)           /* FP64 SVD fallback near the clamp       */
+#define SNOW_FLAG_KEEP_GRID (1u << 2)       /* snow_step also stores m, mv, f per node */
+
+typedef struct snow_ctx snow_ctx;
+
+/* ---- lifetime --------------------------------------------------------------- */
+
+/* Fill defaults: defines.hpp:14-54 (grid 201^3, h=0.05, origin (0.5125,0.5125,1.5125),
+ * E=6e6, nu=0.2, xi=30, theta_c=1e-2, theta_s=1e-3, dt=1e-3). */
+void snow_material_defaults(snow_material* out);
+void snow_grid_defaults(snow_grid* out);
+
+/* Replaces ExplicitTimeUpdate's constructor + ParticleSystem/Grid/CollisionObjects::initSSBO
+ * (explicittimeupdate.hpp:14-25, particlesystem.cpp:101-228, grid.cpp:181-255,
+ * collisionObjects.cpp:25-66). A single scene is a batch of one. */
+int snow_create(const snow_scene_desc* scene, uint32_t flags, snow_ctx** out);
+int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_t flags,
+                      snow_ctx** out);
+int snow_destroy(snow_ctx* ctx);
+const char* snow_last_error(const snow_ctx* ctx);
+int snow_num_scenes(const snow_ctx* ctx);
+int64_t snow_num_particles(const snow_ctx* ctx, int32_t scene);
+
+/* ---- step interface (TimeUpdate::init / update, timeUpdate.hpp:10-18) ---------- */
+
+/* ExplicitTimeUpdate::init (explicittimeupdate.cpp:4-34): grid reset, grid mass,
+ * particle rest density (gMassCompute.glsl, pVolumeCompute.glsl). */
+int snow_init(snow_ctx* ctx);
+/* ExplicitTimeUpdate::update(dt) (explicittimeupdate.cpp:36-63): rigidAdvance,
+ * gReset, p2gTransfer, gForceCompute, g2pTransfer, particleAdvance, for every
+ * scene, with the same dt (narrowed to float as technique.cpp:122-125 does). */
+int snow_step(snow_ctx* ctx, double dt);
+/* Same step with each scene's own snow_scene_desc::dt. */
+int snow_step_batch(snow_ctx* ctx);
+/* Run `steps` batched steps as replays of one captured HIP graph. */
+int snow_run_graph(snow_ctx* ctx, int32_t steps);
+
+/* ---- per-stage entry points (the Technique::dispatch of each compute shader) ---- */
+int snow_stage_grid_mass(snow_ctx* ctx);        /* gMassCompute.glsl:33-84 (init)      */
+int snow_stage_particle_volume(snow_ctx* ctx);  /* pVolumeCompute.glsl:116-166 (init)  */
+int snow_stage_rigid_advance(snow_ctx* ctx, double dt); /* rigidAdvance.glsl:14-18    */
+int snow_stage_grid_reset(snow_ctx* ctx);       /* gReset.glsl:11-16                   */
+int snow_stage_p2g(snow_ctx* ctx);              /* p2gTransfer.glsl:44-151             */
+int snow_stage_grid_update(snow_ctx* ctx, double dt); /* gForceCompute.glsl:57-103     */
+int snow_stage_g2p(snow_ctx* ctx);              /* g2pTransfer.glsl:53-143 (+ advance) */
+int snow_stage_particle_advance(snow_ctx* ctx, double dt); /* particleAdvance.glsl:75-217 */
+
+/* ---- state transfer --------------------------------------------------------- */
+
+/* Particle state of one scene in its original (creation) order. Any pointer may be
+ * NULL to skip that field. rho0 is the rest density from snow_init (pv.w*1e-6). */
+int snow_read_particles(snow_ctx* ctx, int32_t scene, float* x, float* v, float* fe, float* fp,
+                        float* rho0);
+/* Overwrite particle state of one scene (original order); NULL fields are kept. */
+int snow_write_particles(snow_ctx* ctx, int32_t scene, const float* x, const float* v,
+                         const float* fe, const float* fp, const float* rho0);
+/* Dense grid fields of one scene ([D^3] m, [D^3][3] mv, f, vstar), as left by the last
+ * P2G / grid update (gv, gf, gvn buffers, grid.cpp:226-254). NULL fields are skipped. */
+int snow_read_grid(snow_ctx* ctx, int32_t scene, float* m, float* mv, float* f, float* vstar);
+/* Overwrite the dense grid fields read by the grid update and G2P stages. */
+int snow_write_grid(snow_ctx* ctx, int32_t scene, const float* m, const float* mv,
+                    const float* f, const float* vstar);
+/* Collider state (positions advance with rigidAdvance). */
+int snow_read_colliders(snow_ctx* ctx, int32_t scene, snow_collider* out);
+
+/* Reference SSBO encodings (Appendix of particlesystem.cpp:101-228): pxm vec4 (x, m),
+ * pv ivec4 (v*1e8, rho0*1e6), pFE / pFP mat4 (column-major, upper-left 3x3).
+ * Converts to / from the SoA arrays of snow_scene_desc. */
+int snow_import_reference_particles(int64_t n, const float* pxm, const int32_t* pv,
+                                    const float* pfe, const float* pfp, float* x, float* v,
+                                    float* mass, float* fe, float* fp);
+int snow_export_reference_particles(snow_ctx* ctx, int32_t scene, float* pxm, int32_t* pv,
+                                    float* pfe, float* pfp);
+
+/* ---- sync, checks, timing --------------------------------------------------- */
+int snow_sync(snow_ctx* ctx);
+/* Returns SNOW_E_NONFINITE if any particle of any scene has a non-finite x, v, F_E, F_P. */
+int snow_check_finite(snow_ctx* ctx);
+/* Per-stage timing with hipEvents on the context stream (BenchmarkerGPU::time,
+ * benchmarker.hpp:72-78). Stage names follow explicittimeupdate.cpp:39-62. */
+int snow_enable_stage_timers(snow_ctx* ctx, int32_t enable);
+int snow_num_stages(void);
+const char* snow_stage_name(int32_t stage);
+/* Accumulated milliseconds and call counts per stage since the last reset. */
+int snow_stage_times(snow_ctx* ctx, double* total_ms, int64_t* calls);
+int snow_reset_stage_times(snow_ctx* ctx);
+/* Number of grid nodes of a scene with mass > 0 after the last P2G (for byte counts). */
+int snow_active_nodes(snow_ctx* ctx, int32_t scene, int64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SNOW_H */

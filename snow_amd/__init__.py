@@ -1,0 +1,353 @@
+"""snow_amd - MI355X-native explicit snow MPM timestep.
+
+Host-side mirror of the reference's simulation interface over the C ABI of
+``include/snow.h`` (the HIP library ``libsnow_amd.so``):
+
+* ``ParticleSystem``, ``Grid``, ``Collider``/``CollisionObjects``, ``PhysicalScene`` follow
+  src/snow/object/{particlesystem,grid,collider,collisionObjects}.hpp and src/snow/scene/Scene.hpp;
+* ``ExplicitTimeUpdate`` (``init()``, ``update(dt)``) follows
+  src/snow/simulation/timeUpdate/explicittimeupdate.hpp:11-30 and timeUpdate.hpp:10-18;
+* ``MPMPhysicEngine`` follows src/snow/simulation/MPMPhysicEngine.hpp:7-16.
+
+All compute runs in the HIP library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._capi import (FLAG_COMPENSATED_G2P, FLAG_KEEP_GRID, FLAG_MIXED, Collider as _CCollider, Grid as _CGrid,
+                    Material as _CMaterial, SceneDesc as _CSceneDesc, SnowError, load_library, ptr)
+
+__all__ = [
+    "Material", "Grid", "Collider", "CollisionObjects", "ParticleSystem", "PhysicalScene", "ExplicitTimeUpdate",
+    "MPMPhysicEngine", "SnowContext", "SnowError", "FLAG_COMPENSATED_G2P", "FLAG_MIXED", "FLAG_KEEP_GRID",
+    "HALF_PLANE", "SPHERE", "BOX", "default_boundary_colliders",
+]
+
+HALF_PLANE, SPHERE, BOX = 0, 1, 2
+
+
+@dataclass
+class Material:
+    """Material constants; defaults are src/snow/utils/defines.hpp:50-54 and the shader literals."""
+    young: float = 6e6
+    poisson: float = 0.2
+    hardening: float = 30.0
+    crit_comp: float = 1.0e-2
+    crit_stretch: float = 1e-3
+    flip_alpha: float = 0.95
+    gravity: tuple = (0.0, -9.81, 0.0)
+    grad_scale: float = 1.0  # 1 = reference (no 1/h in cubic.include.glsl:25-40); 0 = physical 1/h
+
+    def to_c(self) -> _CMaterial:
+        m = _CMaterial()
+        m.young, m.poisson, m.hardening = self.young, self.poisson, self.hardening
+        m.crit_comp, m.crit_stretch, m.flip_alpha = self.crit_comp, self.crit_stretch, self.flip_alpha
+        for k in range(3):
+            m.gravity[k] = self.gravity[k]
+        m.grad_scale = self.grad_scale
+        return m
+
+
+@dataclass
+class Grid:
+    """Dense node grid, Grid(x, y, z, spacing, pos_x, pos_y, pos_z) as grid.hpp:20-28."""
+    dimx: int = 201
+    dimy: int = 201
+    dimz: int = 201
+    h: float = 0.05
+    x_off: float = 0.5125
+    y_off: float = 0.5125
+    z_off: float = 1.5125
+
+    @property
+    def dims(self):
+        return (self.dimx, self.dimy, self.dimz)
+
+    @property
+    def num_nodes(self) -> int:
+        return self.dimx * self.dimy * self.dimz
+
+    def to_c(self) -> _CGrid:
+        g = _CGrid()
+        g.dim[0], g.dim[1], g.dim[2] = self.dimx, self.dimy, self.dimz
+        g.origin[0], g.origin[1], g.origin[2] = self.x_off, self.y_off, self.z_off
+        g.spacing = self.h
+        return g
+
+    def node_positions(self) -> np.ndarray:
+        """Node positions x0 + h*(i,j,k), x fastest (grid.cpp:205-215)."""
+        k, j, i = np.meshgrid(np.arange(self.dimz), np.arange(self.dimy), np.arange(self.dimx), indexing="ij")
+        pos = np.stack([self.x_off + self.h * i, self.y_off + self.h * j, self.z_off + self.h * k], -1)
+        return pos.reshape(-1, 3).astype(np.float32)
+
+
+@dataclass
+class Collider:
+    """Collision object in the cx/cv/cn encoding of collisionObjects.cpp:25-66."""
+    position: tuple
+    type: int = HALF_PLANE
+    velocity: tuple = (0.0, 0.0, 0.0)
+    radius: float = 0.0
+    normal: tuple = (0.0, 0.0, 0.0)  # box: half-extents
+    friction: float = 0.2
+
+    def to_c(self) -> _CCollider:
+        c = _CCollider()
+        for k in range(3):
+            c.pos[k], c.vel[k], c.normal[k] = self.position[k], self.velocity[k], self.normal[k]
+        c.type, c.radius, c.friction = float(self.type), self.radius, self.friction
+        return c
+
+    def as_array(self) -> np.ndarray:
+        return np.array([*self.position, float(self.type), *self.velocity, self.radius, *self.normal, self.friction],
+                        dtype=np.float64)
+
+
+@dataclass
+class CollisionObjects:
+    colliders: list = field(default_factory=list)
+
+
+def default_boundary_colliders(grid: Grid, offset_cells: int = 4, friction: float = 0.2) -> list:
+    """The six inward half-planes launchSnow adds at GRID_COLLISION_PLANE_OFFSET cells from the grid faces
+    (explicit.cpp:9-67)."""
+    lo = (grid.x_off + offset_cells * grid.h, grid.y_off + offset_cells * grid.h, grid.z_off + offset_cells * grid.h)
+    hi = (grid.x_off + (grid.dimx - offset_cells) * grid.h, grid.y_off + (grid.dimy - offset_cells) * grid.h,
+          grid.z_off + (grid.dimz - offset_cells) * grid.h)
+    out = []
+    for n in ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0)):
+        out.append(Collider(lo, HALF_PLANE, normal=n, friction=friction))
+    for n in ((-1.0, 0.0, 0.0), (0.0, -1.0, 0.0), (0.0, 0.0, -1.0)):
+        out.append(Collider(hi, HALF_PLANE, normal=n, friction=friction))
+    return out
+
+
+class ParticleSystem:
+    """Particle set (particlesystem.hpp:15-59). Arrays are float32: x [N,3], v [N,3] (m/s), mass [N],
+    fe/fp [N,9] column-major (identity by default, particle.hpp:8-11)."""
+
+    def __init__(self, x=None, v=None, mass=None, fe=None, fp=None):
+        self.x = np.zeros((0, 3), np.float32) if x is None else np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        n = len(self.x)
+        self.v = np.zeros((n, 3), np.float32) if v is None else np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+        self.mass = (np.full(n, 1.25e-3, np.float32) if mass is None
+                     else np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (n,))))
+        eye = np.eye(3, dtype=np.float32).reshape(-1)
+        self.fe = np.tile(eye, (n, 1)) if fe is None else np.ascontiguousarray(fe, np.float32).reshape(-1, 9)
+        self.fp = np.tile(eye, (n, 1)) if fp is None else np.ascontiguousarray(fp, np.float32).reshape(-1, 9)
+
+    def __len__(self):
+        return len(self.x)
+
+    @classmethod
+    def from_file(cls, filename: str, pos=(0.0, 0.0, 0.0), scale=(1.0, 1.0, 1.0), mass: float = 6.25e-3):
+        """ParticleSystem::initParticlesFromFile (particlesystem.cpp:14-38): CSV lines "x,y,z", each
+        scaled and offset, velocity 0, mass 6.25e-3."""
+        pts = []
+        with open(filename) as f:
+            for line in f:
+                parts = line.strip().split(",")
+                if len(parts) < 3 or not parts[0]:
+                    continue
+                pts.append([float(parts[0]), float(parts[1]), float(parts[2])])
+        x = np.asarray(pts, np.float32).reshape(-1, 3) * np.asarray(scale, np.float32) + np.asarray(pos, np.float32)
+        return cls(x=x, mass=np.full(len(x), mass, np.float32))
+
+    @classmethod
+    def from_reference_buffers(cls, pxm: np.ndarray, pv: np.ndarray, pfe=None, pfp=None):
+        """Decode the reference SSBO encodings (pxm vec4, pv ivec4 x1e8, mat4 column-major)."""
+        lib = load_library()
+        n = len(pxm)
+        pxm = np.ascontiguousarray(pxm, np.float32).reshape(n, 4)
+        pv = np.ascontiguousarray(pv, np.int32).reshape(n, 4)
+        pfe = None if pfe is None else np.ascontiguousarray(pfe, np.float32).reshape(n, 16)
+        pfp = None if pfp is None else np.ascontiguousarray(pfp, np.float32).reshape(n, 16)
+        x, v = np.empty((n, 3), np.float32), np.empty((n, 3), np.float32)
+        m, fe, fp = np.empty(n, np.float32), np.empty((n, 9), np.float32), np.empty((n, 9), np.float32)
+        rc = lib.snow_import_reference_particles(n, ptr(pxm), ptr(pv, C.c_int32), ptr(pfe), ptr(pfp), ptr(x), ptr(v),
+                                                 ptr(m), ptr(fe), ptr(fp))
+        if rc:
+            raise SnowError(rc, "import of reference buffers failed")
+        return cls(x, v, m, fe, fp)
+
+
+@dataclass
+class PhysicalScene:
+    """PhysicalScene{particleSys, grid, colliderSys} (Scene.hpp:7-11) plus the material and step size."""
+    particles: ParticleSystem
+    grid: Grid
+    colliders: CollisionObjects = field(default_factory=CollisionObjects)
+    material: Material = field(default_factory=Material)
+    dt: float = 1e-3
+
+
+class SnowContext:
+    """Owns one ``snow_ctx`` (one HIP stream) holding a batch of scenes."""
+
+    def __init__(self, scenes, flags: int = 0):
+        if isinstance(scenes, PhysicalScene):
+            scenes = [scenes]
+        self.lib = load_library()
+        self.scenes = list(scenes)
+        self._keep = []
+        descs = (_CSceneDesc * len(self.scenes))()
+        for d, s in zip(descs, self.scenes):
+            ps = s.particles
+            arrs = [ps.x, ps.v, ps.mass, ps.fe, ps.fp]
+            self._keep.extend(arrs)
+            d.num_particles = len(ps)
+            d.x, d.v, d.mass, d.fe, d.fp = (ptr(a) for a in arrs)
+            d.grid = s.grid.to_c()
+            d.material = s.material.to_c()
+            cols = (_CCollider * max(1, len(s.colliders.colliders)))()
+            for k, c in enumerate(s.colliders.colliders):
+                cols[k] = c.to_c()
+            self._keep.append(cols)
+            d.num_colliders = len(s.colliders.colliders)
+            d.colliders = C.cast(cols, C.POINTER(_CCollider))
+            d.dt = s.dt
+        h = C.c_void_p()
+        rc = self.lib.snow_create_batch(descs, len(self.scenes), flags, C.byref(h))
+        self._keep = []
+        if rc:
+            raise SnowError(rc, self.lib.snow_last_error(None).decode())
+        self.h = h
+        self.flags = flags
+
+    # -- plumbing
+    def _chk(self, rc):
+        if rc:
+            raise SnowError(rc, self.lib.snow_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.snow_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- step interface
+    def init(self):
+        self._chk(self.lib.snow_init(self.h))
+
+    def step(self, dt: float | None = None):
+        self._chk(self.lib.snow_step_batch(self.h) if dt is None else self.lib.snow_step(self.h, dt))
+
+    def run_graph(self, steps: int):
+        self._chk(self.lib.snow_run_graph(self.h, steps))
+
+    def sync(self):
+        self._chk(self.lib.snow_sync(self.h))
+
+    def check_finite(self):
+        self._chk(self.lib.snow_check_finite(self.h))
+
+    # -- stages
+    def stage(self, name: str, dt: float | None = None):
+        fn = getattr(self.lib, "snow_stage_" + name)
+        self._chk(fn(self.h) if dt is None else fn(self.h, dt))
+
+    # -- state transfer
+    def read_particles(self, scene: int = 0) -> dict:
+        n = self.lib.snow_num_particles(self.h, scene)
+        out = {"x": np.empty((n, 3), np.float32), "v": np.empty((n, 3), np.float32),
+               "fe": np.empty((n, 9), np.float32), "fp": np.empty((n, 9), np.float32),
+               "rho0": np.empty(n, np.float32)}
+        self._chk(self.lib.snow_read_particles(self.h, scene, ptr(out["x"]), ptr(out["v"]), ptr(out["fe"]),
+                                               ptr(out["fp"]), ptr(out["rho0"])))
+        return out
+
+    def write_particles(self, scene: int = 0, x=None, v=None, fe=None, fp=None, rho0=None):
+        conv = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (x, v, fe, fp, rho0)]
+        self._chk(self.lib.snow_write_particles(self.h, scene, *(ptr(a) for a in conv)))
+
+    def read_grid(self, scene: int = 0, fields=("m", "mv", "f", "vstar")) -> dict:
+        g = self.scenes[scene].grid
+        nn = g.num_nodes
+        out = {}
+        shapes = {"m": (nn,), "mv": (nn, 3), "f": (nn, 3), "vstar": (nn, 3)}
+        for k in fields:
+            out[k] = np.empty(shapes[k], np.float32)
+        self._chk(self.lib.snow_read_grid(self.h, scene, ptr(out.get("m")), ptr(out.get("mv")), ptr(out.get("f")),
+                                          ptr(out.get("vstar"))))
+        return out
+
+    def write_grid(self, scene: int = 0, m=None, mv=None, f=None, vstar=None):
+        conv = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (m, mv, f, vstar)]
+        self._chk(self.lib.snow_write_grid(self.h, scene, *(ptr(a) for a in conv)))
+
+    def read_colliders(self, scene: int = 0) -> np.ndarray:
+        nc = len(self.scenes[scene].colliders.colliders)
+        arr = (_CCollider * max(1, nc))()
+        self._chk(self.lib.snow_read_colliders(self.h, scene, arr))
+        return np.array([[*c.pos, c.type, *c.vel, c.radius, *c.normal, c.friction] for c in arr[:nc]],
+                        dtype=np.float32).reshape(nc, 12)
+
+    def export_reference(self, scene: int = 0):
+        n = self.lib.snow_num_particles(self.h, scene)
+        pxm, pv = np.empty((n, 4), np.float32), np.empty((n, 4), np.int32)
+        pfe, pfp = np.empty((n, 16), np.float32), np.empty((n, 16), np.float32)
+        self._chk(self.lib.snow_export_reference_particles(self.h, scene, ptr(pxm), ptr(pv, C.c_int32), ptr(pfe),
+                                                           ptr(pfp)))
+        return pxm, pv, pfe, pfp
+
+    # -- timing
+    def enable_stage_timers(self, on: bool = True):
+        self._chk(self.lib.snow_enable_stage_timers(self.h, 1 if on else 0))
+
+    def stage_times(self) -> dict:
+        ns = self.lib.snow_num_stages()
+        ms, calls = np.zeros(ns, np.float64), np.zeros(ns, np.int64)
+        self._chk(self.lib.snow_stage_times(self.h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                            calls.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {self.lib.snow_stage_name(i).decode(): (float(ms[i]), int(calls[i])) for i in range(ns)}
+
+    def reset_stage_times(self):
+        self._chk(self.lib.snow_reset_stage_times(self.h))
+
+    def active_nodes(self, scene: int = 0) -> int:
+        v = C.c_int64()
+        self._chk(self.lib.snow_active_nodes(self.h, scene, C.byref(v)))
+        return int(v.value)
+
+
+class ExplicitTimeUpdate:
+    """TimeUpdate-shaped driver (timeUpdate.hpp:10-18): ``init()`` and ``update(dt)``."""
+
+    def __init__(self, scene: PhysicalScene, flags: int = 0):
+        self.scene = scene
+        self.ctx = SnowContext([scene], flags)
+
+    def init(self):
+        self.ctx.init()
+
+    def update(self, dt: float):
+        self.ctx.step(dt)
+
+
+class MPMPhysicEngine:
+    """physicEngine / MPMPhysicEngine (physicEngine.hpp:7-16, MPMPhysicEngine.cpp:3-7)."""
+
+    def __init__(self, integration: ExplicitTimeUpdate):
+        self.integration = integration
+
+    def init(self) -> bool:
+        self.integration.init()
+        return True
+
+    def update(self, dt: float):
+        self.integration.update(dt)

@@ -1,0 +1,786 @@
+// snow_kernels.hpp - HIP kernels of the snow MPM step for gfx950 (CDNA4).
+//
+// Data layout (all scenes of a batch concatenated; see snow_capi.hip for allocation):
+//  particles, ping-pong buffers in "sorted" order after each step:
+//    rec[i]   : 4 x float4 = { x y z m | A00 A01 A02 vx | A10 A11 A12 vy | A20 A21 A22 vz }
+//               (A = per-particle force matrix, row-major, f_node = -A grad w)
+//    fe/fp    : 9 SoA float planes, plane e = c*3+r (column-major 3x3)
+//    rho0,pid : rest density (pv.w of the reference), original index
+//  grid, dense per scene (node i + j*Dx + k*Dx*Dy):
+//    g_vs     : float4 (v*, m)           - gForceCompute output gvn (+ mass)
+//    g_flip   : float4 (v* - alpha p/m)  - FLIP/PIC blend term used by G2P
+//    g_mmv,g_f: float4 (p, m), (f, 0)    - gv / gf of the reference (stage API, debug)
+//  cell tiles: 8x8x8 cells per tile over the shifted cell range c = base + 2, base in [-2, D]
+//    (a cubic stencil of base cell b touches nodes b-1..b+2); a tile's 11^3-node halo starts
+//    at node 8k-3; tpart[tile][q][11^3] holds the tile's partial sums of the 7 P2G quantities.
+#pragma once
+#include "snow_device.hpp"
+
+namespace snow {
+
+constexpr int TILE = 8;              // cells per tile edge
+constexpr int TILE_CELLS = 512;      // 8^3
+constexpr int HALO = 11;             // TILE + 3 nodes per edge
+constexpr int HALO_NODES = 1331;     // 11^3
+constexpr int NQ = 7;                // m, p.xyz, f.xyz
+
+struct SceneDev {
+  int D[3];
+  int ncol;
+  float origin[3];
+  float h;
+  float young, poisson, hardening, crit_comp;
+  float crit_stretch, alpha, gscale, dt;
+  float g[3];
+  int col_off;
+  int ct[3];          // cell tiles per axis
+  uint32_t ct_off;    // global index of the scene's first cell tile
+  int nt[3];          // node tiles per axis
+  uint32_t nt_off;    // global index of the scene's first node tile
+  uint32_t dead_tile; // global index of the scene's out-of-grid tile
+  uint32_t pad0;
+  int64_t node_off;   // offset of node 0 in the dense node arrays
+  int64_t p_off;      // first particle
+  int64_t p_cnt;      // particle count
+  float mu0, lambda0; // E/(2(1+nu)), E nu/((1+nu)(1-2nu))
+  float pad1[2];
+};
+
+// Global cell key of a position: (tile) * 512 + (cell within tile); particles whose base
+// cell lies outside [-2, D] (no in-grid stencil node) go to the scene's dead tile.
+__device__ __forceinline__ uint32_t cell_key(const SceneDev& S, float x, float y, float z) {
+  const float ux = (x - S.origin[0]) / S.h;
+  const float uy = (y - S.origin[1]) / S.h;
+  const float uz = (z - S.origin[2]) / S.h;
+  const bool in = ux > -3.0f && uy > -3.0f && uz > -3.0f && ux < float(S.D[0] + 1) &&
+                  uy < float(S.D[1] + 1) && uz < float(S.D[2] + 1);
+  if (!in) return S.dead_tile * TILE_CELLS;
+  const int cx = int(ux) + 2, cy = int(uy) + 2, cz = int(uz) + 2;
+  const uint32_t tile = S.ct_off + uint32_t((cx >> 3) + S.ct[0] * ((cy >> 3) + S.ct[1] * (cz >> 3)));
+  return tile * TILE_CELLS + uint32_t((cx & 7) + 8 * (cy & 7) + 64 * (cz & 7));
+}
+
+// ---------------------------------------------------------------------------------------
+// Keys of the current particle order (after upload or host writes).
+__global__ void __launch_bounds__(256) k_keys(const SceneDev* __restrict__ scenes,
+                                              const uint16_t* __restrict__ pscene,
+                                              const float4* __restrict__ rec, uint32_t* __restrict__ keys,
+                                              uint32_t* __restrict__ vals, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SceneDev& S = scenes[pscene[i]];
+  const float4 xm = rec[4 * size_t(i)];
+  keys[i] = cell_key(S, xm.x, xm.y, xm.z);
+  vals[i] = i;
+}
+
+// ---------------------------------------------------------------------------------------
+// Start/end of every non-empty cell in the sorted key array. Entries of empty cells are
+// left stale; readers validate them with keys[start] == cell. Block 0 also advances the
+// colliders (rigidAdvance.glsl:14-18): no kernel of this launch reads them.
+__global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict__ keys, uint32_t n,
+                                                     uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend,
+                                                     const SceneDev* __restrict__ scenes, int nscenes,
+                                                     Collider* __restrict__ cols, float dt_override) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) cstart[k] = i;
+    if (i == n - 1 || keys[i + 1] != k) cend[k] = i + 1;
+  }
+  if (blockIdx.x == 0 && dt_override >= 0.0f) {
+    for (int s = 0; s < nscenes; ++s) {
+      const SceneDev& S = scenes[s];
+      const float dt = dt_override > 0.0f ? dt_override : S.dt;
+      for (int c = threadIdx.x; c < S.ncol; c += blockDim.x) {
+        Collider& C = cols[S.col_off + c];
+        C.px += dt * C.vx; C.py += dt * C.vy; C.pz += dt * C.vz;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// P2G on one 8^3 cell tile (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84).
+// Wave w handles one cell per phase; lane l is the stencil offset o = (l&3, (l>>2)&3, l>>4)
+// (getIJK order, neighborIndex.include.glsl:6-12), so the 64 lanes form the particle's
+// 4x4x4 stencil and every particle of the cell is a wave-uniform (scalar) load. The 64
+// phases walk the colour classes (cx&3, cy&3, cz&3): cells of one colour are 4 apart, so
+// their stencils never overlap and the 8 waves update the LDS tile without atomics. Each
+// node therefore sums its contributions in a fixed order (cell colour, then particle order),
+// compensated (TwoSum) across cells: the result is bit-reproducible.
+__global__ void __launch_bounds__(512, 1) k_p2g_tile(
+    const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+    const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+    const float4* __restrict__ rec, float* __restrict__ tpart, int* __restrict__ tile_active,
+    uint32_t n, int skip_heavy) {
+  __shared__ uint32_t s_cs[TILE_CELLS];
+  __shared__ uint32_t s_ce[TILE_CELLS];
+  __shared__ float s_hi[NQ * HALO_NODES];
+  __shared__ float s_lo[NQ * HALO_NODES];
+
+  const uint32_t tile = blockIdx.x;
+  const int sid = tile_scene[tile];
+  const SceneDev& S = scenes[sid];
+  const int t = threadIdx.x;
+  if (tile == S.dead_tile) {
+    if (t == 0) tile_active[tile] = 0;
+    return;
+  }
+  {
+    const uint32_t gc = tile * TILE_CELLS + t;
+    const uint32_t st = cstart[gc];
+    const bool valid = st < n && keys[st] == gc;
+    s_cs[t] = valid ? st : 0u;
+    s_ce[t] = valid ? cend[gc] : 0u;
+    if (!__syncthreads_or(valid)) {
+      if (t == 0) tile_active[tile] = 0;
+      return;
+    }
+  }
+  for (int i = t; i < NQ * HALO_NODES; i += 512) { s_hi[i] = 0.0f; s_lo[i] = 0.0f; }
+  __syncthreads();
+
+  const uint32_t tl = tile - S.ct_off;
+  const int tx = int(tl % uint32_t(S.ct[0]));
+  const int ty = int((tl / uint32_t(S.ct[0])) % uint32_t(S.ct[1]));
+  const int tz = int(tl / uint32_t(S.ct[0] * S.ct[1]));
+  const int wave = t >> 6, lane = t & 63;
+  const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
+  const float orgx = S.origin[0], orgy = S.origin[1], orgz = S.origin[2], h = S.h;
+
+  for (int phase = 0; phase < 64; ++phase) {
+    const int cx = (phase & 3) + 4 * (wave & 1);
+    const int cy = ((phase >> 2) & 3) + 4 * ((wave >> 1) & 1);
+    const int cz = (phase >> 4) + 4 * (wave >> 2);
+    const int cl = cx + 8 * cy + 64 * cz;
+    const uint32_t b = __builtin_amdgcn_readfirstlane(s_cs[cl]);
+    const uint32_t e = __builtin_amdgcn_readfirstlane(s_ce[cl]);
+    if (b < e) {
+      // node of this lane: base + o - 1 with base = 8*t + c - 2
+      const float gx = float(TILE * tx + cx - 3 + ox);
+      const float gy = float(TILE * ty + cy - 3 + oy);
+      const float gz = float(TILE * tz + cz - 3 + oz);
+      float am = 0.f, apx = 0.f, apy = 0.f, apz = 0.f, afx = 0.f, afy = 0.f, afz = 0.f;
+      for (uint32_t j = b; j < e; ++j) {
+        const uint32_t p = vals[j];
+        const float4 r0 = rec[4 * size_t(p)];
+        const float4 r1 = rec[4 * size_t(p) + 1];
+        const float4 r2 = rec[4 * size_t(p) + 2];
+        const float4 r3 = rec[4 * size_t(p) + 3];
+        const float m = r0.w;
+        if (skip_heavy && fabsf(m) > 100.0f) continue;  // p2gTransfer.glsl:64-66 (not in gMassCompute)
+        const float ux = (r0.x - orgx) / h, uy = (r0.y - orgy) / h, uz = (r0.z - orgz) / h;
+        const float dx = gx - ux, dy = gy - uy, dz = gz - uz;
+        const float wx = bspline(dx), wy = bspline(dy), wz = bspline(dz);
+        const float qx = bspline_grad(-dx), qy = bspline_grad(-dy), qz = bspline_grad(-dz);
+        const float wyz = wy * wz;
+        const float w = wx * wyz;
+        const float g0 = qx * wyz, g1 = wx * qy * wz, g2 = wx * wy * qz;
+        am += m * w;
+        apx += (r1.w * m) * w;
+        apy += (r2.w * m) * w;
+        apz += (r3.w * m) * w;
+        afx -= r1.x * g0 + r1.y * g1 + r1.z * g2;
+        afy -= r2.x * g0 + r2.y * g1 + r2.z * g2;
+        afz -= r3.x * g0 + r3.y * g1 + r3.z * g2;
+      }
+      const int jn = (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
+      two_sum_acc(s_hi[0 * HALO_NODES + jn], s_lo[0 * HALO_NODES + jn], am);
+      two_sum_acc(s_hi[1 * HALO_NODES + jn], s_lo[1 * HALO_NODES + jn], apx);
+      two_sum_acc(s_hi[2 * HALO_NODES + jn], s_lo[2 * HALO_NODES + jn], apy);
+      two_sum_acc(s_hi[3 * HALO_NODES + jn], s_lo[3 * HALO_NODES + jn], apz);
+      two_sum_acc(s_hi[4 * HALO_NODES + jn], s_lo[4 * HALO_NODES + jn], afx);
+      two_sum_acc(s_hi[5 * HALO_NODES + jn], s_lo[5 * HALO_NODES + jn], afy);
+      two_sum_acc(s_hi[6 * HALO_NODES + jn], s_lo[6 * HALO_NODES + jn], afz);
+    }
+    __syncthreads();
+  }
+  float* out = tpart + size_t(tile) * (NQ * HALO_NODES);
+  for (int i = t; i < NQ * HALO_NODES; i += 512) out[i] = s_hi[i] + s_lo[i];
+  if (t == 0) tile_active[tile] = 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// Merge of the tile partials into node values + grid update (gForceCompute.glsl:57-103).
+// One block per 8^3 node tile K; node n = 8K + r receives from cell tiles K (halo index
+// r + 3) and, when r >= 5 on an axis, K+1 (halo index r - 5): at most 8 tiles, summed in a
+// fixed order with TwoSum. Node tiles none of whose contributing cell tiles is active are
+// skipped: no particle's stencil reaches them.
+template <bool KEEP, bool UPDATE>
+__global__ void __launch_bounds__(512) k_grid(const SceneDev* __restrict__ scenes,
+                                              const uint16_t* __restrict__ ntile_scene,
+                                              const int* __restrict__ tile_active,
+                                              const float* __restrict__ tpart,
+                                              float4* __restrict__ g_mmv, float4* __restrict__ g_f,
+                                              float4* __restrict__ g_vs, float4* __restrict__ g_flip,
+                                              const Collider* __restrict__ cols, float dt_override,
+                                              unsigned long long* __restrict__ active_count) {
+  const uint32_t nt = blockIdx.x;
+  const int sid = ntile_scene[nt];
+  const SceneDev& S = scenes[sid];
+  const uint32_t ntl = nt - S.nt_off;
+  const int Kx = int(ntl % uint32_t(S.nt[0]));
+  const int Ky = int((ntl / uint32_t(S.nt[0])) % uint32_t(S.nt[1]));
+  const int Kz = int(ntl / uint32_t(S.nt[0] * S.nt[1]));
+  const int t = threadIdx.x;
+  const int rx = t & 7, ry = (t >> 3) & 7, rz = t >> 6;
+  const int nx = TILE * Kx + rx, ny = TILE * Ky + ry, nz = TILE * Kz + rz;
+  const bool inside = nx < S.D[0] && ny < S.D[1] && nz < S.D[2];
+
+  float hi[NQ], lo[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) { hi[q] = 0.f; lo[q] = 0.f; }
+  bool any = false;
+  if (inside) {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
+      if ((ax && rx < 5) || (ay && ry < 5) || (az && rz < 5)) continue;
+      const uint32_t ctile = S.ct_off + uint32_t((Kx + ax) + S.ct[0] * ((Ky + ay) + S.ct[1] * (Kz + az)));
+      if (!tile_active[ctile]) continue;
+      any = true;
+      const int j = (ax ? rx - 5 : rx + 3) + HALO * (ay ? ry - 5 : ry + 3) + HALO * HALO * (az ? rz - 5 : rz + 3);
+      const float* src = tpart + size_t(ctile) * (NQ * HALO_NODES) + j;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) two_sum_acc(hi[q], lo[q], src[q * HALO_NODES]);
+    }
+  }
+  if (!__syncthreads_or(any)) return;
+  if (!inside) return;
+  const float m = hi[0] + lo[0];
+  const float px = hi[1] + lo[1], py = hi[2] + lo[2], pz = hi[3] + lo[3];
+  const float fx = hi[4] + lo[4], fy = hi[5] + lo[5], fz = hi[6] + lo[6];
+  const int64_t node = S.node_off + nx + int64_t(S.D[0]) * (ny + int64_t(S.D[1]) * nz);
+  if (KEEP) {
+    g_mmv[node] = make_float4(px, py, pz, m);
+    g_f[node] = make_float4(fx, fy, fz, 0.0f);
+  }
+  if (UPDATE) {
+    const float dt = dt_override > 0.0f ? dt_override : S.dt;
+    float vx = 0.f, vy = 0.f, vz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+    if (m > 0.0f) {
+      vx = px / m + dt * (fx / m + S.g[0]);
+      vy = py / m + dt * (fy / m + S.g[1]);
+      vz = pz / m + dt * (fz / m + S.g[2]);
+      const float xi = S.origin[0] + S.h * float(nx);
+      const float yi = S.origin[1] + S.h * float(ny);
+      const float zi = S.origin[2] + S.h * float(nz);
+      collide(xi, yi, zi, vx, vy, vz, cols + S.col_off, S.ncol);
+      bx = vx - S.alpha * (px / m);
+      by = vy - S.alpha * (py / m);
+      bz = vz - S.alpha * (pz / m);
+    }
+    g_vs[node] = make_float4(vx, vy, vz, m);
+    g_flip[node] = make_float4(bx, by, bz, 0.0f);
+  }
+  if (active_count) {
+    const unsigned long long ballot = __ballot(m > 0.0f);
+    if ((t & 63) == 0 && ballot) atomicAdd(active_count + sid, (unsigned long long)__popcll(ballot));
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Dense grid update for the per-stage API (reads g_mmv/g_f of every node of every scene).
+__global__ void __launch_bounds__(256) k_grid_update_dense(const SceneDev* __restrict__ scenes, int nscenes,
+                                                           const float4* __restrict__ g_mmv,
+                                                           const float4* __restrict__ g_f,
+                                                           float4* __restrict__ g_vs, float4* __restrict__ g_flip,
+                                                           const Collider* __restrict__ cols, float dt_override,
+                                                           int64_t total_nodes) {
+  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= total_nodes) return;
+  int sid = 0;
+  while (sid + 1 < nscenes && scenes[sid + 1].node_off <= node) ++sid;
+  const SceneDev& S = scenes[sid];
+  const int64_t l = node - S.node_off;
+  const int nx = int(l % S.D[0]), ny = int((l / S.D[0]) % S.D[1]), nz = int(l / (int64_t(S.D[0]) * S.D[1]));
+  const float4 mm = g_mmv[node], ff = g_f[node];
+  const float m = mm.w;
+  const float dt = dt_override > 0.0f ? dt_override : S.dt;
+  float vx = 0.f, vy = 0.f, vz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+  if (m > 0.0f) {
+    vx = mm.x / m + dt * (ff.x / m + S.g[0]);
+    vy = mm.y / m + dt * (ff.y / m + S.g[1]);
+    vz = mm.z / m + dt * (ff.z / m + S.g[2]);
+    collide(S.origin[0] + S.h * float(nx), S.origin[1] + S.h * float(ny), S.origin[2] + S.h * float(nz), vx, vy,
+            vz, cols + S.col_off, S.ncol);
+    bx = vx - S.alpha * (mm.x / m);
+    by = vy - S.alpha * (mm.y / m);
+    bz = vz - S.alpha * (mm.z / m);
+  }
+  g_vs[node] = make_float4(vx, vy, vz, m);
+  g_flip[node] = make_float4(bx, by, bz, 0.0f);
+}
+
+// From dense (m, p, v*) written by the host: the G2P inputs.
+__global__ void __launch_bounds__(256) k_grid_import(const SceneDev* __restrict__ scenes,
+                                                     const float4* __restrict__ g_mmv, float4* __restrict__ g_vs,
+                                                     float4* __restrict__ g_flip, int nscenes, int64_t total_nodes) {
+  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= total_nodes) return;
+  int sid = 0;
+  while (sid + 1 < nscenes && scenes[sid + 1].node_off <= node) ++sid;
+  const float alpha = scenes[sid].alpha;
+  const float4 mm = g_mmv[node];
+  float4 vs = g_vs[node];
+  vs.w = mm.w;
+  g_vs[node] = vs;
+  if (mm.w > 0.0f)
+    g_flip[node] = make_float4(vs.x - alpha * (mm.x / mm.w), vs.y - alpha * (mm.y / mm.w), vs.z - alpha * (mm.z / mm.w), 0.f);
+  else
+    g_flip[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Dense export of one scene's grid (zeros where the tiled path did not touch the node).
+__global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict__ scenes, int sid,
+                                                     const int* __restrict__ tile_active, int dense_valid,
+                                                     const float4* __restrict__ g_mmv, const float4* __restrict__ g_f,
+                                                     const float4* __restrict__ g_vs, float* __restrict__ out_m,
+                                                     float* __restrict__ out_mv, float* __restrict__ out_f,
+                                                     float* __restrict__ out_vs) {
+  const SceneDev& S = scenes[sid];
+  const int64_t nn = int64_t(S.D[0]) * S.D[1] * S.D[2];
+  const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (l >= nn) return;
+  const int nx = int(l % S.D[0]), ny = int((l / S.D[0]) % S.D[1]), nz = int(l / (int64_t(S.D[0]) * S.D[1]));
+  bool live = dense_valid != 0;
+  if (!live) {
+    const int Kx = nx >> 3, Ky = ny >> 3, Kz = nz >> 3, rx = nx & 7, ry = ny & 7, rz = nz & 7;
+    for (int a = 0; a < 8 && !live; ++a) {
+      const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
+      if ((ax && rx < 5) || (ay && ry < 5) || (az && rz < 5)) continue;
+      const uint32_t ctile = S.ct_off + uint32_t((Kx + ax) + S.ct[0] * ((Ky + ay) + S.ct[1] * (Kz + az)));
+      live = tile_active[ctile] != 0;
+    }
+  }
+  const int64_t node = S.node_off + l;
+  const float4 mm = live ? g_mmv[node] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 ff = live ? g_f[node] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 vs = live ? g_vs[node] : make_float4(0.f, 0.f, 0.f, 0.f);
+  if (out_m) out_m[l] = mm.w;
+  if (out_mv) { out_mv[3 * l] = mm.x; out_mv[3 * l + 1] = mm.y; out_mv[3 * l + 2] = mm.z; }
+  if (out_f) { out_f[3 * l] = ff.x; out_f[3 * l + 1] = ff.y; out_f[3 * l + 2] = ff.z; }
+  if (out_vs) { out_vs[3 * l] = vs.x; out_vs[3 * l + 1] = vs.y; out_vs[3 * l + 2] = vs.z; }
+}
+
+// ---------------------------------------------------------------------------------------
+// Per-particle stencil weights: for axis value u, base = trunc(u), offsets o = 0..3 ->
+// node base+o-1, d = node - u, w[o] = N(d), q[o] = N'(-d) (cubic.include.glsl:21-40 with
+// the distance vector of g2pTransfer.glsl:85-90).
+__device__ __forceinline__ void axis_weights(float u, int base, float w[4], float q[4]) {
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    const float d = float(base + o - 1) - u;
+    w[o] = bspline(d);
+    q[o] = bspline_grad(-d);
+  }
+}
+
+// Gather of one particle from the node fields (g2pTransfer.glsl:53-143):
+// vpn = sum_i w (v*_i - alpha p_i/m_i), dvp = sum_i v*_i (x) grad w (column k = v* * dw/dx_k).
+__device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, float z,
+                                           const float4* __restrict__ g_vs, const float4* __restrict__ g_flip,
+                                           float vpn[3], float dvp[9], bool compensated) {
+  vpn[0] = vpn[1] = vpn[2] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dvp[i] = 0.f;
+  const float ux = (x - S.origin[0]) / S.h, uy = (y - S.origin[1]) / S.h, uz = (z - S.origin[2]) / S.h;
+  const bool in = ux > -3.0f && uy > -3.0f && uz > -3.0f && ux < float(S.D[0] + 1) && uy < float(S.D[1] + 1) &&
+                  uz < float(S.D[2] + 1);
+  if (!in) return false;
+  const int bx = int(ux), by = int(uy), bz = int(uz);
+  float wx[4], wy[4], wz[4], qx[4], qy[4], qz[4];
+  axis_weights(ux, bx, wx, qx);
+  axis_weights(uy, by, wy, qy);
+  axis_weights(uz, bz, wz, qz);
+  float lv[3] = {0.f, 0.f, 0.f}, ld[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) ld[i] = 0.f;
+  const float gs = S.gscale;
+  for (int oz = 0; oz < 4; ++oz) {
+    const int iz = bz + oz - 1;
+    if (iz < 0 || iz >= S.D[2]) continue;
+    for (int oy = 0; oy < 4; ++oy) {
+      const int iy = by + oy - 1;
+      if (iy < 0 || iy >= S.D[1]) continue;
+      const int64_t row = S.node_off + int64_t(S.D[0]) * (iy + int64_t(S.D[1]) * iz);
+      const float wyz = wy[oy] * wz[oz];
+#pragma unroll
+      for (int ox = 0; ox < 4; ++ox) {
+        const int ix = bx + ox - 1;
+        if (ix < 0 || ix >= S.D[0]) continue;
+        const float4 vs = g_vs[row + ix];
+        if (!(vs.w > 0.0f)) continue;
+        const float4 fl = g_flip[row + ix];
+        const float w = wx[ox] * wyz;
+        const float g0 = gs * (qx[ox] * wyz), g1 = gs * (wx[ox] * qy[oy] * wz[oz]), g2 = gs * (wx[ox] * wy[oy] * qz[oz]);
+        const float c[12] = {fl.x * w, fl.y * w, fl.z * w, vs.x * g0, vs.y * g0, vs.z * g0,
+                             vs.x * g1, vs.y * g1, vs.z * g1, vs.x * g2, vs.y * g2, vs.z * g2};
+        if (compensated) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) two_sum_acc(vpn[k], lv[k], c[k]);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) two_sum_acc(dvp[k], ld[k], c[3 + k]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) vpn[k] += c[k];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) dvp[k] += c[3 + k];
+        }
+      }
+    }
+  }
+  if (compensated) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vpn[k] += lv[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dvp[k] += ld[k];
+  }
+  return true;
+}
+
+// particleAdvance.glsl:225-283 in precision R: F_E, F_P update with the singular-value clamp.
+template <class R>
+__device__ __forceinline__ void deform_update(const float dvp[9], float dt, const float FEf[9], const float FPf[9],
+                                              float crit_c, float crit_s, float FEo[9], float FPo[9]) {
+  R L[9], FE[9], FP[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { L[i] = R(dt) * R(dvp[i]); FE[i] = R(FEf[i]); FP[i] = R(FPf[i]); }
+  L[0] += R(1); L[4] += R(1); L[8] += R(1);
+  R FEn[9], FEFP[9], Fn[9];
+  mul(L, FE, FEn);
+  mul(FE, FP, FEFP);
+  mul(L, FEFP, Fn);
+  R W[9], S[9], V[9];
+  svd3(FEn, W, S, V);
+  const R lb = R(1) - R(crit_c), ub = R(1) + R(crit_s);
+  S[0] = S[0] < lb ? lb : (S[0] > ub ? ub : S[0]);
+  S[4] = S[4] < lb ? lb : (S[4] > ub ? ub : S[4]);
+  S[8] = S[8] < lb ? lb : (S[8] > ub ? ub : S[8]);
+  R WS[9], Fe2[9];
+  mul(W, S, WS);
+  mul_bt(WS, V, Fe2);
+  // F_P = V diag(1/S) W^T F  (particleAdvance.glsl:270-275)
+  R VSi[9];
+  const R i0 = R(1) / S[0], i1 = R(1) / S[4], i2 = R(1) / S[8];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) { VSi[r] = V[r] * i0; VSi[3 + r] = V[3 + r] * i1; VSi[6 + r] = V[6 + r] * i2; }
+  R VSiWt[9], Fp2[9];
+  mul_bt(VSi, W, VSiWt);
+  mul(VSiWt, Fn, Fp2);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    FEo[i] = float(rnd(R(1e9) * Fe2[i]) / R(1e9));
+    FPo[i] = float(rnd(R(1e9) * Fp2[i]) / R(1e9));
+  }
+}
+
+// Singular values of F_E' near the clamp bounds or an ill-conditioned F_E' send the
+// particle to the FP64 path (mixed precision).
+__device__ __forceinline__ bool needs_fp64(const float dvp[9], float dt, const float FE[9], float crit_c,
+                                           float crit_s) {
+  float L[9], FEn[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) L[i] = dt * dvp[i];
+  L[0] += 1.f; L[4] += 1.f; L[8] += 1.f;
+  mul(L, FE, FEn);
+  float W[9], S[9], V[9];
+  svd3(FEn, W, S, V);
+  const float tol = 2e-4f;
+  const float lb = 1.f - crit_c, ub = 1.f + crit_s;
+  bool near = false;
+  const float d[3] = {S[0], S[4], S[8]};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) near |= fabsf(fabsf(d[k]) - lb) < tol || fabsf(fabsf(d[k]) - ub) < tol;
+  const float smax = fmaxf(fabsf(d[0]), fmaxf(fabsf(d[1]), fabsf(d[2])));
+  const float smin = fminf(fabsf(d[0]), fminf(fabsf(d[1]), fabsf(d[2])));
+  near |= !(smin > 1e-3f * smax);
+  return near;
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused G2P + particle advance (g2pTransfer.glsl:53-143, particleAdvance.glsl:75-217).
+// Thread i handles sorted position i: it reads the particle from its previous slot
+// vals[i] (buffers *_in), writes the new state to slot i (buffers *_out), computes the next
+// step's force matrix (second SVD, of the stored F_E) and the next step's cell key. The
+// permutation of the sort is thereby applied for free by the writes of this kernel.
+template <bool MIXED, bool COMP>
+__global__ void __launch_bounds__(256) k_g2p_advance(
+    const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ pscene, const uint32_t* __restrict__ vals,
+    const float4* __restrict__ rec_in, const float* __restrict__ fe_in, const float* __restrict__ fp_in,
+    const float* __restrict__ rho_in, const uint32_t* __restrict__ pid_in, float4* __restrict__ rec_out,
+    float* __restrict__ fe_out, float* __restrict__ fp_out, float* __restrict__ rho_out,
+    uint32_t* __restrict__ pid_out, uint32_t* __restrict__ keys_next, uint32_t* __restrict__ vals_next,
+    const float4* __restrict__ g_vs, const float4* __restrict__ g_flip, const Collider* __restrict__ cols,
+    float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SceneDev& S = scenes[pscene[i]];
+  const float dt = dt_override > 0.0f ? dt_override : S.dt;
+  const uint32_t p = vals[i];
+  const float4 r0 = rec_in[4 * size_t(p)];
+  const float4 r1 = rec_in[4 * size_t(p) + 1];
+  const float4 r2 = rec_in[4 * size_t(p) + 2];
+  const float4 r3 = rec_in[4 * size_t(p) + 3];
+  float FE[9], FP[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { FE[e] = fe_in[size_t(e) * n + p]; FP[e] = fp_in[size_t(e) * n + p]; }
+  const float rho0 = rho_in[p];
+  const uint32_t id = pid_in[p];
+
+  float vpn[3], dvp[9];
+  g2p_gather(S, r0.x, r0.y, r0.z, g_vs, g_flip, vpn, dvp, COMP);
+
+  float FEo[9], FPo[9];
+  bool wide = false;
+  if (MIXED) wide = needs_fp64(dvp, dt, FE, S.crit_comp, S.crit_stretch);
+  if (wide) {
+    deform_update<double>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo);
+    if (fp64_count) atomicAdd(fp64_count, 1ull);
+  } else {
+    deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo);
+  }
+
+  // velocity, collisions at the pre-advection position, advection (particleAdvance.glsl:318-359)
+  float vx = r1.w * S.alpha + vpn[0], vy = r2.w * S.alpha + vpn[1], vz = r3.w * S.alpha + vpn[2];
+  collide(r0.x, r0.y, r0.z, vx, vy, vz, cols + S.col_off, S.ncol);
+  const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
+
+  // next step's force matrix (p2gTransfer.glsl:109-141, once per particle)
+  float A[9];
+  if (MIXED && wide) {
+    double FEd[9], FPd[9], Ad[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) { FEd[e] = FEo[e]; FPd[e] = FPo[e]; }
+    force_matrix<double>(FEd, FPd, double(r0.w), double(rho0), double(S.mu0), double(S.lambda0),
+                         double(S.hardening), double(S.gscale), Ad);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) A[e] = float(Ad[e]);
+  } else {
+    force_matrix<float>(FEo, FPo, r0.w, rho0, S.mu0, S.lambda0, S.hardening, S.gscale, A);
+  }
+  // rec rows: A row r = (A[r], A[3+r], A[6+r]) from the column-major A
+  rec_out[4 * size_t(i)] = make_float4(x, y, z, r0.w);
+  rec_out[4 * size_t(i) + 1] = make_float4(A[0], A[3], A[6], vx);
+  rec_out[4 * size_t(i) + 2] = make_float4(A[1], A[4], A[7], vy);
+  rec_out[4 * size_t(i) + 3] = make_float4(A[2], A[5], A[8], vz);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { fe_out[size_t(e) * n + i] = FEo[e]; fp_out[size_t(e) * n + i] = FPo[e]; }
+  rho_out[i] = rho0;
+  pid_out[i] = id;
+  keys_next[i] = cell_key(S, x, y, z);
+  vals_next[i] = i;
+}
+
+// Split stages for the per-stage API: G2P alone writes (vpn, dvp) per particle slot;
+// the advance alone consumes them in place (g2pTransfer.glsl / particleAdvance.glsl).
+__global__ void __launch_bounds__(256) k_g2p_only(const SceneDev* __restrict__ scenes,
+                                                  const uint16_t* __restrict__ pscene,
+                                                  const float4* __restrict__ rec, const float4* __restrict__ g_vs,
+                                                  const float4* __restrict__ g_flip, float* __restrict__ aux,
+                                                  uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SceneDev& S = scenes[pscene[i]];
+  const float4 r0 = rec[4 * size_t(i)];
+  float vpn[3], dvp[9];
+  g2p_gather(S, r0.x, r0.y, r0.z, g_vs, g_flip, vpn, dvp, false);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) aux[size_t(k) * n + i] = vpn[k];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) aux[size_t(3 + k) * n + i] = dvp[k];
+}
+
+__global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict__ scenes,
+                                                      const uint16_t* __restrict__ pscene, float4* __restrict__ rec,
+                                                      float* __restrict__ fe, float* __restrict__ fp,
+                                                      const float* __restrict__ rho, const float* __restrict__ aux,
+                                                      const Collider* __restrict__ cols, float dt_override,
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                      uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SceneDev& S = scenes[pscene[i]];
+  const float dt = dt_override > 0.0f ? dt_override : S.dt;
+  const float4 r0 = rec[4 * size_t(i)];
+  const float4 r1 = rec[4 * size_t(i) + 1];
+  const float4 r2 = rec[4 * size_t(i) + 2];
+  const float4 r3 = rec[4 * size_t(i) + 3];
+  float FE[9], FP[9], vpn[3], dvp[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { FE[e] = fe[size_t(e) * n + i]; FP[e] = fp[size_t(e) * n + i]; }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) vpn[k] = aux[size_t(k) * n + i];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dvp[k] = aux[size_t(3 + k) * n + i];
+  float FEo[9], FPo[9];
+  deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo);
+  float vx = r1.w * S.alpha + vpn[0], vy = r2.w * S.alpha + vpn[1], vz = r3.w * S.alpha + vpn[2];
+  collide(r0.x, r0.y, r0.z, vx, vy, vz, cols + S.col_off, S.ncol);
+  const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
+  float A[9];
+  force_matrix<float>(FEo, FPo, r0.w, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
+  rec[4 * size_t(i)] = make_float4(x, y, z, r0.w);
+  rec[4 * size_t(i) + 1] = make_float4(A[0], A[3], A[6], vx);
+  rec[4 * size_t(i) + 2] = make_float4(A[1], A[4], A[7], vy);
+  rec[4 * size_t(i) + 3] = make_float4(A[2], A[5], A[8], vz);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { fe[size_t(e) * n + i] = FEo[e]; fp[size_t(e) * n + i] = FPo[e]; }
+  keys[i] = cell_key(S, x, y, z);
+  vals[i] = i;
+}
+
+// ---------------------------------------------------------------------------------------
+// Init: rest density rho_p0 = sum_i m_i w_ip / h^3 (pVolumeCompute.glsl:116-166), gathered
+// per particle (the reference scatters with a 64-way per-particle atomic).
+__global__ void __launch_bounds__(256) k_particle_volume(const SceneDev* __restrict__ scenes,
+                                                         const uint16_t* __restrict__ pscene,
+                                                         const float4* __restrict__ rec,
+                                                         const float4* __restrict__ g_mmv,
+                                                         float* __restrict__ rho, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SceneDev& S = scenes[pscene[i]];
+  const float4 r0 = rec[4 * size_t(i)];
+  const float ux = (r0.x - S.origin[0]) / S.h, uy = (r0.y - S.origin[1]) / S.h, uz = (r0.z - S.origin[2]) / S.h;
+  const bool in = ux > -3.0f && uy > -3.0f && uz > -3.0f && ux < float(S.D[0] + 1) && uy < float(S.D[1] + 1) &&
+                  uz < float(S.D[2] + 1);
+  float acc = 0.f;
+  if (in) {
+    const int bx = int(ux), by = int(uy), bz = int(uz);
+    float wx[4], wy[4], wz[4], q[4];
+    axis_weights(ux, bx, wx, q);
+    axis_weights(uy, by, wy, q);
+    axis_weights(uz, bz, wz, q);
+    const float cell = S.h * S.h * S.h;
+    for (int oz = 0; oz < 4; ++oz) {
+      const int iz = bz + oz - 1;
+      if (iz < 0 || iz >= S.D[2]) continue;
+      for (int oy = 0; oy < 4; ++oy) {
+        const int iy = by + oy - 1;
+        if (iy < 0 || iy >= S.D[1]) continue;
+        for (int ox = 0; ox < 4; ++ox) {
+          const int ix = bx + ox - 1;
+          if (ix < 0 || ix >= S.D[0]) continue;
+          const float mi = g_mmv[S.node_off + ix + int64_t(S.D[0]) * (iy + int64_t(S.D[1]) * iz)].w;
+          acc += mi * (wx[ox] * wy[oy] * wz[oz]) / cell;
+        }
+      }
+    }
+  }
+  rho[i] = acc;
+}
+
+// Force matrices of the current state (after init or host writes).
+__global__ void __launch_bounds__(256) k_force_prepass(const SceneDev* __restrict__ scenes,
+                                                       const uint16_t* __restrict__ pscene, float4* __restrict__ rec,
+                                                       const float* __restrict__ fe, const float* __restrict__ fp,
+                                                       const float* __restrict__ rho, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SceneDev& S = scenes[pscene[i]];
+  float FE[9], FP[9], A[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { FE[e] = fe[size_t(e) * n + i]; FP[e] = fp[size_t(e) * n + i]; }
+  const float m = rec[4 * size_t(i)].w;
+  force_matrix<float>(FE, FP, m, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
+  float4 r1 = rec[4 * size_t(i) + 1], r2 = rec[4 * size_t(i) + 2], r3 = rec[4 * size_t(i) + 3];
+  r1.x = A[0]; r1.y = A[3]; r1.z = A[6];
+  r2.x = A[1]; r2.y = A[4]; r2.z = A[7];
+  r3.x = A[2]; r3.y = A[5]; r3.z = A[8];
+  rec[4 * size_t(i) + 1] = r1;
+  rec[4 * size_t(i) + 2] = r2;
+  rec[4 * size_t(i) + 3] = r3;
+}
+
+// Zero the force matrices (init: the grid-mass pass needs m only).
+__global__ void __launch_bounds__(256) k_zero_force(float4* __restrict__ rec, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    float4 v = rec[4 * size_t(i) + r];
+    v.x = 0.f; v.y = 0.f; v.z = 0.f;
+    rec[4 * size_t(i) + r] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Host transfer helpers: original order <-> current slot order through pid.
+__global__ void __launch_bounds__(256) k_gather_out(const float4* __restrict__ rec, const float* __restrict__ fe,
+                                                    const float* __restrict__ fp, const float* __restrict__ rho,
+                                                    const uint32_t* __restrict__ pid, int64_t p_off, int64_t cnt,
+                                                    uint32_t n, float* __restrict__ ox, float* __restrict__ ov,
+                                                    float* __restrict__ ofe, float* __restrict__ ofp,
+                                                    float* __restrict__ orho, float* __restrict__ om) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  const uint32_t i = uint32_t(p_off + k);
+  const int64_t o = int64_t(pid[i]) - p_off;
+  const float4 r0 = rec[4 * size_t(i)];
+  ox[3 * o] = r0.x; ox[3 * o + 1] = r0.y; ox[3 * o + 2] = r0.z;
+  om[o] = r0.w;
+  ov[3 * o] = rec[4 * size_t(i) + 1].w; ov[3 * o + 1] = rec[4 * size_t(i) + 2].w; ov[3 * o + 2] = rec[4 * size_t(i) + 3].w;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { ofe[9 * o + e] = fe[size_t(e) * n + i]; ofp[9 * o + e] = fp[size_t(e) * n + i]; }
+  orho[o] = rho[i];
+}
+
+// mask bits: 1 x, 2 v, 4 fe, 8 fp, 16 rho, 32 mass
+__global__ void __launch_bounds__(256) k_scatter_in(float4* __restrict__ rec, float* __restrict__ fe,
+                                                    float* __restrict__ fp, float* __restrict__ rho,
+                                                    const uint32_t* __restrict__ pid, int64_t p_off, int64_t cnt,
+                                                    uint32_t n, const float* __restrict__ ix,
+                                                    const float* __restrict__ iv, const float* __restrict__ ife,
+                                                    const float* __restrict__ ifp, const float* __restrict__ irho,
+                                                    const float* __restrict__ im, int mask) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  const uint32_t i = uint32_t(p_off + k);
+  const int64_t o = int64_t(pid[i]) - p_off;
+  float4 r0 = rec[4 * size_t(i)];
+  if (mask & 1) { r0.x = ix[3 * o]; r0.y = ix[3 * o + 1]; r0.z = ix[3 * o + 2]; }
+  if (mask & 32) r0.w = im[o];
+  rec[4 * size_t(i)] = r0;
+  if (mask & 2) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float4 r = rec[4 * size_t(i) + 1 + c];
+      r.w = iv[3 * o + c];
+      rec[4 * size_t(i) + 1 + c] = r;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    if (mask & 4) fe[size_t(e) * n + i] = ife[9 * o + e];
+    if (mask & 8) fp[size_t(e) * n + i] = ifp[9 * o + e];
+  }
+  if (mask & 16) rho[i] = irho[o];
+}
+
+// Non-finite scan of the particle state.
+__global__ void __launch_bounds__(256) k_check_finite(const float4* __restrict__ rec, const float* __restrict__ fe,
+                                                      const float* __restrict__ fp, uint32_t n,
+                                                      int* __restrict__ bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool ok = true;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float4 v = rec[4 * size_t(i) + r];
+    ok &= isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w);
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) ok &= isfinite(fe[size_t(e) * n + i]) && isfinite(fp[size_t(e) * n + i]);
+  if (!ok) atomicOr(bad, 1);
+}
+
+// Fill kernel (used instead of hipMemsetAsync so the step stays graph-capturable).
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint32_t v, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace snow

@@ -1,0 +1,24 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
+
+
+def _ensure_oracle():
+    from oracle import pyoracle
+    if not os.path.exists(pyoracle.LIB_PATH):
+        pyoracle.build()
+    return pyoracle
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    return _ensure_oracle()
