@@ -284,7 +284,7 @@ int run_sort(snow_ctx* c, float dt_rigid) {
 
 int run_p2g(snow_ctx* c, int mass_only) {
   StageTimer tm(c, ST_P2G);
-  k_p2g_tile<<<c->total_ctiles, 512, 0, c->stream>>>(c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1],
+  k_p2g_tile<<<c->total_ctiles, 64 * P2G_WAVES, 0, c->stream>>>(c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1],
                                                       c->d_cstart, c->d_cend, c->d_rec[c->cur], c->d_tpart,
                                                       c->d_tile_active, c->N, mass_only ? 0 : 1);
   LAUNCHCHK(c);
@@ -295,15 +295,15 @@ int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
   StageTimer tm(c, ST_GRID);
   unsigned long long* cnt = c->d_counts;
   if (keep && update)
-    k_grid<true, true><<<c->total_ntiles, 512, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
+    k_grid<true, true><<<c->total_ntiles, 64, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
                                                                c->d_tpart, c->d_g_mmv, c->d_g_f, c->d_g_vs,
                                                                c->d_g_flip, c->d_cols, dt, cnt);
   else if (update)
-    k_grid<false, true><<<c->total_ntiles, 512, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
+    k_grid<false, true><<<c->total_ntiles, 64, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
                                                                 c->d_tpart, c->d_g_mmv, c->d_g_f, c->d_g_vs,
                                                                 c->d_g_flip, c->d_cols, dt, cnt);
   else
-    k_grid<true, false><<<c->total_ntiles, 512, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
+    k_grid<true, false><<<c->total_ntiles, 64, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
                                                                 c->d_tpart, c->d_g_mmv, c->d_g_f, c->d_g_vs,
                                                                 c->d_g_flip, c->d_cols, dt, cnt);
   LAUNCHCHK(c);

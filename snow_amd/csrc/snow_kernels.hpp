@@ -10,19 +10,20 @@
 //    g_vs     : float4 (v*, m)           - gForceCompute output gvn (+ mass)
 //    g_flip   : float4 (v* - alpha p/m)  - FLIP/PIC blend term used by G2P
 //    g_mmv,g_f: float4 (p, m), (f, 0)    - gv / gf of the reference (stage API, debug)
-//  cell tiles: 8x8x8 cells per tile over the shifted cell range c = base + 2, base in [-2, D]
-//    (a cubic stencil of base cell b touches nodes b-1..b+2); a tile's 11^3-node halo starts
-//    at node 8k-3; tpart[tile][q][11^3] holds the tile's partial sums of the 7 P2G quantities.
+//  cell tiles: 4x4x4 cells per tile over the shifted cell range c = base + 2, base in [-2, D]
+//    (a cubic stencil of base cell b touches nodes b-1..b+2); a tile's 7^3-node halo starts
+//    at node 4k-3; tpart[tile][q][7^3] holds the tile's partial sums of the 7 P2G quantities.
 #pragma once
 #include "snow_device.hpp"
 
 namespace snow {
 
-constexpr int TILE = 8;              // cells per tile edge
-constexpr int TILE_CELLS = 512;      // 8^3
-constexpr int HALO = 11;             // TILE + 3 nodes per edge
-constexpr int HALO_NODES = 1331;     // 11^3
+constexpr int TILE = 4;              // cells per tile edge
+constexpr int TILE_CELLS = 64;       // 4^3
+constexpr int HALO = 7;              // TILE + 3 nodes per edge
+constexpr int HALO_NODES = 343;      // 7^3
 constexpr int NQ = 7;                // m, p.xyz, f.xyz
+constexpr int P2G_WAVES = 8;         // waves per P2G workgroup
 
 struct SceneDev {
   int D[3];
@@ -46,7 +47,7 @@ struct SceneDev {
   float pad1[2];
 };
 
-// Global cell key of a position: (tile) * 512 + (cell within tile); particles whose base
+// Global cell key of a position: (tile) * 64 + (cell within tile); particles whose base
 // cell lies outside [-2, D] (no in-grid stencil node) go to the scene's dead tile.
 __device__ __forceinline__ uint32_t cell_key(const SceneDev& S, float x, float y, float z) {
   const float ux = (x - S.origin[0]) / S.h;
@@ -56,8 +57,8 @@ __device__ __forceinline__ uint32_t cell_key(const SceneDev& S, float x, float y
                   uy < float(S.D[1] + 1) && uz < float(S.D[2] + 1);
   if (!in) return S.dead_tile * TILE_CELLS;
   const int cx = int(ux) + 2, cy = int(uy) + 2, cz = int(uz) + 2;
-  const uint32_t tile = S.ct_off + uint32_t((cx >> 3) + S.ct[0] * ((cy >> 3) + S.ct[1] * (cz >> 3)));
-  return tile * TILE_CELLS + uint32_t((cx & 7) + 8 * (cy & 7) + 64 * (cz & 7));
+  const uint32_t tile = S.ct_off + uint32_t((cx >> 2) + S.ct[0] * ((cy >> 2) + S.ct[1] * (cz >> 2)));
+  return tile * TILE_CELLS + uint32_t((cx & 3) + 4 * (cy & 3) + 16 * (cz & 3));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -101,24 +102,53 @@ __global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------------------
-// P2G on one 8^3 cell tile (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84).
-// Wave w handles one cell per phase; lane l is the stencil offset o = (l&3, (l>>2)&3, l>>4)
-// (getIJK order, neighborIndex.include.glsl:6-12), so the 64 lanes form the particle's
-// 4x4x4 stencil and every particle of the cell is a wave-uniform (scalar) load. The 64
-// phases walk the colour classes (cx&3, cy&3, cz&3): cells of one colour are 4 apart, so
-// their stencils never overlap and the 8 waves update the LDS tile without atomics. Each
-// node therefore sums its contributions in a fixed order (cell colour, then particle order),
-// compensated (TwoSum) across cells: the result is bit-reproducible.
-__global__ void __launch_bounds__(512, 1) k_p2g_tile(
+// P2G on one 4^3 cell tile (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84).
+// A wave takes one cell at a time; lane l is the stencil offset o = (l&3, (l>>2)&3, l>>4)
+// (getIJK order, neighborIndex.include.glsl:6-12), so the 64 lanes cover the particle's 4x4x4
+// stencil and each particle of the cell is a wave-uniform scalar load. The cell's particles are
+// summed in registers in their (stable, sorted) order, then added to the tile's LDS accumulator
+// as 64-bit fixed point: integer addition is associative, so the tile sums are exact and do not
+// depend on the order in which the waves commit. The fixed-point scale is a power of two per
+// tile and quantity, 2^61 / (count * max |per-particle term|), from a vectorised pre-pass over
+// the tile's particles, so no sum can overflow and the resolution is ~2^-40 of the largest term.
+struct P2GBound { float m, p, f; };
+
+__device__ __forceinline__ P2GBound particle_bound(const float4& r0, const float4& r1, const float4& r2,
+                                                   const float4& r3, int skip_heavy) {
+  P2GBound b;
+  const float m = r0.w;
+  const bool skip = skip_heavy && fabsf(m) > 100.0f;
+  b.m = skip ? 0.f : fabsf(m);
+  b.p = skip ? 0.f : fmaxf(fabsf(r1.w * m), fmaxf(fabsf(r2.w * m), fabsf(r3.w * m)));
+  const float f0 = fabsf(r1.x) + fabsf(r1.y) + fabsf(r1.z);
+  const float f1 = fabsf(r2.x) + fabsf(r2.y) + fabsf(r2.z);
+  const float f2 = fabsf(r3.x) + fabsf(r3.y) + fabsf(r3.z);
+  b.f = skip ? 0.f : fmaxf(f0, fmaxf(f1, f2));
+  return b;
+}
+
+// exponent e with count*maxval*2^e <= 2^61 (clamped to the float range)
+__device__ __forceinline__ int fixed_exponent(float maxval, uint32_t count) {
+  if (!(maxval > 0.0f)) return 0;
+  const float bound = maxval * float(count);
+  int ex = 0;
+  frexpf(bound, &ex);  // bound < 2^ex
+  const int e = 61 - ex;
+  return e > 126 ? 126 : (e < -126 ? -126 : e);
+}
+
+__global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
     const float4* __restrict__ rec, float* __restrict__ tpart, int* __restrict__ tile_active,
     uint32_t n, int skip_heavy) {
+  constexpr int NT = 64 * P2G_WAVES;
   __shared__ uint32_t s_cs[TILE_CELLS];
   __shared__ uint32_t s_ce[TILE_CELLS];
-  __shared__ float s_hi[NQ * HALO_NODES];
-  __shared__ float s_lo[NQ * HALO_NODES];
+  __shared__ unsigned long long s_acc[NQ * HALO_NODES];
+  __shared__ uint32_t s_rng[2];
+  __shared__ uint32_t s_bnd[3];
 
   const uint32_t tile = blockIdx.x;
   const int sid = tile_scene[tile];
@@ -128,19 +158,44 @@ __global__ void __launch_bounds__(512, 1) k_p2g_tile(
     if (t == 0) tile_active[tile] = 0;
     return;
   }
-  {
+  if (t == 0) { s_rng[0] = 0xffffffffu; s_rng[1] = 0u; s_bnd[0] = s_bnd[1] = s_bnd[2] = 0u; }
+  __syncthreads();
+  bool valid = false;
+  if (t < TILE_CELLS) {
     const uint32_t gc = tile * TILE_CELLS + t;
     const uint32_t st = cstart[gc];
-    const bool valid = st < n && keys[st] == gc;
+    valid = st < n && keys[st] == gc;
+    const uint32_t en = valid ? cend[gc] : 0u;
     s_cs[t] = valid ? st : 0u;
-    s_ce[t] = valid ? cend[gc] : 0u;
-    if (!__syncthreads_or(valid)) {
-      if (t == 0) tile_active[tile] = 0;
-      return;
-    }
+    s_ce[t] = en;
+    if (valid) { atomicMin(&s_rng[0], st); atomicMax(&s_rng[1], en); }
   }
-  for (int i = t; i < NQ * HALO_NODES; i += 512) { s_hi[i] = 0.0f; s_lo[i] = 0.0f; }
+  if (!__syncthreads_or(valid)) {
+    if (t == 0) tile_active[tile] = 0;
+    return;
+  }
+  // bound pre-pass over the tile's particles (contiguous in sorted order)
+  const uint32_t r_lo = s_rng[0], r_hi = s_rng[1];
+  {
+    float bm = 0.f, bp = 0.f, bf = 0.f;
+    for (uint32_t i = r_lo + t; i < r_hi; i += NT) {
+      const uint32_t p = vals[i];
+      const P2GBound b = particle_bound(rec[4 * size_t(p)], rec[4 * size_t(p) + 1], rec[4 * size_t(p) + 2],
+                                        rec[4 * size_t(p) + 3], skip_heavy);
+      bm = fmaxf(bm, b.m); bp = fmaxf(bp, b.p); bf = fmaxf(bf, b.f);
+    }
+    // non-negative floats order like their bit patterns
+    atomicMax(&s_bnd[0], __float_as_uint(bm));
+    atomicMax(&s_bnd[1], __float_as_uint(bp));
+    atomicMax(&s_bnd[2], __float_as_uint(bf));
+  }
+  for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0ull;
   __syncthreads();
+  const uint32_t count = r_hi - r_lo;
+  const int em = fixed_exponent(__uint_as_float(s_bnd[0]), count);
+  const int ep = fixed_exponent(__uint_as_float(s_bnd[1]), count);
+  const int ef = fixed_exponent(__uint_as_float(s_bnd[2]), count);
+  const float sm = ldexpf(1.0f, em), sp = ldexpf(1.0f, ep), sf = ldexpf(1.0f, ef);
 
   const uint32_t tl = tile - S.ct_off;
   const int tx = int(tl % uint32_t(S.ct[0]));
@@ -150,66 +205,66 @@ __global__ void __launch_bounds__(512, 1) k_p2g_tile(
   const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
   const float orgx = S.origin[0], orgy = S.origin[1], orgz = S.origin[2], h = S.h;
 
-  for (int phase = 0; phase < 64; ++phase) {
-    const int cx = (phase & 3) + 4 * (wave & 1);
-    const int cy = ((phase >> 2) & 3) + 4 * ((wave >> 1) & 1);
-    const int cz = (phase >> 4) + 4 * (wave >> 2);
-    const int cl = cx + 8 * cy + 64 * cz;
+  for (int cl = wave; cl < TILE_CELLS; cl += P2G_WAVES) {
     const uint32_t b = __builtin_amdgcn_readfirstlane(s_cs[cl]);
     const uint32_t e = __builtin_amdgcn_readfirstlane(s_ce[cl]);
-    if (b < e) {
-      // node of this lane: base + o - 1 with base = 8*t + c - 2
-      const float gx = float(TILE * tx + cx - 3 + ox);
-      const float gy = float(TILE * ty + cy - 3 + oy);
-      const float gz = float(TILE * tz + cz - 3 + oz);
-      float am = 0.f, apx = 0.f, apy = 0.f, apz = 0.f, afx = 0.f, afy = 0.f, afz = 0.f;
-      for (uint32_t j = b; j < e; ++j) {
-        const uint32_t p = vals[j];
-        const float4 r0 = rec[4 * size_t(p)];
-        const float4 r1 = rec[4 * size_t(p) + 1];
-        const float4 r2 = rec[4 * size_t(p) + 2];
-        const float4 r3 = rec[4 * size_t(p) + 3];
-        const float m = r0.w;
-        if (skip_heavy && fabsf(m) > 100.0f) continue;  // p2gTransfer.glsl:64-66 (not in gMassCompute)
-        const float ux = (r0.x - orgx) / h, uy = (r0.y - orgy) / h, uz = (r0.z - orgz) / h;
-        const float dx = gx - ux, dy = gy - uy, dz = gz - uz;
-        const float wx = bspline(dx), wy = bspline(dy), wz = bspline(dz);
-        const float qx = bspline_grad(-dx), qy = bspline_grad(-dy), qz = bspline_grad(-dz);
-        const float wyz = wy * wz;
-        const float w = wx * wyz;
-        const float g0 = qx * wyz, g1 = wx * qy * wz, g2 = wx * wy * qz;
-        am += m * w;
-        apx += (r1.w * m) * w;
-        apy += (r2.w * m) * w;
-        apz += (r3.w * m) * w;
-        afx -= r1.x * g0 + r1.y * g1 + r1.z * g2;
-        afy -= r2.x * g0 + r2.y * g1 + r2.z * g2;
-        afz -= r3.x * g0 + r3.y * g1 + r3.z * g2;
-      }
-      const int jn = (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
-      two_sum_acc(s_hi[0 * HALO_NODES + jn], s_lo[0 * HALO_NODES + jn], am);
-      two_sum_acc(s_hi[1 * HALO_NODES + jn], s_lo[1 * HALO_NODES + jn], apx);
-      two_sum_acc(s_hi[2 * HALO_NODES + jn], s_lo[2 * HALO_NODES + jn], apy);
-      two_sum_acc(s_hi[3 * HALO_NODES + jn], s_lo[3 * HALO_NODES + jn], apz);
-      two_sum_acc(s_hi[4 * HALO_NODES + jn], s_lo[4 * HALO_NODES + jn], afx);
-      two_sum_acc(s_hi[5 * HALO_NODES + jn], s_lo[5 * HALO_NODES + jn], afy);
-      two_sum_acc(s_hi[6 * HALO_NODES + jn], s_lo[6 * HALO_NODES + jn], afz);
+    if (b >= e) continue;
+    const int cx = cl & 3, cy = (cl >> 2) & 3, cz = cl >> 4;
+    // node of this lane: base + o - 1 with base = 4*tile + c - 2
+    const float gx = float(TILE * tx + cx - 3 + ox);
+    const float gy = float(TILE * ty + cy - 3 + oy);
+    const float gz = float(TILE * tz + cz - 3 + oz);
+    float am = 0.f, apx = 0.f, apy = 0.f, apz = 0.f, afx = 0.f, afy = 0.f, afz = 0.f;
+    for (uint32_t j = b; j < e; ++j) {
+      const uint32_t p = vals[j];
+      const float4 r0 = rec[4 * size_t(p)];
+      const float4 r1 = rec[4 * size_t(p) + 1];
+      const float4 r2 = rec[4 * size_t(p) + 2];
+      const float4 r3 = rec[4 * size_t(p) + 3];
+      const float m = r0.w;
+      if (skip_heavy && fabsf(m) > 100.0f) continue;  // p2gTransfer.glsl:64-66 (not in gMassCompute)
+      const float ux = (r0.x - orgx) / h, uy = (r0.y - orgy) / h, uz = (r0.z - orgz) / h;
+      const float dx = gx - ux, dy = gy - uy, dz = gz - uz;
+      const float wx = bspline(dx), wy = bspline(dy), wz = bspline(dz);
+      const float qx = bspline_grad(-dx), qy = bspline_grad(-dy), qz = bspline_grad(-dz);
+      const float wyz = wy * wz;
+      const float w = wx * wyz;
+      const float g0 = qx * wyz, g1 = wx * qy * wz, g2 = wx * wy * qz;
+      am += m * w;
+      apx += (r1.w * m) * w;
+      apy += (r2.w * m) * w;
+      apz += (r3.w * m) * w;
+      afx -= r1.x * g0 + r1.y * g1 + r1.z * g2;
+      afy -= r2.x * g0 + r2.y * g1 + r2.z * g2;
+      afz -= r3.x * g0 + r3.y * g1 + r3.z * g2;
     }
-    __syncthreads();
+    const int jn = (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
+    atomicAdd(&s_acc[0 * HALO_NODES + jn], (unsigned long long)llrintf(am * sm));
+    atomicAdd(&s_acc[1 * HALO_NODES + jn], (unsigned long long)llrintf(apx * sp));
+    atomicAdd(&s_acc[2 * HALO_NODES + jn], (unsigned long long)llrintf(apy * sp));
+    atomicAdd(&s_acc[3 * HALO_NODES + jn], (unsigned long long)llrintf(apz * sp));
+    atomicAdd(&s_acc[4 * HALO_NODES + jn], (unsigned long long)llrintf(afx * sf));
+    atomicAdd(&s_acc[5 * HALO_NODES + jn], (unsigned long long)llrintf(afy * sf));
+    atomicAdd(&s_acc[6 * HALO_NODES + jn], (unsigned long long)llrintf(afz * sf));
   }
+  __syncthreads();
   float* out = tpart + size_t(tile) * (NQ * HALO_NODES);
-  for (int i = t; i < NQ * HALO_NODES; i += 512) out[i] = s_hi[i] + s_lo[i];
+  for (int i = t; i < NQ * HALO_NODES; i += NT) {
+    const int q = i / HALO_NODES;
+    const int e = q == 0 ? em : (q < 4 ? ep : ef);
+    out[i] = ldexpf(float((long long)s_acc[i]), -e);
+  }
   if (t == 0) tile_active[tile] = 1;
 }
 
 // ---------------------------------------------------------------------------------------
 // Merge of the tile partials into node values + grid update (gForceCompute.glsl:57-103).
-// One block per 8^3 node tile K; node n = 8K + r receives from cell tiles K (halo index
-// r + 3) and, when r >= 5 on an axis, K+1 (halo index r - 5): at most 8 tiles, summed in a
+// One wave per 4^3 node tile K; node n = 4K + r receives from cell tiles K (halo index
+// r + 3) and, when r >= 1 on an axis, K+1 (halo index r - 1): at most 8 tiles, summed in a
 // fixed order with TwoSum. Node tiles none of whose contributing cell tiles is active are
 // skipped: no particle's stencil reaches them.
 template <bool KEEP, bool UPDATE>
-__global__ void __launch_bounds__(512) k_grid(const SceneDev* __restrict__ scenes,
+__global__ void __launch_bounds__(64) k_grid(const SceneDev* __restrict__ scenes,
                                               const uint16_t* __restrict__ ntile_scene,
                                               const int* __restrict__ tile_active,
                                               const float* __restrict__ tpart,
@@ -225,7 +280,7 @@ __global__ void __launch_bounds__(512) k_grid(const SceneDev* __restrict__ scene
   const int Ky = int((ntl / uint32_t(S.nt[0])) % uint32_t(S.nt[1]));
   const int Kz = int(ntl / uint32_t(S.nt[0] * S.nt[1]));
   const int t = threadIdx.x;
-  const int rx = t & 7, ry = (t >> 3) & 7, rz = t >> 6;
+  const int rx = t & 3, ry = (t >> 2) & 3, rz = t >> 4;
   const int nx = TILE * Kx + rx, ny = TILE * Ky + ry, nz = TILE * Kz + rz;
   const bool inside = nx < S.D[0] && ny < S.D[1] && nz < S.D[2];
 
@@ -237,11 +292,11 @@ __global__ void __launch_bounds__(512) k_grid(const SceneDev* __restrict__ scene
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
       const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
-      if ((ax && rx < 5) || (ay && ry < 5) || (az && rz < 5)) continue;
+      if ((ax && rx < 1) || (ay && ry < 1) || (az && rz < 1)) continue;
       const uint32_t ctile = S.ct_off + uint32_t((Kx + ax) + S.ct[0] * ((Ky + ay) + S.ct[1] * (Kz + az)));
       if (!tile_active[ctile]) continue;
       any = true;
-      const int j = (ax ? rx - 5 : rx + 3) + HALO * (ay ? ry - 5 : ry + 3) + HALO * HALO * (az ? rz - 5 : rz + 3);
+      const int j = (ax ? rx - 1 : rx + 3) + HALO * (ay ? ry - 1 : ry + 3) + HALO * HALO * (az ? rz - 1 : rz + 3);
       const float* src = tpart + size_t(ctile) * (NQ * HALO_NODES) + j;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) two_sum_acc(hi[q], lo[q], src[q * HALO_NODES]);
@@ -347,10 +402,10 @@ __global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict_
   const int nx = int(l % S.D[0]), ny = int((l / S.D[0]) % S.D[1]), nz = int(l / (int64_t(S.D[0]) * S.D[1]));
   bool live = dense_valid != 0;
   if (!live) {
-    const int Kx = nx >> 3, Ky = ny >> 3, Kz = nz >> 3, rx = nx & 7, ry = ny & 7, rz = nz & 7;
+    const int Kx = nx >> 2, Ky = ny >> 2, Kz = nz >> 2, rx = nx & 3, ry = ny & 3, rz = nz & 3;
     for (int a = 0; a < 8 && !live; ++a) {
       const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
-      if ((ax && rx < 5) || (ay && ry < 5) || (az && rz < 5)) continue;
+      if ((ax && rx < 1) || (ay && ry < 1) || (az && rz < 1)) continue;
       const uint32_t ctile = S.ct_off + uint32_t((Kx + ax) + S.ct[0] * ((Ky + ay) + S.ct[1] * (Kz + az)));
       live = tile_active[ctile] != 0;
     }
