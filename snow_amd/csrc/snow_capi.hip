@@ -85,7 +85,10 @@ struct snow_ctx {
   size_t sort_tmp_bytes = 0;
   uint32_t *d_cstart = nullptr, *d_cend = nullptr;
   float* d_tpart = nullptr;
-  int* d_tile_active = nullptr;
+  StepCtl* d_ctl = nullptr;
+  uint32_t* d_tile_stamp = nullptr;
+  uint32_t* d_active_list = nullptr;
+  int num_cu = 256;
   float4 *d_g_mmv = nullptr, *d_g_f = nullptr, *d_g_vs = nullptr, *d_g_flip = nullptr;
   Collider* d_cols = nullptr;
   unsigned long long* d_counts = nullptr;  // [nscenes] active nodes, then [1] fp64 count
@@ -266,8 +269,11 @@ int run_force_prepass(snow_ctx* c) {
   return 0;
 }
 
-// Sort + cell bounds (+ rigid advance when dt_rigid >= 0).
+// Step start (stamp, counters, rigid advance when dt_rigid >= 0) + sort + cell bounds.
 int run_sort(snow_ctx* c, float dt_rigid) {
+  k_step_begin<<<1, 256, 0, c->stream>>>(c->d_ctl, c->d_counts, c->nscenes + 1, c->d_scenes, c->nscenes, c->d_cols,
+                                         dt_rigid);
+  LAUNCHCHK(c);
   int rc = run_keys(c);
   if (rc) return rc;
   StageTimer tm(c, ST_SORT);
@@ -276,17 +282,21 @@ int run_sort(snow_ctx* c, float dt_rigid) {
     HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->N,
                                c->key_bits, c->stream));
   }
-  k_cell_bounds<<<std::max(1u, grid_for(c->N, 256)), 256, 0, c->stream>>>(
-      c->d_keys[1], c->N, c->d_cstart, c->d_cend, c->d_scenes, c->nscenes, c->d_cols, dt_rigid);
-  LAUNCHCHK(c);
+  if (c->N) {
+    k_cell_bounds<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_keys[1], c->N, c->d_cstart, c->d_cend,
+                                                               c->d_scenes, c->d_tile_scene, c->d_ctl,
+                                                               c->d_tile_stamp, c->d_active_list);
+    LAUNCHCHK(c);
+  }
   return tm.done();
 }
 
 int run_p2g(snow_ctx* c, int mass_only) {
   StageTimer tm(c, ST_P2G);
-  k_p2g_tile<<<c->total_ctiles, 64 * P2G_WAVES, 0, c->stream>>>(c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1],
-                                                      c->d_cstart, c->d_cend, c->d_rec[c->cur], c->d_tpart,
-                                                      c->d_tile_active, c->N, mass_only ? 0 : 1);
+  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu) * 2));
+  k_p2g_tile<<<blocks, 64 * P2G_WAVES, 0, c->stream>>>(c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1],
+                                                       c->d_cstart, c->d_cend, c->d_rec[c->cur], c->d_tpart,
+                                                       c->d_ctl, c->d_active_list, c->N, mass_only ? 0 : 1);
   LAUNCHCHK(c);
   return tm.done();
 }
@@ -294,27 +304,19 @@ int run_p2g(snow_ctx* c, int mass_only) {
 int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
   StageTimer tm(c, ST_GRID);
   unsigned long long* cnt = c->d_counts;
+  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles * 8u, unsigned(c->num_cu) * 32));
+#define GRID_ARGS                                                                                          \
+  c->d_scenes, c->d_tile_scene, c->d_ctl, c->d_active_list, c->d_tile_stamp, c->d_tpart, c->d_g_mmv, c->d_g_f, \
+      c->d_g_vs, c->d_g_flip, c->d_cols, dt, cnt
   if (keep && update)
-    k_grid<true, true><<<c->total_ntiles, 64, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
-                                                               c->d_tpart, c->d_g_mmv, c->d_g_f, c->d_g_vs,
-                                                               c->d_g_flip, c->d_cols, dt, cnt);
+    k_grid<true, true><<<blocks, 64, 0, c->stream>>>(GRID_ARGS);
   else if (update)
-    k_grid<false, true><<<c->total_ntiles, 64, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
-                                                                c->d_tpart, c->d_g_mmv, c->d_g_f, c->d_g_vs,
-                                                                c->d_g_flip, c->d_cols, dt, cnt);
+    k_grid<false, true><<<blocks, 64, 0, c->stream>>>(GRID_ARGS);
   else
-    k_grid<true, false><<<c->total_ntiles, 64, 0, c->stream>>>(c->d_scenes, c->d_ntile_scene, c->d_tile_active,
-                                                                c->d_tpart, c->d_g_mmv, c->d_g_f, c->d_g_vs,
-                                                                c->d_g_flip, c->d_cols, dt, cnt);
+    k_grid<true, false><<<blocks, 64, 0, c->stream>>>(GRID_ARGS);
+#undef GRID_ARGS
   LAUNCHCHK(c);
   return tm.done();
-}
-
-int zero_counts(snow_ctx* c) {
-  k_fill_u32<<<grid_for(2 * (c->nscenes + 1), 256), 256, 0, c->stream>>>(
-      reinterpret_cast<uint32_t*>(c->d_counts), 0u, int64_t(2) * (c->nscenes + 1));
-  LAUNCHCHK(c);
-  return 0;
 }
 
 int run_g2p_advance(snow_ctx* c, float dt) {
@@ -348,7 +350,6 @@ int step_impl(snow_ctx* c, float dt) {
   if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before stepping");
   int rc;
   if ((rc = run_force_prepass(c))) return rc;
-  if ((rc = zero_counts(c))) return rc;
   if ((rc = run_sort(c, dt))) return rc;
   if ((rc = run_p2g(c, 0))) return rc;
   const bool keep = (c->flags & SNOW_FLAG_KEEP_GRID) != 0;
@@ -366,7 +367,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_sort_tmp,
-                  c->d_cstart, c->d_cend, c->d_tpart, c->d_tile_active, c->d_g_mmv, c->d_g_f, c->d_g_vs,
+                  c->d_cstart, c->d_cend, c->d_tpart, c->d_ctl, c->d_tile_stamp, c->d_active_list, c->d_g_mmv, c->d_g_f, c->d_g_vs,
                   c->d_g_flip, c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -430,6 +431,11 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   c->nscenes = num_scenes;
   c->N = uint32_t(ntot);
   (void)hipGetDevice(&c->device);
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
+      c->num_cu = ncu;
+  }
 
   // scene table, tile index space
   uint64_t ct_off = 0, nt_off = 0;
@@ -461,6 +467,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     S.node_off = node_off;
     S.p_off = p_off;
     S.p_cnt = d.num_particles;
+    S.inv_h = 1.0f / d.grid.spacing;
+    for (int k = 0; k < 3; ++k) S.uo[k] = -d.grid.origin[k] / d.grid.spacing;
     S.mu0 = d.material.young / (2.0f * (1.0f + d.material.poisson));
     S.lambda0 = d.material.young * d.material.poisson / ((1.0f + d.material.poisson) * (1.0f - 2.0f * d.material.poisson));
     ct_off += nct + 1;
@@ -528,7 +536,9 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_cstart, ncells));
   CREATECHK(dalloc(&c->d_cend, ncells));
   CREATECHK(dalloc(&c->d_tpart, size_t(c->total_ctiles) * NQ * HALO_NODES));
-  CREATECHK(dalloc(&c->d_tile_active, c->total_ctiles));
+  CREATECHK(dalloc(&c->d_ctl, 1));
+  CREATECHK(dalloc(&c->d_tile_stamp, c->total_ctiles));
+  CREATECHK(dalloc(&c->d_active_list, c->total_ctiles));
   CREATECHK(dalloc(&c->d_g_mmv, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_g_f, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_g_vs, size_t(c->total_nodes)));
@@ -587,7 +597,9 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   }
   CREATECHK(hipMemset(c->d_cstart, 0xff, sizeof(uint32_t) * ncells));
   CREATECHK(hipMemset(c->d_cend, 0, sizeof(uint32_t) * ncells));
-  CREATECHK(hipMemset(c->d_tile_active, 0, sizeof(int) * c->total_ctiles));
+  CREATECHK(hipMemset(c->d_ctl, 0, sizeof(StepCtl)));
+  CREATECHK(hipMemset(c->d_tile_stamp, 0, sizeof(uint32_t) * c->total_ctiles));
+  CREATECHK(hipMemset(c->d_active_list, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_g_mmv, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_f, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_vs, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
@@ -625,7 +637,6 @@ int snow_stage_grid_mass(snow_ctx* c) {
     LAUNCHCHK(c);
   }
   c->force_valid = false;
-  if ((rc = zero_counts(c))) return rc;
   if ((rc = run_sort(c, -1.0f))) return rc;
   if ((rc = run_p2g(c, 1))) return rc;
   if ((rc = run_grid(c, true, false, 0.0f))) return rc;
@@ -684,8 +695,7 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
 int snow_stage_rigid_advance(snow_ctx* c, double dt) {
   if (!c) return SNOW_E_ARG;
   StageTimer tm(c, ST_RIGID);
-  k_cell_bounds<<<1, 256, 0, c->stream>>>(c->d_keys[1], 0u, c->d_cstart, c->d_cend, c->d_scenes, c->nscenes,
-                                          c->d_cols, float(dt));
+  k_rigid_advance<<<1, 256, 0, c->stream>>>(c->d_scenes, c->nscenes, c->d_cols, float(dt));
   LAUNCHCHK(c);
   return tm.done();
 }
@@ -709,7 +719,6 @@ int snow_stage_p2g(snow_ctx* c) {
   if (!c) return SNOW_E_ARG;
   int rc;
   if ((rc = run_force_prepass(c))) return rc;
-  if ((rc = zero_counts(c))) return rc;
   if ((rc = run_sort(c, -1.0f))) return rc;
   if ((rc = run_p2g(c, 0))) return rc;
   if ((rc = run_grid(c, true, false, 0.0f))) return rc;
@@ -830,7 +839,8 @@ int snow_read_grid(snow_ctx* c, int32_t s, float* m, float* mv, float* f, float*
   float *om = st, *omv = om + nn, *of = omv + 3 * nn, *ovs = of + 3 * nn;
   // With keep off, the mass lives in g_vs.w: export it from there.
   const float4* mm_src = c->keep_valid ? c->d_g_mmv : c->d_g_vs;
-  k_grid_export<<<grid_for(nn, 256), 256, 0, c->stream>>>(c->d_scenes, s, c->d_tile_active, c->grid_state == 2,
+  k_grid_export<<<grid_for(nn, 256), 256, 0, c->stream>>>(c->d_scenes, s, c->d_tile_stamp, c->d_ctl,
+                                                          c->grid_state == 2,
                                                           mm_src, c->d_g_f, c->d_g_vs, om, omv, of, ovs);
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));

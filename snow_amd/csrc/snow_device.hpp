@@ -1,7 +1,7 @@
 // snow_device.hpp - device math of the snow MPM step for gfx950.
 //
 // Restates, for registers on CDNA4, the GLSL include files of the reference:
-//   cubic B-spline weights ........ shader/compute/interpolation/cubic.include.glsl:7-40
+//   (cubic B-spline weights, cubic.include.glsl:7-40, are in snow_kernels.hpp: spline_w/q)
 //   Lame parameters with hardening  shader/compute/lame/lame.include.glsl:69-85
 //   3x3 SVD / polar decomposition .. shader/compute/svd/SVD.include.glsl:24-385
 //                                   (McAdams et al. 2011, UW-Madison TR1690: quaternion
@@ -29,6 +29,13 @@ template <class R> __device__ __forceinline__ R rnd(R x);
 template <> __device__ __forceinline__ float rnd<float>(float x) { return roundf(x); }
 template <> __device__ __forceinline__ double rnd<double>(double x) { return round(x); }
 template <class R> __device__ __forceinline__ R absv(R x) { return x < R(0) ? -x : x; }
+// reciprocal: v_rcp_f32 (1 ulp) for float, exact division for double
+template <class R> __device__ __forceinline__ R recip(R x);
+template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double recip<double>(double x) { return 1.0 / x; }
+// round(scale*x)/scale of the reference shaders (SVD.include.glsl:356, p2gTransfer.glsl:114,
+// particleAdvance.glsl:265/281), with the division by the constant done as a multiplication.
+template <class R> __device__ __forceinline__ R round_to(R x, R scale, R inv_scale) { return rnd(scale * x) * inv_scale; }
 
 // C = A * B (column-major 3x3)
 template <class R>
@@ -60,24 +67,6 @@ __device__ __forceinline__ R det3(const R* m) {
   // m[c*3+r]: expansion along the first column, as GLSL determinant()
   return m[0] * (m[4] * m[8] - m[7] * m[5]) - m[3] * (m[1] * m[8] - m[7] * m[2]) +
          m[6] * (m[1] * m[5] - m[4] * m[2]);
-}
-
-// ---- cubic B-spline (cubic.include.glsl:7-34), branch-free ----------------------------
-template <class R>
-__device__ __forceinline__ R bspline(R x) {
-  const R a = absv(x);
-  const R inner = R(0.5) * a * a * a - x * x + R(2) / R(3);
-  const R outer = -R(1) / R(6) * a * a * a + x * x - R(2) * a + R(4) / R(3);
-  return a < R(1) ? inner : (a < R(2) ? outer : R(0));
-}
-template <class R>
-__device__ __forceinline__ R bspline_grad(R x) {
-  const R a = absv(x);
-  const R inner = R(1.5) * x * a - R(2) * x;
-  // -a*x/2 + 2x - 2x/|x| ; for 1<=|x|<2 the last term is 2*sign(x)
-  const R sgn = x < R(0) ? R(-1) : R(1);
-  const R outer = -R(0.5) * a * x + R(2) * x - R(2) * sgn;
-  return a < R(1) ? inner : (a < R(2) ? outer : R(0));
 }
 
 // ---- Lame parameters (lame.include.glsl:69-85) -----------------------------------------
@@ -115,7 +104,7 @@ __device__ __forceinline__ void jacobi_conj(int x, int y, int z, R& s00, R& s11,
   const R w = rsq(ch2 + sh2);
   ch = flag ? w * ch : CSTAR; ch2 = ch * ch;
   sh = flag ? w * sh : SSTAR; sh2 = sh * sh;
-  const R scale = R(1) / (ch2 + sh2);
+  const R scale = recip(ch2 + sh2);
   const R a = (ch2 - sh2) * scale;
   const R b = (R(2) * sh * ch) * scale;
   const R a2 = a * a, b2 = b * b, ab = a * b;
@@ -172,7 +161,7 @@ __device__ __forceinline__ void svd3(const R* A, R* W, R* S, R* V) {
   }
   quat_to_mat(q[0], q[1], q[2], q[3], V);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) V[i] = rnd(R(1e5) * V[i]) / R(1e5);
+  for (int i = 0; i < 9; ++i) V[i] = round_to(V[i], R(1e5), R(1e-5));
   R B[9];
   mul(A, V, B);
   // sort columns by squared norm, descending, with negating swaps (SVD.include.glsl:191-228)
@@ -262,7 +251,7 @@ __device__ __forceinline__ void force_matrix(const R* FE, const R* FP, R m, R rh
   svd3(FE, W, S, V);
   mul_bt(W, V, Rot);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) Rot[i] = rnd(R(1e9) * Rot[i]) / R(1e9);
+  for (int i = 0; i < 9; ++i) Rot[i] = round_to(Rot[i], R(1e9), R(1e-9));
   const R JP = det3(FP), JE = det3(FE);
   const R hf = hardening_factor(xi, JP);
   const R mu = mu0 * hf, la = lambda0 * hf;

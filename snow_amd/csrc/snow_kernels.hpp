@@ -44,19 +44,50 @@ struct SceneDev {
   int64_t p_off;      // first particle
   int64_t p_cnt;      // particle count
   float mu0, lambda0; // E/(2(1+nu)), E nu/((1+nu)(1-2nu))
-  float pad1[2];
+  float inv_h;        // 1/h
+  float pad1;
+  float uo[3];        // -origin/h: grid coordinate u = x/h + uo
+  float pad2;
 };
 
-// Global cell key of a position: (tile) * 64 + (cell within tile); particles whose base
-// cell lies outside [-2, D] (no in-grid stencil node) go to the scene's dead tile.
+// Grid coordinate of a position: u = (x - origin)/h evaluated as x*(1/h) - origin/h (one FMA).
+__device__ __forceinline__ float grid_u(const SceneDev& S, float x, int k) { return fmaf(x, S.inv_h, S.uo[k]); }
+
+// Cubic B-spline weights of the four stencil nodes of one axis, for base = floor(u) and
+// f = u - base in [0,1). Node o (o = 0..3) sits at base + o - 1, so its distance d = o - 1 - f
+// always falls in the same polynomial piece of cubic.include.glsl:7-16:
+//   w0 = (1-f)^3/6, w1 = (3f^3 - 6f^2 + 4)/6, w2 = (-3f^3 + 3f^2 + 3f + 1)/6, w3 = f^3/6,
+// and weightingGradient(-d) (cubic.include.glsl:25-34) is dw_o/df:
+//   q0 = -(1-f)^2/2, q1 = 3f^2/2 - 2f, q2 = -3f^2/2 + f + 1/2, q3 = f^2/2.
+// Flooring (instead of the reference's truncation, Q6) only changes which zero-weight or
+// out-of-grid nodes are visited: in-grid contributions are identical.
+__device__ __forceinline__ float spline_w(int o, float f) {
+  const float c3 = o == 0 ? -1.f / 6.f : (o == 1 ? 0.5f : (o == 2 ? -0.5f : 1.f / 6.f));
+  const float c2 = o == 0 ? 0.5f : (o == 1 ? -1.0f : (o == 2 ? 0.5f : 0.f));
+  const float c1 = o == 0 ? -0.5f : (o == 2 ? 0.5f : 0.f);
+  const float c0 = o == 0 ? 1.f / 6.f : (o == 1 ? 2.f / 3.f : (o == 2 ? 1.f / 6.f : 0.f));
+  return fmaf(fmaf(fmaf(c3, f, c2), f, c1), f, c0);
+}
+__device__ __forceinline__ float spline_q(int o, float f) {
+  const float b2 = o == 0 ? -0.5f : (o == 1 ? 1.5f : (o == 2 ? -1.5f : 0.5f));
+  const float b1 = o == 0 ? 1.0f : (o == 1 ? -2.0f : (o == 2 ? 1.0f : 0.f));
+  const float b0 = o == 0 ? -0.5f : (o == 2 ? 0.5f : 0.f);
+  return fmaf(fmaf(b2, f, b1), f, b0);
+}
+
+// Global cell key of a position: (tile) * 64 + (cell within tile) for the cell floor(u) + 2;
+// particles whose base cell lies outside [-2, D] (no in-grid stencil node) go to the scene's
+// dead tile.
+__device__ __forceinline__ bool in_stencil_range(const SceneDev& S, float ux, float uy, float uz) {
+  // floor(u) in [-2, D]: at least one in-grid node within the cubic support
+  return ux >= -2.0f && uy >= -2.0f && uz >= -2.0f && ux < float(S.D[0] + 1) && uy < float(S.D[1] + 1) &&
+         uz < float(S.D[2] + 1);
+}
+
 __device__ __forceinline__ uint32_t cell_key(const SceneDev& S, float x, float y, float z) {
-  const float ux = (x - S.origin[0]) / S.h;
-  const float uy = (y - S.origin[1]) / S.h;
-  const float uz = (z - S.origin[2]) / S.h;
-  const bool in = ux > -3.0f && uy > -3.0f && uz > -3.0f && ux < float(S.D[0] + 1) &&
-                  uy < float(S.D[1] + 1) && uz < float(S.D[2] + 1);
-  if (!in) return S.dead_tile * TILE_CELLS;
-  const int cx = int(ux) + 2, cy = int(uy) + 2, cz = int(uz) + 2;
+  const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
+  if (!in_stencil_range(S, ux, uy, uz)) return S.dead_tile * TILE_CELLS;
+  const int cx = int(floorf(ux)) + 2, cy = int(floorf(uy)) + 2, cz = int(floorf(uz)) + 2;
   const uint32_t tile = S.ct_off + uint32_t((cx >> 2) + S.ct[0] * ((cy >> 2) + S.ct[1] * (cz >> 2)));
   return tile * TILE_CELLS + uint32_t((cx & 3) + 4 * (cy & 3) + 16 * (cz & 3));
 }
@@ -76,28 +107,73 @@ __global__ void __launch_bounds__(256) k_keys(const SceneDev* __restrict__ scene
 }
 
 // ---------------------------------------------------------------------------------------
-// Start/end of every non-empty cell in the sorted key array. Entries of empty cells are
-// left stale; readers validate them with keys[start] == cell. Block 0 also advances the
-// colliders (rigidAdvance.glsl:14-18): no kernel of this launch reads them.
-__global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict__ keys, uint32_t n,
-                                                     uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend,
-                                                     const SceneDev* __restrict__ scenes, int nscenes,
-                                                     Collider* __restrict__ cols, float dt_override) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const uint32_t k = keys[i];
-    if (i == 0 || keys[i - 1] != k) cstart[k] = i;
-    if (i == n - 1 || keys[i + 1] != k) cend[k] = i + 1;
+// Per-step control block in device memory, so every launch of a step has fixed arguments
+// (graph-capturable) while the set of active tiles changes from step to step.
+struct StepCtl {
+  uint32_t step;      // step stamp, incremented by k_step_begin
+  uint32_t n_active;  // active cell tiles this step (entries of active_list)
+  uint32_t p2g_next;  // work counter of the persistent P2G kernel
+  uint32_t pad;
+};
+
+// Start of a step: new stamp, cleared counters, and the collider advance of rigidAdvance.glsl:14-18
+// (cx += dt*cv) when dt_rigid >= 0 (0 selects each scene's own dt).
+__global__ void __launch_bounds__(256) k_step_begin(StepCtl* __restrict__ ctl, unsigned long long* __restrict__ counts,
+                                                    int ncounts, const SceneDev* __restrict__ scenes, int nscenes,
+                                                    Collider* __restrict__ cols, float dt_rigid) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    ctl->step += 1u;
+    ctl->n_active = 0u;
+    ctl->p2g_next = 0u;
   }
-  if (blockIdx.x == 0 && dt_override >= 0.0f) {
+  for (int i = t; i < ncounts; i += blockDim.x) counts[i] = 0ull;
+  if (dt_rigid >= 0.0f) {
     for (int s = 0; s < nscenes; ++s) {
       const SceneDev& S = scenes[s];
-      const float dt = dt_override > 0.0f ? dt_override : S.dt;
-      for (int c = threadIdx.x; c < S.ncol; c += blockDim.x) {
+      const float dt = dt_rigid > 0.0f ? dt_rigid : S.dt;
+      for (int c = t; c < S.ncol; c += blockDim.x) {
         Collider& C = cols[S.col_off + c];
         C.px += dt * C.vx; C.py += dt * C.vy; C.pz += dt * C.vz;
       }
     }
+  }
+}
+
+// Collider advance alone (rigidAdvance.glsl:14-18) for the per-stage API.
+__global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restrict__ scenes, int nscenes,
+                                                       Collider* __restrict__ cols, float dt_rigid) {
+  for (int s = 0; s < nscenes; ++s) {
+    const SceneDev& S = scenes[s];
+    const float dt = dt_rigid > 0.0f ? dt_rigid : S.dt;
+    for (int c = threadIdx.x; c < S.ncol; c += blockDim.x) {
+      Collider& C = cols[S.col_off + c];
+      C.px += dt * C.vx; C.py += dt * C.vy; C.pz += dt * C.vz;
+    }
+  }
+}
+
+// Start/end of every non-empty cell in the sorted key array, and the list of active cell
+// tiles (stamped with this step). Entries of empty cells are left stale; readers validate
+// them with keys[start] == cell. The order of active_list is not deterministic, but nothing
+// computed from it depends on that order.
+__global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict__ keys, uint32_t n,
+                                                     uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend,
+                                                     const SceneDev* __restrict__ scenes,
+                                                     const uint16_t* __restrict__ tile_scene, StepCtl* __restrict__ ctl,
+                                                     uint32_t* __restrict__ tile_stamp,
+                                                     uint32_t* __restrict__ active_list) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  const uint32_t prev = i ? keys[i - 1] : 0xffffffffu;
+  if (i == 0 || prev != k) cstart[k] = i;
+  if (i == n - 1 || keys[i + 1] != k) cend[k] = i + 1;
+  const uint32_t tile = k / TILE_CELLS;
+  if ((i == 0 || prev / TILE_CELLS != tile) && scenes[tile_scene[tile]].dead_tile != tile) {
+    const uint32_t slot = atomicAdd(&ctl->n_active, 1u);
+    active_list[slot] = tile;
+    tile_stamp[tile] = ctl->step;
   }
 }
 
@@ -137,27 +213,37 @@ __device__ __forceinline__ int fixed_exponent(float maxval, uint32_t count) {
   return e > 126 ? 126 : (e < -126 ? -126 : e);
 }
 
+constexpr int P2G_CHUNK = 32;        // particles staged per wave at a time
+constexpr int P2G_STAGE = 40;        // floats per staged particle
+
+// Persistent: each workgroup pulls active tiles from active_list until the list is exhausted.
 __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
-    const float4* __restrict__ rec, float* __restrict__ tpart, int* __restrict__ tile_active,
-    uint32_t n, int skip_heavy) {
+    const float4* __restrict__ rec, float* __restrict__ tpart, StepCtl* __restrict__ ctl,
+    const uint32_t* __restrict__ active_list, uint32_t n, int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   __shared__ uint32_t s_cs[TILE_CELLS];
   __shared__ uint32_t s_ce[TILE_CELLS];
   __shared__ unsigned long long s_acc[NQ * HALO_NODES];
+  // per-wave staging of up to P2G_CHUNK particles: for each axis the interleaved (w_o, q_o),
+  // o = 0..3 (24 floats), then m, m*v (3), A row-major (9), 3 pad
+  __shared__ __attribute__((aligned(16))) float s_stage[P2G_WAVES][P2G_CHUNK * P2G_STAGE];
   __shared__ uint32_t s_rng[2];
   __shared__ uint32_t s_bnd[3];
+  __shared__ uint32_t s_work;
 
-  const uint32_t tile = blockIdx.x;
+  const int t = threadIdx.x;
+  const uint32_t n_active = ctl->n_active;
+  for (;;) {
+  if (t == 0) s_work = atomicAdd(&ctl->p2g_next, 1u);
+  __syncthreads();
+  const uint32_t work = s_work;
+  if (work >= n_active) break;
+  const uint32_t tile = active_list[work];
   const int sid = tile_scene[tile];
   const SceneDev& S = scenes[sid];
-  const int t = threadIdx.x;
-  if (tile == S.dead_tile) {
-    if (t == 0) tile_active[tile] = 0;
-    return;
-  }
   if (t == 0) { s_rng[0] = 0xffffffffu; s_rng[1] = 0u; s_bnd[0] = s_bnd[1] = s_bnd[2] = 0u; }
   __syncthreads();
   bool valid = false;
@@ -170,10 +256,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
     s_ce[t] = en;
     if (valid) { atomicMin(&s_rng[0], st); atomicMax(&s_rng[1], en); }
   }
-  if (!__syncthreads_or(valid)) {
-    if (t == 0) tile_active[tile] = 0;
-    return;
-  }
+  if (!__syncthreads_or(valid)) continue;  // listed tiles always hold particles
   // bound pre-pass over the tile's particles (contiguous in sorted order)
   const uint32_t r_lo = s_rng[0], r_hi = s_rng[1];
   {
@@ -197,46 +280,67 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
   const int ef = fixed_exponent(__uint_as_float(s_bnd[2]), count);
   const float sm = ldexpf(1.0f, em), sp = ldexpf(1.0f, ep), sf = ldexpf(1.0f, ef);
 
-  const uint32_t tl = tile - S.ct_off;
-  const int tx = int(tl % uint32_t(S.ct[0]));
-  const int ty = int((tl / uint32_t(S.ct[0])) % uint32_t(S.ct[1]));
-  const int tz = int(tl / uint32_t(S.ct[0] * S.ct[1]));
   const int wave = t >> 6, lane = t & 63;
   const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
-  const float orgx = S.origin[0], orgy = S.origin[1], orgz = S.origin[2], h = S.h;
+  const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
+  float* stage = s_stage[wave];
 
   for (int cl = wave; cl < TILE_CELLS; cl += P2G_WAVES) {
     const uint32_t b = __builtin_amdgcn_readfirstlane(s_cs[cl]);
     const uint32_t e = __builtin_amdgcn_readfirstlane(s_ce[cl]);
     if (b >= e) continue;
     const int cx = cl & 3, cy = (cl >> 2) & 3, cz = cl >> 4;
-    // node of this lane: base + o - 1 with base = 4*tile + c - 2
-    const float gx = float(TILE * tx + cx - 3 + ox);
-    const float gy = float(TILE * ty + cy - 3 + oy);
-    const float gz = float(TILE * tz + cz - 3 + oz);
     float am = 0.f, apx = 0.f, apy = 0.f, apz = 0.f, afx = 0.f, afy = 0.f, afz = 0.f;
-    for (uint32_t j = b; j < e; ++j) {
-      const uint32_t p = vals[j];
-      const float4 r0 = rec[4 * size_t(p)];
-      const float4 r1 = rec[4 * size_t(p) + 1];
-      const float4 r2 = rec[4 * size_t(p) + 2];
-      const float4 r3 = rec[4 * size_t(p) + 3];
-      const float m = r0.w;
-      if (skip_heavy && fabsf(m) > 100.0f) continue;  // p2gTransfer.glsl:64-66 (not in gMassCompute)
-      const float ux = (r0.x - orgx) / h, uy = (r0.y - orgy) / h, uz = (r0.z - orgz) / h;
-      const float dx = gx - ux, dy = gy - uy, dz = gz - uz;
-      const float wx = bspline(dx), wy = bspline(dy), wz = bspline(dz);
-      const float qx = bspline_grad(-dx), qy = bspline_grad(-dy), qz = bspline_grad(-dz);
-      const float wyz = wy * wz;
-      const float w = wx * wyz;
-      const float g0 = qx * wyz, g1 = wx * qy * wz, g2 = wx * wy * qz;
-      am += m * w;
-      apx += (r1.w * m) * w;
-      apy += (r2.w * m) * w;
-      apz += (r3.w * m) * w;
-      afx -= r1.x * g0 + r1.y * g1 + r1.z * g2;
-      afy -= r2.x * g0 + r2.y * g1 + r2.z * g2;
-      afz -= r3.x * g0 + r3.y * g1 + r3.z * g2;
+    for (uint32_t c0 = b; c0 < e; c0 += P2G_CHUNK) {
+      const uint32_t cnt = min(uint32_t(P2G_CHUNK), e - c0);
+      // stage: lane k < cnt loads particle c0+k and evaluates its weights once
+      if (uint32_t(lane) < cnt) {
+        const uint32_t p = vals[c0 + lane];
+        const float4 r0 = rec[4 * size_t(p)];
+        const float4 r1 = rec[4 * size_t(p) + 1];
+        const float4 r2 = rec[4 * size_t(p) + 2];
+        const float4 r3 = rec[4 * size_t(p) + 3];
+        // p2gTransfer.glsl:64-66 skips |m| > 100 (gMassCompute does not)
+        const float use = (skip_heavy && fabsf(r0.w) > 100.0f) ? 0.0f : 1.0f;
+        const float m = r0.w * use;
+        const float ux = fmaf(r0.x, inv_h, uo0), uy = fmaf(r0.y, inv_h, uo1), uz = fmaf(r0.z, inv_h, uo2);
+        const float fx = ux - floorf(ux), fy = uy - floorf(uy), fz = uz - floorf(uz);
+        float* d = stage + lane * P2G_STAGE;
+        float4* d4 = reinterpret_cast<float4*>(d);
+        d4[0] = make_float4(spline_w(0, fx), spline_q(0, fx), spline_w(1, fx), spline_q(1, fx));
+        d4[1] = make_float4(spline_w(2, fx), spline_q(2, fx), spline_w(3, fx), spline_q(3, fx));
+        d4[2] = make_float4(spline_w(0, fy), spline_q(0, fy), spline_w(1, fy), spline_q(1, fy));
+        d4[3] = make_float4(spline_w(2, fy), spline_q(2, fy), spline_w(3, fy), spline_q(3, fy));
+        d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
+        d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
+        d4[6] = make_float4(m, r1.w * m, r2.w * m, r3.w * m);
+        d4[7] = make_float4(r1.x * use, r1.y * use, r1.z * use, r2.x * use);
+        d4[8] = make_float4(r2.y * use, r2.z * use, r3.x * use, r3.y * use);
+        d4[9] = make_float4(r3.z * use, 0.f, 0.f, 0.f);
+      }
+      // the wave's own LDS writes complete before its reads (in-order LDS per wave)
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const float* d = stage + k * P2G_STAGE;
+        const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
+        const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
+        const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
+        const float4 u0 = reinterpret_cast<const float4*>(d)[6];
+        const float4 u1 = reinterpret_cast<const float4*>(d)[7];
+        const float4 u2 = reinterpret_cast<const float4*>(d)[8];
+        const float a22 = d[36];
+        const float wyz = wqy.x * wqz.x;
+        const float w = wqx.x * wyz;
+        const float g0 = wqx.y * wyz, g1 = wqy.y * (wqx.x * wqz.x), g2 = wqz.y * (wqx.x * wqy.x);
+        am = fmaf(u0.x, w, am);
+        apx = fmaf(u0.y, w, apx);
+        apy = fmaf(u0.z, w, apy);
+        apz = fmaf(u0.w, w, apz);
+        afx -= u1.x * g0 + u1.y * g1 + u1.z * g2;
+        afy -= u1.w * g0 + u2.x * g1 + u2.y * g2;
+        afz -= u2.z * g0 + u2.w * g1 + a22 * g2;
+      }
+      __builtin_amdgcn_wave_barrier();
     }
     const int jn = (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
     atomicAdd(&s_acc[0 * HALO_NODES + jn], (unsigned long long)llrintf(am * sm));
@@ -254,85 +358,99 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
     const int e = q == 0 ? em : (q < 4 ? ep : ef);
     out[i] = ldexpf(float((long long)s_acc[i]), -e);
   }
-  if (t == 0) tile_active[tile] = 1;
+  __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------------------
 // Merge of the tile partials into node values + grid update (gForceCompute.glsl:57-103).
-// One wave per 4^3 node tile K; node n = 4K + r receives from cell tiles K (halo index
-// r + 3) and, when r >= 1 on an axis, K+1 (halo index r - 1): at most 8 tiles, summed in a
-// fixed order with TwoSum. Node tiles none of whose contributing cell tiles is active are
-// skipped: no particle's stencil reaches them.
+// A node n = 4K + r (per axis) receives from cell tiles K (halo index r + 3) and, when r >= 1,
+// K+1 (halo index r - 1): at most 8 tiles, summed in a fixed order with TwoSum. Work items
+// are (active cell tile k, offset a in {0,1}^3): node tile K = k - a is processed by the
+// block whose k is the first stamped tile among K + b (b in the fixed order 000,100,010,...),
+// so every node tile reached by a particle stencil is processed exactly once.
+__device__ __forceinline__ bool tile_live(const SceneDev& S, const uint32_t* __restrict__ stamp, uint32_t step,
+                                          int kx, int ky, int kz) {
+  if (kx < 0 || ky < 0 || kz < 0 || kx >= S.ct[0] || ky >= S.ct[1] || kz >= S.ct[2]) return false;
+  return stamp[S.ct_off + uint32_t(kx + S.ct[0] * (ky + S.ct[1] * kz))] == step;
+}
+
 template <bool KEEP, bool UPDATE>
 __global__ void __launch_bounds__(64) k_grid(const SceneDev* __restrict__ scenes,
-                                              const uint16_t* __restrict__ ntile_scene,
-                                              const int* __restrict__ tile_active,
-                                              const float* __restrict__ tpart,
-                                              float4* __restrict__ g_mmv, float4* __restrict__ g_f,
-                                              float4* __restrict__ g_vs, float4* __restrict__ g_flip,
-                                              const Collider* __restrict__ cols, float dt_override,
-                                              unsigned long long* __restrict__ active_count) {
-  const uint32_t nt = blockIdx.x;
-  const int sid = ntile_scene[nt];
-  const SceneDev& S = scenes[sid];
-  const uint32_t ntl = nt - S.nt_off;
-  const int Kx = int(ntl % uint32_t(S.nt[0]));
-  const int Ky = int((ntl / uint32_t(S.nt[0])) % uint32_t(S.nt[1]));
-  const int Kz = int(ntl / uint32_t(S.nt[0] * S.nt[1]));
+                                             const uint16_t* __restrict__ tile_scene,
+                                             const StepCtl* __restrict__ ctl,
+                                             const uint32_t* __restrict__ active_list,
+                                             const uint32_t* __restrict__ tile_stamp,
+                                             const float* __restrict__ tpart,
+                                             float4* __restrict__ g_mmv, float4* __restrict__ g_f,
+                                             float4* __restrict__ g_vs, float4* __restrict__ g_flip,
+                                             const Collider* __restrict__ cols, float dt_override,
+                                             unsigned long long* __restrict__ active_count) {
+  const uint32_t step = ctl->step;
+  const uint32_t items = ctl->n_active * 8u;
   const int t = threadIdx.x;
   const int rx = t & 3, ry = (t >> 2) & 3, rz = t >> 4;
-  const int nx = TILE * Kx + rx, ny = TILE * Ky + ry, nz = TILE * Kz + rz;
-  const bool inside = nx < S.D[0] && ny < S.D[1] && nz < S.D[2];
+  for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
+    const uint32_t k = active_list[w >> 3];
+    const int a = int(w & 7u);
+    const int sid = tile_scene[k];
+    const SceneDev& S = scenes[sid];
+    const uint32_t kl = k - S.ct_off;
+    const int kx = int(kl % uint32_t(S.ct[0])), ky = int((kl / uint32_t(S.ct[0])) % uint32_t(S.ct[1]));
+    const int kz = int(kl / uint32_t(S.ct[0] * S.ct[1]));
+    const int Kx = kx - (a & 1), Ky = ky - ((a >> 1) & 1), Kz = kz - (a >> 2);
+    if (Kx < 0 || Ky < 0 || Kz < 0 || Kx >= S.nt[0] || Ky >= S.nt[1] || Kz >= S.nt[2]) continue;
+    // owner: first live tile among K + b, b = 0..7 in order
+    int owner = -1;
+#pragma unroll
+    for (int bb = 0; bb < 8; ++bb)
+      if (owner < 0 && tile_live(S, tile_stamp, step, Kx + (bb & 1), Ky + ((bb >> 1) & 1), Kz + (bb >> 2))) owner = bb;
+    if (owner != a) continue;
 
-  float hi[NQ], lo[NQ];
+    const int nx = TILE * Kx + rx, ny = TILE * Ky + ry, nz = TILE * Kz + rz;
+    const bool inside = nx < S.D[0] && ny < S.D[1] && nz < S.D[2];
+    if (!inside) continue;
+    float hi[NQ], lo[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) { hi[q] = 0.f; lo[q] = 0.f; }
-  bool any = false;
-  if (inside) {
+    for (int q = 0; q < NQ; ++q) { hi[q] = 0.f; lo[q] = 0.f; }
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
-      if ((ax && rx < 1) || (ay && ry < 1) || (az && rz < 1)) continue;
-      const uint32_t ctile = S.ct_off + uint32_t((Kx + ax) + S.ct[0] * ((Ky + ay) + S.ct[1] * (Kz + az)));
-      if (!tile_active[ctile]) continue;
-      any = true;
-      const int j = (ax ? rx - 1 : rx + 3) + HALO * (ay ? ry - 1 : ry + 3) + HALO * HALO * (az ? rz - 1 : rz + 3);
+    for (int bb = 0; bb < 8; ++bb) {
+      const int bx = bb & 1, by = (bb >> 1) & 1, bz = bb >> 2;
+      if ((bx && rx < 1) || (by && ry < 1) || (bz && rz < 1)) continue;
+      if (!tile_live(S, tile_stamp, step, Kx + bx, Ky + by, Kz + bz)) continue;
+      const uint32_t ctile = S.ct_off + uint32_t((Kx + bx) + S.ct[0] * ((Ky + by) + S.ct[1] * (Kz + bz)));
+      const int j = (bx ? rx - 1 : rx + 3) + HALO * (by ? ry - 1 : ry + 3) + HALO * HALO * (bz ? rz - 1 : rz + 3);
       const float* src = tpart + size_t(ctile) * (NQ * HALO_NODES) + j;
 #pragma unroll
       for (int q = 0; q < NQ; ++q) two_sum_acc(hi[q], lo[q], src[q * HALO_NODES]);
     }
-  }
-  if (!__syncthreads_or(any)) return;
-  if (!inside) return;
-  const float m = hi[0] + lo[0];
-  const float px = hi[1] + lo[1], py = hi[2] + lo[2], pz = hi[3] + lo[3];
-  const float fx = hi[4] + lo[4], fy = hi[5] + lo[5], fz = hi[6] + lo[6];
-  const int64_t node = S.node_off + nx + int64_t(S.D[0]) * (ny + int64_t(S.D[1]) * nz);
-  if (KEEP) {
-    g_mmv[node] = make_float4(px, py, pz, m);
-    g_f[node] = make_float4(fx, fy, fz, 0.0f);
-  }
-  if (UPDATE) {
-    const float dt = dt_override > 0.0f ? dt_override : S.dt;
-    float vx = 0.f, vy = 0.f, vz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
-    if (m > 0.0f) {
-      vx = px / m + dt * (fx / m + S.g[0]);
-      vy = py / m + dt * (fy / m + S.g[1]);
-      vz = pz / m + dt * (fz / m + S.g[2]);
-      const float xi = S.origin[0] + S.h * float(nx);
-      const float yi = S.origin[1] + S.h * float(ny);
-      const float zi = S.origin[2] + S.h * float(nz);
-      collide(xi, yi, zi, vx, vy, vz, cols + S.col_off, S.ncol);
-      bx = vx - S.alpha * (px / m);
-      by = vy - S.alpha * (py / m);
-      bz = vz - S.alpha * (pz / m);
+    const float m = hi[0] + lo[0];
+    const float px = hi[1] + lo[1], py = hi[2] + lo[2], pz = hi[3] + lo[3];
+    const float fx = hi[4] + lo[4], fy = hi[5] + lo[5], fz = hi[6] + lo[6];
+    const int64_t node = S.node_off + nx + int64_t(S.D[0]) * (ny + int64_t(S.D[1]) * nz);
+    if (KEEP) {
+      g_mmv[node] = make_float4(px, py, pz, m);
+      g_f[node] = make_float4(fx, fy, fz, 0.0f);
     }
-    g_vs[node] = make_float4(vx, vy, vz, m);
-    g_flip[node] = make_float4(bx, by, bz, 0.0f);
-  }
-  if (active_count) {
-    const unsigned long long ballot = __ballot(m > 0.0f);
-    if ((t & 63) == 0 && ballot) atomicAdd(active_count + sid, (unsigned long long)__popcll(ballot));
+    if (UPDATE) {
+      const float dt = dt_override > 0.0f ? dt_override : S.dt;
+      float vx = 0.f, vy = 0.f, vz = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+      if (m > 0.0f) {
+        vx = px / m + dt * (fx / m + S.g[0]);
+        vy = py / m + dt * (fy / m + S.g[1]);
+        vz = pz / m + dt * (fz / m + S.g[2]);
+        const float xi = S.origin[0] + S.h * float(nx);
+        const float yi = S.origin[1] + S.h * float(ny);
+        const float zi = S.origin[2] + S.h * float(nz);
+        collide(xi, yi, zi, vx, vy, vz, cols + S.col_off, S.ncol);
+        bx = vx - S.alpha * (px / m);
+        by = vy - S.alpha * (py / m);
+        bz = vz - S.alpha * (pz / m);
+      }
+      g_vs[node] = make_float4(vx, vy, vz, m);
+      g_flip[node] = make_float4(bx, by, bz, 0.0f);
+    }
+    if (active_count && m > 0.0f) atomicAdd(active_count + sid, 1ull);
   }
 }
 
@@ -390,7 +508,8 @@ __global__ void __launch_bounds__(256) k_grid_import(const SceneDev* __restrict_
 
 // Dense export of one scene's grid (zeros where the tiled path did not touch the node).
 __global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict__ scenes, int sid,
-                                                     const int* __restrict__ tile_active, int dense_valid,
+                                                     const uint32_t* __restrict__ tile_stamp,
+                                                     const StepCtl* __restrict__ ctl, int dense_valid,
                                                      const float4* __restrict__ g_mmv, const float4* __restrict__ g_f,
                                                      const float4* __restrict__ g_vs, float* __restrict__ out_m,
                                                      float* __restrict__ out_mv, float* __restrict__ out_f,
@@ -402,12 +521,12 @@ __global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict_
   const int nx = int(l % S.D[0]), ny = int((l / S.D[0]) % S.D[1]), nz = int(l / (int64_t(S.D[0]) * S.D[1]));
   bool live = dense_valid != 0;
   if (!live) {
+    const uint32_t step = ctl->step;
     const int Kx = nx >> 2, Ky = ny >> 2, Kz = nz >> 2, rx = nx & 3, ry = ny & 3, rz = nz & 3;
     for (int a = 0; a < 8 && !live; ++a) {
       const int ax = a & 1, ay = (a >> 1) & 1, az = a >> 2;
       if ((ax && rx < 1) || (ay && ry < 1) || (az && rz < 1)) continue;
-      const uint32_t ctile = S.ct_off + uint32_t((Kx + ax) + S.ct[0] * ((Ky + ay) + S.ct[1] * (Kz + az)));
-      live = tile_active[ctile] != 0;
+      live = tile_live(S, tile_stamp, step, Kx + ax, Ky + ay, Kz + az);
     }
   }
   const int64_t node = S.node_off + l;
@@ -421,16 +540,18 @@ __global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict_
 }
 
 // ---------------------------------------------------------------------------------------
-// Per-particle stencil weights: for axis value u, base = trunc(u), offsets o = 0..3 ->
-// node base+o-1, d = node - u, w[o] = N(d), q[o] = N'(-d) (cubic.include.glsl:21-40 with
-// the distance vector of g2pTransfer.glsl:85-90).
-__device__ __forceinline__ void axis_weights(float u, int base, float w[4], float q[4]) {
+// Per-particle stencil weights of one axis: base = floor(u), offsets o = 0..3 -> node base+o-1,
+// w[o] = N(node - u), q[o] = N'(u - node) (cubic.include.glsl:21-40 with the distance vector of
+// g2pTransfer.glsl:85-90), from the fixed polynomial pieces in f = u - base.
+__device__ __forceinline__ int axis_weights(float u, float w[4], float q[4]) {
+  const float fl = floorf(u);
+  const float f = u - fl;
 #pragma unroll
   for (int o = 0; o < 4; ++o) {
-    const float d = float(base + o - 1) - u;
-    w[o] = bspline(d);
-    q[o] = bspline_grad(-d);
+    w[o] = spline_w(o, f);
+    q[o] = spline_q(o, f);
   }
+  return int(fl);
 }
 
 // Gather of one particle from the node fields (g2pTransfer.glsl:53-143):
@@ -441,15 +562,12 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
   vpn[0] = vpn[1] = vpn[2] = 0.f;
 #pragma unroll
   for (int i = 0; i < 9; ++i) dvp[i] = 0.f;
-  const float ux = (x - S.origin[0]) / S.h, uy = (y - S.origin[1]) / S.h, uz = (z - S.origin[2]) / S.h;
-  const bool in = ux > -3.0f && uy > -3.0f && uz > -3.0f && ux < float(S.D[0] + 1) && uy < float(S.D[1] + 1) &&
-                  uz < float(S.D[2] + 1);
-  if (!in) return false;
-  const int bx = int(ux), by = int(uy), bz = int(uz);
+  const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
+  if (!in_stencil_range(S, ux, uy, uz)) return false;
   float wx[4], wy[4], wz[4], qx[4], qy[4], qz[4];
-  axis_weights(ux, bx, wx, qx);
-  axis_weights(uy, by, wy, qy);
-  axis_weights(uz, bz, wz, qz);
+  const int bx = axis_weights(ux, wx, qx);
+  const int by = axis_weights(uy, wy, qy);
+  const int bz = axis_weights(uz, wz, qz);
   float lv[3] = {0.f, 0.f, 0.f}, ld[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) ld[i] = 0.f;
@@ -519,7 +637,7 @@ __device__ __forceinline__ void deform_update(const float dvp[9], float dt, cons
   mul_bt(WS, V, Fe2);
   // F_P = V diag(1/S) W^T F  (particleAdvance.glsl:270-275)
   R VSi[9];
-  const R i0 = R(1) / S[0], i1 = R(1) / S[4], i2 = R(1) / S[8];
+  const R i0 = recip(S[0]), i1 = recip(S[4]), i2 = recip(S[8]);
 #pragma unroll
   for (int r = 0; r < 3; ++r) { VSi[r] = V[r] * i0; VSi[3 + r] = V[3 + r] * i1; VSi[6 + r] = V[6 + r] * i2; }
   R VSiWt[9], Fp2[9];
@@ -527,8 +645,8 @@ __device__ __forceinline__ void deform_update(const float dvp[9], float dt, cons
   mul(VSiWt, Fn, Fp2);
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    FEo[i] = float(rnd(R(1e9) * Fe2[i]) / R(1e9));
-    FPo[i] = float(rnd(R(1e9) * Fp2[i]) / R(1e9));
+    FEo[i] = float(round_to(Fe2[i], R(1e9), R(1e-9)));
+    FPo[i] = float(round_to(Fp2[i], R(1e9), R(1e-9)));
   }
 }
 
@@ -699,16 +817,13 @@ __global__ void __launch_bounds__(256) k_particle_volume(const SceneDev* __restr
   if (i >= n) return;
   const SceneDev& S = scenes[pscene[i]];
   const float4 r0 = rec[4 * size_t(i)];
-  const float ux = (r0.x - S.origin[0]) / S.h, uy = (r0.y - S.origin[1]) / S.h, uz = (r0.z - S.origin[2]) / S.h;
-  const bool in = ux > -3.0f && uy > -3.0f && uz > -3.0f && ux < float(S.D[0] + 1) && uy < float(S.D[1] + 1) &&
-                  uz < float(S.D[2] + 1);
+  const float ux = grid_u(S, r0.x, 0), uy = grid_u(S, r0.y, 1), uz = grid_u(S, r0.z, 2);
   float acc = 0.f;
-  if (in) {
-    const int bx = int(ux), by = int(uy), bz = int(uz);
+  if (in_stencil_range(S, ux, uy, uz)) {
     float wx[4], wy[4], wz[4], q[4];
-    axis_weights(ux, bx, wx, q);
-    axis_weights(uy, by, wy, q);
-    axis_weights(uz, bz, wz, q);
+    const int bx = axis_weights(ux, wx, q);
+    const int by = axis_weights(uy, wy, q);
+    const int bz = axis_weights(uz, wz, q);
     const float cell = S.h * S.h * S.h;
     for (int oz = 0; oz < 4; ++oz) {
       const int iz = bz + oz - 1;
