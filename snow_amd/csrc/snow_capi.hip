@@ -89,7 +89,7 @@ struct snow_ctx {
   uint32_t* d_tile_stamp = nullptr;
   uint32_t* d_active_list = nullptr;
   int num_cu = 256;
-  float4 *d_g_mmv = nullptr, *d_g_f = nullptr, *d_g_vs = nullptr, *d_g_flip = nullptr;
+  float4 *d_g_mmv = nullptr, *d_g_f = nullptr, *d_g_node = nullptr;  // g_node: 2 float4 per node
   Collider* d_cols = nullptr;
   unsigned long long* d_counts = nullptr;  // [nscenes] active nodes, then [1] fp64 count
   int* d_bad = nullptr;
@@ -307,7 +307,7 @@ int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
   const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles * 8u, unsigned(c->num_cu) * 32));
 #define GRID_ARGS                                                                                          \
   c->d_scenes, c->d_tile_scene, c->d_ctl, c->d_active_list, c->d_tile_stamp, c->d_tpart, c->d_g_mmv, c->d_g_f, \
-      c->d_g_vs, c->d_g_flip, c->d_cols, dt, cnt
+      c->d_g_node, c->d_cols, dt, cnt
   if (keep && update)
     k_grid<true, true><<<blocks, 64, 0, c->stream>>>(GRID_ARGS);
   else if (update)
@@ -329,7 +329,7 @@ int run_g2p_advance(snow_ctx* c, float dt) {
 #define G2P_ARGS                                                                                        \
   c->d_scenes, c->d_pscene, c->d_vals[1], c->d_rec[a], c->d_fe[a], c->d_fp[a], c->d_rho[a], c->d_pid[a], \
       c->d_rec[b], c->d_fe[b], c->d_fp[b], c->d_rho[b], c->d_pid[b], c->d_keys[0], c->d_vals[0],         \
-      c->d_g_vs, c->d_g_flip, c->d_cols, dt, c->N, f64
+      c->d_g_node, c->d_cols, dt, c->N, f64
     const unsigned g = grid_for(c->N, 256);
     if (mixed && comp) k_g2p_advance<true, true><<<g, 256, 0, c->stream>>>(G2P_ARGS);
     else if (mixed) k_g2p_advance<true, false><<<g, 256, 0, c->stream>>>(G2P_ARGS);
@@ -367,8 +367,8 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_sort_tmp,
-                  c->d_cstart, c->d_cend, c->d_tpart, c->d_ctl, c->d_tile_stamp, c->d_active_list, c->d_g_mmv, c->d_g_f, c->d_g_vs,
-                  c->d_g_flip, c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
+                  c->d_cstart, c->d_cend, c->d_tpart, c->d_ctl, c->d_tile_stamp, c->d_active_list, c->d_g_mmv, c->d_g_f, c->d_g_node,
+                  c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto& e : c->ev)
@@ -541,8 +541,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_active_list, c->total_ctiles));
   CREATECHK(dalloc(&c->d_g_mmv, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_g_f, size_t(c->total_nodes)));
-  CREATECHK(dalloc(&c->d_g_vs, size_t(c->total_nodes)));
-  CREATECHK(dalloc(&c->d_g_flip, size_t(c->total_nodes)));
+  CREATECHK(dalloc(&c->d_g_node, 2 * size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_cols, std::max<size_t>(1, c->cols.size())));
   CREATECHK(dalloc(&c->d_counts, size_t(c->nscenes + 1)));
   CREATECHK(dalloc(&c->d_bad, 1));
@@ -602,8 +601,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(hipMemset(c->d_active_list, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_g_mmv, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_f, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
-  CREATECHK(hipMemset(c->d_g_vs, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
-  CREATECHK(hipMemset(c->d_g_flip, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
+  CREATECHK(hipMemset(c->d_g_node, 0, 2 * sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_counts, 0, sizeof(unsigned long long) * (c->nscenes + 1)));
   CREATECHK(hipDeviceSynchronize());
 #undef CREATECHK
@@ -704,9 +702,11 @@ int snow_stage_grid_reset(snow_ctx* c) {
   if (!c) return SNOW_E_ARG;
   StageTimer tm(c, ST_RESET);
   const int64_t words = c->total_nodes * 4;
-  for (float4* p : {c->d_g_mmv, c->d_g_f, c->d_g_vs, c->d_g_flip}) {
-    if (words) {
-      k_fill_u32<<<grid_for(words, 256), 256, 0, c->stream>>>(reinterpret_cast<uint32_t*>(p), 0u, words);
+  for (int k = 0; k < 3; ++k) {
+    uint32_t* p = reinterpret_cast<uint32_t*>(k == 0 ? c->d_g_mmv : (k == 1 ? c->d_g_f : c->d_g_node));
+    const int64_t w = k == 2 ? 2 * words : words;
+    if (w) {
+      k_fill_u32<<<grid_for(w, 256), 256, 0, c->stream>>>(p, 0u, w);
       LAUNCHCHK(c);
     }
   }
@@ -736,7 +736,7 @@ int snow_stage_grid_update(snow_ctx* c, double dt) {
   StageTimer tm(c, ST_GRID);
   if (c->total_nodes) {
     k_grid_update_dense<<<grid_for(c->total_nodes, 256), 256, 0, c->stream>>>(
-        c->d_scenes, c->nscenes, c->d_g_mmv, c->d_g_f, c->d_g_vs, c->d_g_flip, c->d_cols, fdt, c->total_nodes);
+        c->d_scenes, c->nscenes, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_cols, fdt, c->total_nodes);
     LAUNCHCHK(c);
   }
   return tm.done();
@@ -748,8 +748,8 @@ int snow_stage_g2p(snow_ctx* c) {
   if (!c->d_aux) HIPCHK(c, dalloc(&c->d_aux, size_t(12) * std::max<uint32_t>(1, c->N)));
   StageTimer tm(c, ST_G2P);
   if (c->N) {
-    k_g2p_only<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->d_g_vs,
-                                                            c->d_g_flip, c->d_aux, c->N);
+    k_g2p_only<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->d_g_node,
+                                                            c->d_aux, c->N);
     LAUNCHCHK(c);
   }
   c->aux_valid = true;
@@ -837,11 +837,11 @@ int snow_read_grid(snow_ctx* c, int32_t s, float* m, float* mv, float* f, float*
   if (rc) return rc;
   float* st = static_cast<float*>(c->d_stage);
   float *om = st, *omv = om + nn, *of = omv + 3 * nn, *ovs = of + 3 * nn;
-  // With keep off, the mass lives in g_vs.w: export it from there.
-  const float4* mm_src = c->keep_valid ? c->d_g_mmv : c->d_g_vs;
+  // With keep off, the mass is read from the node records.
   k_grid_export<<<grid_for(nn, 256), 256, 0, c->stream>>>(c->d_scenes, s, c->d_tile_stamp, c->d_ctl,
                                                           c->grid_state == 2,
-                                                          mm_src, c->d_g_f, c->d_g_vs, om, omv, of, ovs);
+                                                          c->d_g_mmv, c->d_g_f, c->d_g_node, c->keep_valid ? 0 : 1,
+                                                          om, omv, of, ovs);
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (m) HIPCHK(c, hipMemcpy(m, om, sizeof(float) * nn, hipMemcpyDeviceToHost));
@@ -865,16 +865,18 @@ int snow_write_grid(snow_ctx* c, int32_t s, const float* m, const float* mv, con
   for (int64_t i = 0; i < nn; ++i) {
     mm[i] = make_float4(mv ? mv[3 * i] : 0.f, mv ? mv[3 * i + 1] : 0.f, mv ? mv[3 * i + 2] : 0.f, m[i]);
     ff[i] = make_float4(f ? f[3 * i] : 0.f, f ? f[3 * i + 1] : 0.f, f ? f[3 * i + 2] : 0.f, 0.f);
-    vs[i] = make_float4(vstar ? vstar[3 * i] : 0.f, vstar ? vstar[3 * i + 1] : 0.f, vstar ? vstar[3 * i + 2] : 0.f, m[i]);
+    vs[i] = make_float4(vstar ? vstar[3 * i] : 0.f, vstar ? vstar[3 * i + 1] : 0.f, vstar ? vstar[3 * i + 2] : 0.f, 0.f);
   }
+  int rc = ensure_stage(c, sizeof(float4) * size_t(nn));
+  if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int64_t off = c->scenes[s].node_off;
   HIPCHK(c, hipMemcpy(c->d_g_mmv + off, mm.data(), sizeof(float4) * nn, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_g_f + off, ff.data(), sizeof(float4) * nn, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(c->d_g_vs + off, vs.data(), sizeof(float4) * nn, hipMemcpyHostToDevice));
-  if (c->total_nodes) {
-    k_grid_import<<<grid_for(c->total_nodes, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_g_mmv, c->d_g_vs,
-                                                                          c->d_g_flip, c->nscenes, c->total_nodes);
+  HIPCHK(c, hipMemcpy(c->d_stage, vs.data(), sizeof(float4) * nn, hipMemcpyHostToDevice));
+  if (nn) {
+    k_grid_import<<<grid_for(nn, 256), 256, 0, c->stream>>>(c->d_scenes, s, c->d_g_mmv,
+                                                            static_cast<const float4*>(c->d_stage), c->d_g_node);
     LAUNCHCHK(c);
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));

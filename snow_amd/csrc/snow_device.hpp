@@ -25,9 +25,11 @@ template <> __device__ __forceinline__ double rsq<double>(double x) { return rsq
 template <class R> __device__ __forceinline__ R sq_root(R x);
 template <> __device__ __forceinline__ float sq_root<float>(float x) { return sqrtf(x); }
 template <> __device__ __forceinline__ double sq_root<double>(double x) { return sqrt(x); }
+// GLSL round(): nearest integer; the direction of exact .5 ties is left to the implementation,
+// here the hardware's round-to-nearest-even (one instruction).
 template <class R> __device__ __forceinline__ R rnd(R x);
-template <> __device__ __forceinline__ float rnd<float>(float x) { return roundf(x); }
-template <> __device__ __forceinline__ double rnd<double>(double x) { return round(x); }
+template <> __device__ __forceinline__ float rnd<float>(float x) { return rintf(x); }
+template <> __device__ __forceinline__ double rnd<double>(double x) { return rint(x); }
 template <class R> __device__ __forceinline__ R absv(R x) { return x < R(0) ? -x : x; }
 // reciprocal: v_rcp_f32 (1 ulp) for float, exact division for double
 template <class R> __device__ __forceinline__ R recip(R x);

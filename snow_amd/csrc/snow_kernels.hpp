@@ -7,8 +7,8 @@
 //    fe/fp    : 9 SoA float planes, plane e = c*3+r (column-major 3x3)
 //    rho0,pid : rest density (pv.w of the reference), original index
 //  grid, dense per scene (node i + j*Dx + k*Dx*Dy):
-//    g_vs     : float4 (v*, m)           - gForceCompute output gvn (+ mass)
-//    g_flip   : float4 (v* - alpha p/m)  - FLIP/PIC blend term used by G2P
+//    g_node   : 2 x float4 per node: (v*.x v*.y v*.z u.x | u.y u.z m 0), u = v* - alpha p/m,
+//               the gForceCompute output gvn and the FLIP/PIC blend term read together by G2P
 //    g_mmv,g_f: float4 (p, m), (f, 0)    - gv / gf of the reference (stage API, debug)
 //  cell tiles: 4x4x4 cells per tile over the shifted cell range c = base + 2, base in [-2, D]
 //    (a cubic stencil of base cell b touches nodes b-1..b+2); a tile's 7^3-node halo starts
@@ -106,6 +106,15 @@ __global__ void __launch_bounds__(256) k_keys(const SceneDev* __restrict__ scene
   vals[i] = i;
 }
 
+
+// Node record written by the grid update and read by G2P: v* (gvn of gForceCompute.glsl) and the
+// FLIP/PIC term u = v* - alpha p/m of g2pTransfer.glsl:100, packed as two aligned float4.
+__device__ __forceinline__ void store_node(float4* __restrict__ g, int64_t n, float vx, float vy, float vz, float ux,
+                                           float uy, float uz, float m) {
+  g[2 * n] = make_float4(vx, vy, vz, ux);
+  g[2 * n + 1] = make_float4(uy, uz, m, 0.0f);
+}
+
 // ---------------------------------------------------------------------------------------
 // Per-step control block in device memory, so every launch of a step has fixed arguments
 // (graph-capturable) while the set of active tiles changes from step to step.
@@ -164,14 +173,23 @@ __global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ tile_stamp,
                                                      uint32_t* __restrict__ active_list) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = keys[i];
-  const uint32_t prev = i ? keys[i - 1] : 0xffffffffu;
-  if (i == 0 || prev != k) cstart[k] = i;
-  if (i == n - 1 || keys[i + 1] != k) cend[k] = i + 1;
+  const bool in = i < n;
+  const uint32_t k = in ? keys[i] : 0u;
+  const uint32_t prev = (in && i) ? keys[i - 1] : 0xffffffffu;
+  if (in && (i == 0 || prev != k)) cstart[k] = i;
+  if (in && (i == n - 1 || keys[i + 1] != k)) cend[k] = i + 1;
   const uint32_t tile = k / TILE_CELLS;
-  if ((i == 0 || prev / TILE_CELLS != tile) && scenes[tile_scene[tile]].dead_tile != tile) {
-    const uint32_t slot = atomicAdd(&ctl->n_active, 1u);
+  const bool start = in && (i == 0 || prev / TILE_CELLS != tile) && scenes[tile_scene[tile]].dead_tile != tile;
+  // one atomic per wave for the tile-list append
+  const uint64_t ball = __ballot(start);
+  if (!ball) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)ball) - 1;
+  uint32_t slot0 = 0u;
+  if (lane == leader) slot0 = atomicAdd(&ctl->n_active, uint32_t(__popcll(ball)));
+  slot0 = __shfl(slot0, leader, 64);
+  if (start) {
+    const uint32_t slot = slot0 + uint32_t(__popcll(ball & ((1ull << lane) - 1ull)));
     active_list[slot] = tile;
     tile_stamp[tile] = ctl->step;
   }
@@ -383,7 +401,7 @@ __global__ void __launch_bounds__(64) k_grid(const SceneDev* __restrict__ scenes
                                              const uint32_t* __restrict__ tile_stamp,
                                              const float* __restrict__ tpart,
                                              float4* __restrict__ g_mmv, float4* __restrict__ g_f,
-                                             float4* __restrict__ g_vs, float4* __restrict__ g_flip,
+                                             float4* __restrict__ g_node,
                                              const Collider* __restrict__ cols, float dt_override,
                                              unsigned long long* __restrict__ active_count) {
   const uint32_t step = ctl->step;
@@ -447,8 +465,7 @@ __global__ void __launch_bounds__(64) k_grid(const SceneDev* __restrict__ scenes
         by = vy - S.alpha * (py / m);
         bz = vz - S.alpha * (pz / m);
       }
-      g_vs[node] = make_float4(vx, vy, vz, m);
-      g_flip[node] = make_float4(bx, by, bz, 0.0f);
+      store_node(g_node, node, vx, vy, vz, bx, by, bz, m);
     }
     if (active_count && m > 0.0f) atomicAdd(active_count + sid, 1ull);
   }
@@ -459,7 +476,7 @@ __global__ void __launch_bounds__(64) k_grid(const SceneDev* __restrict__ scenes
 __global__ void __launch_bounds__(256) k_grid_update_dense(const SceneDev* __restrict__ scenes, int nscenes,
                                                            const float4* __restrict__ g_mmv,
                                                            const float4* __restrict__ g_f,
-                                                           float4* __restrict__ g_vs, float4* __restrict__ g_flip,
+                                                           float4* __restrict__ g_node,
                                                            const Collider* __restrict__ cols, float dt_override,
                                                            int64_t total_nodes) {
   const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -483,27 +500,27 @@ __global__ void __launch_bounds__(256) k_grid_update_dense(const SceneDev* __res
     by = vy - S.alpha * (mm.y / m);
     bz = vz - S.alpha * (mm.z / m);
   }
-  g_vs[node] = make_float4(vx, vy, vz, m);
-  g_flip[node] = make_float4(bx, by, bz, 0.0f);
+  store_node(g_node, node, vx, vy, vz, bx, by, bz, m);
 }
 
-// From dense (m, p, v*) written by the host: the G2P inputs.
-__global__ void __launch_bounds__(256) k_grid_import(const SceneDev* __restrict__ scenes,
-                                                     const float4* __restrict__ g_mmv, float4* __restrict__ g_vs,
-                                                     float4* __restrict__ g_flip, int nscenes, int64_t total_nodes) {
-  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (node >= total_nodes) return;
-  int sid = 0;
-  while (sid + 1 < nscenes && scenes[sid + 1].node_off <= node) ++sid;
-  const float alpha = scenes[sid].alpha;
+// From dense (m, p) and a host-written v* (float4 (v*, -) per node of one scene): the G2P inputs.
+__global__ void __launch_bounds__(256) k_grid_import(const SceneDev* __restrict__ scenes, int sid,
+                                                     const float4* __restrict__ g_mmv,
+                                                     const float4* __restrict__ vstar, float4* __restrict__ g_node) {
+  const SceneDev& S = scenes[sid];
+  const int64_t nn = int64_t(S.D[0]) * S.D[1] * S.D[2];
+  const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (l >= nn) return;
+  const int64_t node = S.node_off + l;
   const float4 mm = g_mmv[node];
-  float4 vs = g_vs[node];
-  vs.w = mm.w;
-  g_vs[node] = vs;
-  if (mm.w > 0.0f)
-    g_flip[node] = make_float4(vs.x - alpha * (mm.x / mm.w), vs.y - alpha * (mm.y / mm.w), vs.z - alpha * (mm.z / mm.w), 0.f);
-  else
-    g_flip[node] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float m = mm.w;
+  if (m > 0.0f) {
+    const float4 vs = vstar[l];
+    store_node(g_node, node, vs.x, vs.y, vs.z, vs.x - S.alpha * (mm.x / m), vs.y - S.alpha * (mm.y / m),
+               vs.z - S.alpha * (mm.z / m), m);
+  } else {
+    store_node(g_node, node, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, m);  // gvn = 0 where m <= 0 (gForceCompute.glsl:251-253)
+  }
 }
 
 // Dense export of one scene's grid (zeros where the tiled path did not touch the node).
@@ -511,7 +528,8 @@ __global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict_
                                                      const uint32_t* __restrict__ tile_stamp,
                                                      const StepCtl* __restrict__ ctl, int dense_valid,
                                                      const float4* __restrict__ g_mmv, const float4* __restrict__ g_f,
-                                                     const float4* __restrict__ g_vs, float* __restrict__ out_m,
+                                                     const float4* __restrict__ g_node, int mass_from_node,
+                                                     float* __restrict__ out_m,
                                                      float* __restrict__ out_mv, float* __restrict__ out_f,
                                                      float* __restrict__ out_vs) {
   const SceneDev& S = scenes[sid];
@@ -532,8 +550,9 @@ __global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict_
   const int64_t node = S.node_off + l;
   const float4 mm = live ? g_mmv[node] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 ff = live ? g_f[node] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 vs = live ? g_vs[node] : make_float4(0.f, 0.f, 0.f, 0.f);
-  if (out_m) out_m[l] = mm.w;
+  const float4 vs = live ? g_node[2 * node] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float mass = mass_from_node ? (live ? g_node[2 * node + 1].z : 0.f) : mm.w;
+  if (out_m) out_m[l] = mass;
   if (out_mv) { out_mv[3 * l] = mm.x; out_mv[3 * l + 1] = mm.y; out_mv[3 * l + 2] = mm.z; }
   if (out_f) { out_f[3 * l] = ff.x; out_f[3 * l + 1] = ff.y; out_f[3 * l + 2] = ff.z; }
   if (out_vs) { out_vs[3 * l] = vs.x; out_vs[3 * l + 1] = vs.y; out_vs[3 * l + 2] = vs.z; }
@@ -554,11 +573,34 @@ __device__ __forceinline__ int axis_weights(float u, float w[4], float q[4]) {
   return int(fl);
 }
 
-// Gather of one particle from the node fields (g2pTransfer.glsl:53-143):
+// Gather of one particle from the node records (g2pTransfer.glsl:53-143):
 // vpn = sum_i w (v*_i - alpha p_i/m_i), dvp = sum_i v*_i (x) grad w (column k = v* * dw/dx_k).
+// Nodes with m <= 0 hold v* = 0 and u = 0 (written so by the grid update), so they add nothing
+// and the m_i > 0 test of g2pTransfer.glsl:99 needs no branch. Each (y,z) row of the stencil is
+// reduced along x first (separable weights); interior particles (whole stencil in the grid)
+// take an unchecked path with one row address per (y,z) and immediate x offsets.
+__device__ __forceinline__ void g2p_row(const float4* __restrict__ rowp, const float wx[4], const float qx[4],
+                                        float wyz, float qyz, float wqz, float vpn[3], float dvp[9], int xlo,
+                                        int xhi) {
+  float sq0 = 0.f, sq1 = 0.f, sq2 = 0.f, sw0 = 0.f, sw1 = 0.f, sw2 = 0.f, fw0 = 0.f, fw1 = 0.f, fw2 = 0.f;
+#pragma unroll
+  for (int ox = 0; ox < 4; ++ox) {
+    if (ox < xlo || ox > xhi) continue;
+    const float4 a = rowp[2 * ox];
+    const float4 b = rowp[2 * ox + 1];
+    sq0 = fmaf(a.x, qx[ox], sq0); sq1 = fmaf(a.y, qx[ox], sq1); sq2 = fmaf(a.z, qx[ox], sq2);
+    sw0 = fmaf(a.x, wx[ox], sw0); sw1 = fmaf(a.y, wx[ox], sw1); sw2 = fmaf(a.z, wx[ox], sw2);
+    fw0 = fmaf(a.w, wx[ox], fw0); fw1 = fmaf(b.x, wx[ox], fw1); fw2 = fmaf(b.y, wx[ox], fw2);
+  }
+  vpn[0] = fmaf(fw0, wyz, vpn[0]); vpn[1] = fmaf(fw1, wyz, vpn[1]); vpn[2] = fmaf(fw2, wyz, vpn[2]);
+  dvp[0] = fmaf(sq0, wyz, dvp[0]); dvp[1] = fmaf(sq1, wyz, dvp[1]); dvp[2] = fmaf(sq2, wyz, dvp[2]);
+  dvp[3] = fmaf(sw0, qyz, dvp[3]); dvp[4] = fmaf(sw1, qyz, dvp[4]); dvp[5] = fmaf(sw2, qyz, dvp[5]);
+  dvp[6] = fmaf(sw0, wqz, dvp[6]); dvp[7] = fmaf(sw1, wqz, dvp[7]); dvp[8] = fmaf(sw2, wqz, dvp[8]);
+}
+
 __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, float z,
-                                           const float4* __restrict__ g_vs, const float4* __restrict__ g_flip,
-                                           float vpn[3], float dvp[9], bool compensated) {
+                                           const float4* __restrict__ g_node, float vpn[3], float dvp[9],
+                                           bool compensated) {
   vpn[0] = vpn[1] = vpn[2] = 0.f;
 #pragma unroll
   for (int i = 0; i < 9; ++i) dvp[i] = 0.f;
@@ -568,49 +610,64 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
   const int bx = axis_weights(ux, wx, qx);
   const int by = axis_weights(uy, wy, qy);
   const int bz = axis_weights(uz, wz, qz);
-  float lv[3] = {0.f, 0.f, 0.f}, ld[9];
+  const int D0 = S.D[0], D1 = S.D[1], D2 = S.D[2];
+  const bool interior = bx >= 1 && by >= 1 && bz >= 1 && bx + 2 < D0 && by + 2 < D1 && bz + 2 < D2;
+  if (compensated) {
+    float lv[3] = {0.f, 0.f, 0.f}, ld[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) ld[i] = 0.f;
-  const float gs = S.gscale;
-  for (int oz = 0; oz < 4; ++oz) {
-    const int iz = bz + oz - 1;
-    if (iz < 0 || iz >= S.D[2]) continue;
-    for (int oy = 0; oy < 4; ++oy) {
-      const int iy = by + oy - 1;
-      if (iy < 0 || iy >= S.D[1]) continue;
-      const int64_t row = S.node_off + int64_t(S.D[0]) * (iy + int64_t(S.D[1]) * iz);
-      const float wyz = wy[oy] * wz[oz];
-#pragma unroll
-      for (int ox = 0; ox < 4; ++ox) {
-        const int ix = bx + ox - 1;
-        if (ix < 0 || ix >= S.D[0]) continue;
-        const float4 vs = g_vs[row + ix];
-        if (!(vs.w > 0.0f)) continue;
-        const float4 fl = g_flip[row + ix];
-        const float w = wx[ox] * wyz;
-        const float g0 = gs * (qx[ox] * wyz), g1 = gs * (wx[ox] * qy[oy] * wz[oz]), g2 = gs * (wx[ox] * wy[oy] * qz[oz]);
-        const float c[12] = {fl.x * w, fl.y * w, fl.z * w, vs.x * g0, vs.y * g0, vs.z * g0,
-                             vs.x * g1, vs.y * g1, vs.z * g1, vs.x * g2, vs.y * g2, vs.z * g2};
-        if (compensated) {
+    for (int i = 0; i < 9; ++i) ld[i] = 0.f;
+    for (int oz = 0; oz < 4; ++oz) {
+      const int iz = bz + oz - 1;
+      if (iz < 0 || iz >= D2) continue;
+      for (int oy = 0; oy < 4; ++oy) {
+        const int iy = by + oy - 1;
+        if (iy < 0 || iy >= D1) continue;
+        const int64_t row = S.node_off + int64_t(D0) * (iy + int64_t(D1) * iz);
+        const float wyz = wy[oy] * wz[oz], qyz = qy[oy] * wz[oz], wqz = wy[oy] * qz[oz];
+        for (int ox = 0; ox < 4; ++ox) {
+          const int ix = bx + ox - 1;
+          if (ix < 0 || ix >= D0) continue;
+          const float4 a = g_node[2 * (row + ix)];
+          const float4 b = g_node[2 * (row + ix) + 1];
+          const float w = wx[ox] * wyz;
+          const float g0 = qx[ox] * wyz, g1 = wx[ox] * qyz, g2 = wx[ox] * wqz;
+          const float c[12] = {a.w * w, b.x * w, b.y * w, a.x * g0, a.y * g0, a.z * g0,
+                               a.x * g1, a.y * g1, a.z * g1, a.x * g2, a.y * g2, a.z * g2};
 #pragma unroll
           for (int k = 0; k < 3; ++k) two_sum_acc(vpn[k], lv[k], c[k]);
 #pragma unroll
           for (int k = 0; k < 9; ++k) two_sum_acc(dvp[k], ld[k], c[3 + k]);
-        } else {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) vpn[k] += c[k];
-#pragma unroll
-          for (int k = 0; k < 9; ++k) dvp[k] += c[3 + k];
         }
       }
     }
-  }
-  if (compensated) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) vpn[k] += lv[k];
 #pragma unroll
     for (int k = 0; k < 9; ++k) dvp[k] += ld[k];
+  } else if (interior) {
+    const float4* p0 = g_node + 2 * (S.node_off + (bx - 1) + int64_t(D0) * ((by - 1) + int64_t(D1) * (bz - 1)));
+    const int64_t sy = 2 * int64_t(D0), sz = 2 * int64_t(D0) * D1;
+#pragma unroll 1
+    for (int oz = 0; oz < 4; ++oz)
+#pragma unroll
+      for (int oy = 0; oy < 4; ++oy)
+        g2p_row(p0 + oz * sz + oy * sy, wx, qx, wy[oy] * wz[oz], qy[oy] * wz[oz], wy[oy] * qz[oz], vpn, dvp, 0, 3);
+  } else {
+    const int xlo = max(0, 1 - bx), xhi = min(3, D0 - bx);  // ox with 0 <= bx + ox - 1 < D0
+    for (int oz = 0; oz < 4; ++oz) {
+      const int iz = bz + oz - 1;
+      if (iz < 0 || iz >= D2) continue;
+      for (int oy = 0; oy < 4; ++oy) {
+        const int iy = by + oy - 1;
+        if (iy < 0 || iy >= D1) continue;
+        const float4* rowp = g_node + 2 * (S.node_off + (bx - 1) + int64_t(D0) * (iy + int64_t(D1) * iz));
+        g2p_row(rowp, wx, qx, wy[oy] * wz[oz], qy[oy] * wz[oz], wy[oy] * qz[oz], vpn, dvp, xlo, xhi);
+      }
+    }
   }
+  const float gs = S.gscale;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dvp[k] *= gs;  // weight gradient scale (1 in the reference)
   return true;
 }
 
@@ -686,7 +743,7 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
     const float* __restrict__ rho_in, const uint32_t* __restrict__ pid_in, float4* __restrict__ rec_out,
     float* __restrict__ fe_out, float* __restrict__ fp_out, float* __restrict__ rho_out,
     uint32_t* __restrict__ pid_out, uint32_t* __restrict__ keys_next, uint32_t* __restrict__ vals_next,
-    const float4* __restrict__ g_vs, const float4* __restrict__ g_flip, const Collider* __restrict__ cols,
+    const float4* __restrict__ g_node, const Collider* __restrict__ cols,
     float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -704,7 +761,7 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
   const uint32_t id = pid_in[p];
 
   float vpn[3], dvp[9];
-  g2p_gather(S, r0.x, r0.y, r0.z, g_vs, g_flip, vpn, dvp, COMP);
+  g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, COMP);
 
   float FEo[9], FPo[9];
   bool wide = false;
@@ -751,15 +808,15 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
 // the advance alone consumes them in place (g2pTransfer.glsl / particleAdvance.glsl).
 __global__ void __launch_bounds__(256) k_g2p_only(const SceneDev* __restrict__ scenes,
                                                   const uint16_t* __restrict__ pscene,
-                                                  const float4* __restrict__ rec, const float4* __restrict__ g_vs,
-                                                  const float4* __restrict__ g_flip, float* __restrict__ aux,
+                                                  const float4* __restrict__ rec, const float4* __restrict__ g_node,
+                                                  float* __restrict__ aux,
                                                   uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SceneDev& S = scenes[pscene[i]];
   const float4 r0 = rec[4 * size_t(i)];
   float vpn[3], dvp[9];
-  g2p_gather(S, r0.x, r0.y, r0.z, g_vs, g_flip, vpn, dvp, false);
+  g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, false);
 #pragma unroll
   for (int k = 0; k < 3; ++k) aux[size_t(k) * n + i] = vpn[k];
 #pragma unroll
