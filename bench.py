@@ -1,10 +1,18 @@
 """Benchmark of the MI355X snow MPM timestep.
 
-Default workload: two snowballs (2^20 particles, radius 0.12, +-3 m/s) on a 128^3 grid over the
+Headline workload: two snowballs (2^20 particles, radius 0.12, +-3 m/s) on a 128^3 grid over the
 unit cube, dt = 5e-5, ground plane. One "step" is one full explicit MPM step of all particles
-(sort, P2G, grid update, G2P + advance) with the state resident in HBM. Prints ONE JSON line.
+(sort, P2G, grid update, G2P + advance) with the state resident in HBM. `value` is the plain
+FP32 form's particle-steps/s. The same line also reports
+  * forms.mixed    - FP32 with the FP64 SVD fallback, same workload,
+  * errors         - max/RMS error of x, v, F_E, F_P and grid m, mv against the FP64 CPU oracle
+                     after a few steps, for both forms,
+  * forms.batched  - the 64-scene parameter sweep as one batch (captured step graph, autotuned
+                     launch shapes) against the same scenes stepped one context at a time,
+  * cpu_baseline   - the FP32 CPU restatement of the step on the host cores.
+Prints ONE JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain|mixed] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-errors] [--no-sweep]
 """
 from __future__ import annotations
 
@@ -55,6 +63,76 @@ def cpu_baseline(scene, budget_s: float = 12.0):
                       f"{cores} std::threads"}
 
 
+def timed_steps(ctx, steps: int, dt=None, graph: bool = False) -> float:
+    ctx.sync()
+    t0 = time.perf_counter()
+    if graph:
+        ctx.run_graph(steps)
+    else:
+        for _ in range(steps):
+            ctx.step(dt)
+    ctx.sync()
+    return time.perf_counter() - t0
+
+
+def error_report(scene, flags, steps: int) -> dict:
+    """Max / RMS error of every quantity against the FP64 oracle after `steps` steps."""
+    import numpy as np
+    from oracle import pyoracle
+    from snow_amd import FLAG_KEEP_GRID, SnowContext
+    with SnowContext(scene, flags | FLAG_KEEP_GRID) as ctx:
+        ctx.init()
+        for _ in range(steps):
+            ctx.step(scene.dt)
+        p = ctx.read_particles()
+        g = ctx.read_grid(fields=("m", "mv"))
+    o = pyoracle.Oracle(scene, nthreads=min(16, os.cpu_count() or 1))
+    o.init()
+    o.step(scene.dt, steps)
+    out = {}
+    for name, a, b in (("x", p["x"], o.x), ("v", p["v"], o.v), ("F_E", p["fe"], o.fe), ("F_P", p["fp"], o.fp),
+                       ("grid_m", g["m"], o.gm), ("grid_mv", g["mv"], o.gmv)):
+        d = np.asarray(a, np.float64) - b
+        if d.ndim > 1 and name in ("F_E", "F_P"):
+            d = np.sqrt((d ** 2).sum(-1))  # Frobenius per particle
+        elif d.ndim > 1:
+            d = np.sqrt((d ** 2).sum(-1))
+        out[name] = {"max": float(np.abs(d).max()), "rms": float(np.sqrt((d ** 2).mean()))}
+    return out
+
+
+def batched_sweep(steps: int) -> dict:
+    """64 scenes in one batch (graph + autotune) vs the same scenes one context at a time."""
+    from snow_amd import SnowContext, scenes
+    sc = scenes.parameter_sweep()
+    n = sum(len(s.particles) for s in sc)
+    with SnowContext(sc) as ctx:
+        ctx.init()
+        shape = ctx.autotune(steps=3)
+        ctx.run_graph(5)
+        el = timed_steps(ctx, steps, graph=True)
+    batched = n * steps / el
+    ctxs = [SnowContext(s) for s in sc]
+    for c in ctxs:
+        c.init()
+        c.step(c.scenes[0].dt)
+    for c in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c in ctxs:
+            c.step(c.scenes[0].dt)
+    for c in ctxs:
+        c.sync()
+    el_s = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    single = n * steps / el_s
+    return {"value": batched, "scene_by_scene": single, "speedup": batched / single, "scenes": len(sc),
+            "particles": n, "steps": steps, "launch_shape": shape,
+            "grids": sorted({s.grid.dimx for s in sc}), "unit": "particle-steps/s"}
+
+
 def copy_calibration(mib: int) -> None:
     """One device-to-device hipMemcpy of `mib` MiB (HIP's copy kernel): the reference point for the
     FETCH_SIZE / WRITE_SIZE counters of the traffic measurement."""
@@ -82,11 +160,19 @@ def main():
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--only-sweep", action="store_true", help="run only the batched 64-scene sweep")
+    ap.add_argument("--error-steps", type=int, default=10)
+    ap.add_argument("--sweep-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.only_sweep:
+        print(json.dumps({"forms": {"batched": batched_sweep(args.sweep_steps)}}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding of the "
@@ -155,6 +241,26 @@ def main():
         "kernels": kernels,
     }
     ctx.close()
+
+    other = FLAG_MIXED if args.form == "plain" else 0
+    with SnowContext(scene, other) as ctx2:
+        ctx2.init()
+        for _ in range(args.warmup):
+            ctx2.step(dt)
+        el2 = timed_steps(ctx2, args.steps, dt)
+        fp64 = ctx2.fp64_particles() if other else 0
+    out["forms"] = {args.form: {"value": value, "unit": "particle-steps/s"},
+                    ("mixed" if other else "plain"): {"value": n * args.steps / el2, "unit": "particle-steps/s",
+                                                      "fp64_particles_last_step": fp64}}
+    print("[bench] second form timed", file=sys.stderr, flush=True)
+    if not args.no_errors:
+        out["errors"] = {"steps": args.error_steps, "reference": "FP64 CPU oracle",
+                         "plain": error_report(scene, 0, args.error_steps),
+                         "mixed": error_report(scene, FLAG_MIXED, args.error_steps)}
+        print("[bench] error report done", file=sys.stderr, flush=True)
+    if not args.no_sweep:
+        out["forms"]["batched"] = batched_sweep(args.sweep_steps)
+        print("[bench] batched sweep done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget)

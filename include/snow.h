@@ -130,8 +130,16 @@ int snow_init(snow_ctx* ctx);
 int snow_step(snow_ctx* ctx, double dt);
 /* Same step with each scene's own snow_scene_desc::dt. */
 int snow_step_batch(snow_ctx* ctx);
-/* Run `steps` batched steps as replays of one captured HIP graph. */
+/* Run `steps` batched steps as replays of a captured HIP graph of the step (one graph per
+ * buffer parity, captured on first use). No reference counterpart: the reference dispatches
+ * each compute shader from the host every step (explicittimeupdate.cpp:36-63). */
 int snow_run_graph(snow_ctx* ctx, int32_t steps);
+/* Time a few launch shapes of the batched step (G2P block size, widths of the persistent P2G
+ * and grid-merge kernels), `steps` steps per candidate on a snapshot of the current state, keep
+ * the fastest and restore the state. Any of the out pointers may be NULL. No reference
+ * counterpart (the reference's work-group sizes are fixed at 1024, defines.hpp:28). */
+int snow_autotune(snow_ctx* ctx, int32_t steps, int32_t* g2p_block, int32_t* p2g_per_cu,
+                  int32_t* grid_per_cu);
 
 /* ---- per-stage entry points (the Technique::dispatch of each compute shader) ---- */
 int snow_stage_grid_mass(snow_ctx* ctx);        /* gMassCompute.glsl:33-84 (init)      */
@@ -184,6 +192,8 @@ int snow_stage_times(snow_ctx* ctx, double* total_ms, int64_t* calls);
 int snow_reset_stage_times(snow_ctx* ctx);
 /* Number of grid nodes of a scene with mass > 0 after the last P2G (for byte counts). */
 int snow_active_nodes(snow_ctx* ctx, int32_t scene, int64_t* out);
+/* Particles (all scenes) that took the FP64 SVD path in the last step (SNOW_FLAG_MIXED). */
+int snow_fp64_particles(snow_ctx* ctx, int64_t* out);
 
 #ifdef __cplusplus
 }

@@ -250,6 +250,11 @@ class SnowContext:
     def run_graph(self, steps: int):
         self._chk(self.lib.snow_run_graph(self.h, steps))
 
+    def autotune(self, steps: int = 5) -> dict:
+        g, p, q = C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.snow_autotune(self.h, steps, C.byref(g), C.byref(p), C.byref(q)))
+        return {"g2p_block": g.value, "p2g_per_cu": p.value, "grid_per_cu": q.value}
+
     def sync(self):
         self._chk(self.lib.snow_sync(self.h))
 
@@ -322,6 +327,11 @@ class SnowContext:
     def active_nodes(self, scene: int = 0) -> int:
         v = C.c_int64()
         self._chk(self.lib.snow_active_nodes(self.h, scene, C.byref(v)))
+        return int(v.value)
+
+    def fp64_particles(self) -> int:
+        v = C.c_int64()
+        self._chk(self.lib.snow_fp64_particles(self.h, C.byref(v)))
         return int(v.value)
 
 

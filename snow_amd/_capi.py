@@ -72,6 +72,7 @@ SIGNATURES = [
     ("snow_step", C.c_int, [C.c_void_p, C.c_double]),
     ("snow_step_batch", C.c_int, [C.c_void_p]),
     ("snow_run_graph", C.c_int, [C.c_void_p, C.c_int32]),
+    ("snow_autotune", C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, i32p]),
     ("snow_stage_grid_mass", C.c_int, [C.c_void_p]),
     ("snow_stage_particle_volume", C.c_int, [C.c_void_p]),
     ("snow_stage_rigid_advance", C.c_int, [C.c_void_p, C.c_double]),
@@ -96,6 +97,7 @@ SIGNATURES = [
     ("snow_stage_times", C.c_int, [C.c_void_p, f64p, i64p]),
     ("snow_reset_stage_times", C.c_int, [C.c_void_p]),
     ("snow_active_nodes", C.c_int, [C.c_void_p, C.c_int32, i64p]),
+    ("snow_fp64_particles", C.c_int, [C.c_void_p, i64p]),
 ]
 
 _lib = None

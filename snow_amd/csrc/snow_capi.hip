@@ -83,7 +83,7 @@ struct snow_ctx {
   uint32_t* d_vals[2] = {nullptr, nullptr};
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  uint32_t *d_cstart = nullptr, *d_cend = nullptr;
+  uint32_t *d_tile_lo = nullptr, *d_tile_hi = nullptr;
   float* d_tpart = nullptr;
   StepCtl* d_ctl = nullptr;
   uint32_t* d_tile_stamp = nullptr;
@@ -113,8 +113,13 @@ struct snow_ctx {
   double stage_ms[ST_COUNT] = {0};
   int64_t stage_calls[ST_COUNT] = {0};
 
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};      // batched step starting from buffer parity 0 / 1
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
+  // launch shapes (autotuned)
+  int g2p_block = 256;
+  int p2g_per_cu = 2;
+  int grid_per_cu = 32;
 };
 
 namespace {
@@ -283,7 +288,7 @@ int run_sort(snow_ctx* c, float dt_rigid) {
                                c->key_bits, c->stream));
   }
   if (c->N) {
-    k_cell_bounds<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_keys[1], c->N, c->d_cstart, c->d_cend,
+    k_tile_bounds<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_keys[1], c->N, c->d_tile_lo, c->d_tile_hi,
                                                                c->d_scenes, c->d_tile_scene, c->d_ctl,
                                                                c->d_tile_stamp, c->d_active_list);
     LAUNCHCHK(c);
@@ -293,10 +298,10 @@ int run_sort(snow_ctx* c, float dt_rigid) {
 
 int run_p2g(snow_ctx* c, int mass_only) {
   StageTimer tm(c, ST_P2G);
-  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu) * 2));
+  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu * c->p2g_per_cu)));
   k_p2g_tile<<<blocks, 64 * P2G_WAVES, 0, c->stream>>>(c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1],
-                                                       c->d_cstart, c->d_cend, c->d_rec[c->cur], c->d_tpart,
-                                                       c->d_ctl, c->d_active_list, c->N, mass_only ? 0 : 1);
+                                                       c->d_tile_lo, c->d_tile_hi, c->d_rec[c->cur], c->d_tpart,
+                                                       c->d_ctl, c->d_active_list, mass_only ? 0 : 1);
   LAUNCHCHK(c);
   return tm.done();
 }
@@ -304,7 +309,7 @@ int run_p2g(snow_ctx* c, int mass_only) {
 int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
   StageTimer tm(c, ST_GRID);
   unsigned long long* cnt = c->d_counts;
-  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles * 8u, unsigned(c->num_cu) * 32));
+  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles * 8u, unsigned(c->num_cu * c->grid_per_cu)));
 #define GRID_ARGS                                                                                          \
   c->d_scenes, c->d_tile_scene, c->d_ctl, c->d_active_list, c->d_tile_stamp, c->d_tpart, c->d_g_mmv, c->d_g_f, \
       c->d_g_node, c->d_cols, dt, cnt
@@ -330,11 +335,12 @@ int run_g2p_advance(snow_ctx* c, float dt) {
   c->d_scenes, c->d_pscene, c->d_vals[1], c->d_rec[a], c->d_fe[a], c->d_fp[a], c->d_rho[a], c->d_pid[a], \
       c->d_rec[b], c->d_fe[b], c->d_fp[b], c->d_rho[b], c->d_pid[b], c->d_keys[0], c->d_vals[0],         \
       c->d_g_node, c->d_cols, dt, c->N, f64
-    const unsigned g = grid_for(c->N, 256);
-    if (mixed && comp) k_g2p_advance<true, true><<<g, 256, 0, c->stream>>>(G2P_ARGS);
-    else if (mixed) k_g2p_advance<true, false><<<g, 256, 0, c->stream>>>(G2P_ARGS);
-    else if (comp) k_g2p_advance<false, true><<<g, 256, 0, c->stream>>>(G2P_ARGS);
-    else k_g2p_advance<false, false><<<g, 256, 0, c->stream>>>(G2P_ARGS);
+    const unsigned bs = unsigned(c->g2p_block);
+    const unsigned g = grid_for(c->N, bs);
+    if (mixed && comp) k_g2p_advance<true, true><<<g, bs, 0, c->stream>>>(G2P_ARGS);
+    else if (mixed) k_g2p_advance<true, false><<<g, bs, 0, c->stream>>>(G2P_ARGS);
+    else if (comp) k_g2p_advance<false, true><<<g, bs, 0, c->stream>>>(G2P_ARGS);
+    else k_g2p_advance<false, false><<<g, bs, 0, c->stream>>>(G2P_ARGS);
 #undef G2P_ARGS
     LAUNCHCHK(c);
   }
@@ -362,12 +368,14 @@ int step_impl(snow_ctx* c, float dt) {
 
 void release(snow_ctx* c) {
   if (!c) return;
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (int p = 0; p < 2; ++p) {
+    if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
+    if (c->graph[p]) (void)hipGraphDestroy(c->graph[p]);
+  }
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_sort_tmp,
-                  c->d_cstart, c->d_cend, c->d_tpart, c->d_ctl, c->d_tile_stamp, c->d_active_list, c->d_g_mmv, c->d_g_f, c->d_g_node,
+                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_ctl, c->d_tile_stamp, c->d_active_list, c->d_g_mmv, c->d_g_f, c->d_g_node,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -533,8 +541,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     CREATECHK(hipMalloc(&c->d_sort_tmp, std::max<size_t>(tb, 16)));
   }
   const size_t ncells = size_t(c->total_ctiles) * TILE_CELLS;
-  CREATECHK(dalloc(&c->d_cstart, ncells));
-  CREATECHK(dalloc(&c->d_cend, ncells));
+  CREATECHK(dalloc(&c->d_tile_lo, c->total_ctiles));
+  CREATECHK(dalloc(&c->d_tile_hi, c->total_ctiles));
   CREATECHK(dalloc(&c->d_tpart, size_t(c->total_ctiles) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_ctl, 1));
   CREATECHK(dalloc(&c->d_tile_stamp, c->total_ctiles));
@@ -594,8 +602,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
       CREATECHK(hipMemset(c->d_rho[0], 0, sizeof(float) * N));
     }
   }
-  CREATECHK(hipMemset(c->d_cstart, 0xff, sizeof(uint32_t) * ncells));
-  CREATECHK(hipMemset(c->d_cend, 0, sizeof(uint32_t) * ncells));
+  CREATECHK(hipMemset(c->d_tile_lo, 0, sizeof(uint32_t) * c->total_ctiles));
+  CREATECHK(hipMemset(c->d_tile_hi, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_ctl, 0, sizeof(StepCtl)));
   CREATECHK(hipMemset(c->d_tile_stamp, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_active_list, 0, sizeof(uint32_t) * c->total_ctiles));
@@ -682,11 +690,156 @@ int snow_step_batch(snow_ctx* c) {
 
 int snow_run_graph(snow_ctx* c, int32_t steps) {
   if (!c || steps < 0) return SNOW_E_ARG;
-  // Graph capture of the step is not enabled in this build: run the batched step eagerly.
-  for (int32_t i = 0; i < steps; ++i) {
-    int rc = snow_step_batch(c);
-    if (rc) return rc;
+  if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before stepping");
+  for (const auto& S : c->scenes)
+    if (!(S.dt > 0.0f)) return fail(c, SNOW_E_ARG, "every scene needs dt > 0 for snow_run_graph");
+  int rc;
+  // prerequisites of the captured step, run eagerly: force matrices and cell keys of the state
+  if ((rc = run_force_prepass(c))) return rc;
+  if ((rc = run_keys(c))) return rc;
+  if (!c->gexec[0]) {
+    // One graph per buffer parity: the step reads buffers `cur` and writes `1 - cur`. Every launch
+    // in it has fixed arguments (per-step counters live in the device control block), and the
+    // captured kernels include no memset node.
+    const bool timers = c->timers;
+    const int cur0 = c->cur;
+    c->timers = false;
+    for (int p = 0; p < 2; ++p) {
+      c->cur = p;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = run_sort(c, 0.0f);
+      if (!rc) rc = run_p2g(c, 0);
+      if (!rc) rc = run_grid(c, (c->flags & SNOW_FLAG_KEEP_GRID) != 0, true, 0.0f);
+      if (!rc) rc = run_g2p_advance(c, 0.0f);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        c->timers = timers;
+        c->cur = cur0;
+        return rc;
+      }
+      if (ec != hipSuccess) {
+        c->timers = timers;
+        c->cur = cur0;
+        return fail(c, SNOW_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      }
+      c->graph[p] = g;
+      HIPCHK(c, hipGraphInstantiate(&c->gexec[p], g, nullptr, nullptr, 0));
+    }
+    c->timers = timers;
+    c->cur = cur0;
+    c->keys_valid = true;
+    c->force_valid = true;
   }
+  for (int32_t i = 0; i < steps; ++i) {
+    HIPCHK(c, hipGraphLaunch(c->gexec[c->cur], c->stream));
+    c->cur = 1 - c->cur;
+  }
+  if (steps > 0) {
+    c->keys_valid = true;
+    c->force_valid = true;
+    c->aux_valid = false;
+    c->grid_state = 1;
+    c->keep_valid = (c->flags & SNOW_FLAG_KEEP_GRID) != 0;
+  }
+  return 0;
+}
+
+int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_per_cu, int32_t* grid_per_cu) {
+  if (!c || steps <= 0) return SNOW_E_ARG;
+  if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before autotuning");
+  for (const auto& S : c->scenes)
+    if (!(S.dt > 0.0f)) return fail(c, SNOW_E_ARG, "every scene needs dt > 0 for snow_autotune");
+  int rc;
+  if ((rc = run_force_prepass(c))) return rc;
+  if ((rc = run_keys(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // snapshot of everything a step reads and writes
+  const size_t N = std::max<size_t>(1, c->N);
+  const int k = c->cur;
+  struct Buf { void* dst; const void* src; size_t bytes; };
+  std::vector<Buf> bufs = {
+      {nullptr, c->d_rec[k], sizeof(float4) * 4 * N}, {nullptr, c->d_fe[k], sizeof(float) * 9 * N},
+      {nullptr, c->d_fp[k], sizeof(float) * 9 * N},  {nullptr, c->d_rho[k], sizeof(float) * N},
+      {nullptr, c->d_pid[k], sizeof(uint32_t) * N},   {nullptr, c->d_keys[0], sizeof(uint32_t) * N},
+      {nullptr, c->d_vals[0], sizeof(uint32_t) * N},  {nullptr, c->d_ctl, sizeof(StepCtl)},
+      {nullptr, c->d_cols, sizeof(Collider) * std::max<size_t>(1, c->cols.size())}};
+  auto free_all = [&]() {
+    for (auto& b : bufs)
+      if (b.dst) (void)hipFree(b.dst);
+  };
+  for (auto& b : bufs) {
+    if (hipMalloc(&b.dst, b.bytes) != hipSuccess) {
+      free_all();
+      return fail(c, SNOW_E_CAPACITY, "autotune: no device memory for the state snapshot");
+    }
+    HIPCHK(c, hipMemcpyAsync(b.dst, b.src, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  auto restore = [&]() -> int {
+    for (auto& b : bufs) HIPCHK(c, hipMemcpyAsync(const_cast<void*>(b.src), b.dst, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+    c->cur = k;
+    c->keys_valid = true;
+    c->force_valid = true;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+  };
+  const bool timers = c->timers;
+  c->timers = false;
+  auto trial = [&](double& ms) -> int {
+    int r = restore();
+    if (r) return r;
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int32_t i = 0; i < steps; ++i)
+      if ((r = step_impl(c, 0.0f))) return r;
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[1]));
+    float f = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&f, c->ev[0], c->ev[1]));
+    ms = f;
+    return 0;
+  };
+  // coordinate descent over the three launch shapes
+  const int g2p_cands[] = {64, 128, 256};
+  const int p2g_cands[] = {1, 2};
+  const int grid_cands[] = {8, 32};
+  int best_g = c->g2p_block, best_p = c->p2g_per_cu, best_q = c->grid_per_cu;
+  double best = 1e30, t = 0.0;
+  for (int g : g2p_cands) {
+    c->g2p_block = g;
+    if ((rc = trial(t))) break;
+    if (t < best) { best = t; best_g = g; }
+  }
+  c->g2p_block = best_g;
+  for (int pp : p2g_cands) {
+    if (rc) break;
+    c->p2g_per_cu = pp;
+    if ((rc = trial(t))) break;
+    if (t < best) { best = t; best_p = pp; }
+  }
+  c->p2g_per_cu = best_p;
+  for (int q : grid_cands) {
+    if (rc) break;
+    c->grid_per_cu = q;
+    if ((rc = trial(t))) break;
+    if (t < best) { best = t; best_q = q; }
+  }
+  c->grid_per_cu = best_q;
+  c->timers = timers;
+  const int rr = restore();
+  free_all();
+  if (rc) return rc;
+  if (rr) return rr;
+  // captured graphs hold the old launch shapes
+  for (int p = 0; p < 2; ++p) {
+    if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
+    if (c->graph[p]) (void)hipGraphDestroy(c->graph[p]);
+    c->gexec[p] = nullptr;
+    c->graph[p] = nullptr;
+  }
+  if (g2p_block) *g2p_block = c->g2p_block;
+  if (p2g_per_cu) *p2g_per_cu = c->p2g_per_cu;
+  if (grid_per_cu) *grid_per_cu = c->grid_per_cu;
   return 0;
 }
 
@@ -987,6 +1140,15 @@ int snow_active_nodes(snow_ctx* c, int32_t s, int64_t* out) {
   unsigned long long v = 0;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(&v, c->d_counts + s, sizeof v, hipMemcpyDeviceToHost));
+  *out = int64_t(v);
+  return 0;
+}
+
+int snow_fp64_particles(snow_ctx* c, int64_t* out) {
+  if (!c || !out) return SNOW_E_ARG;
+  unsigned long long v = 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(&v, c->d_counts + c->nscenes, sizeof v, hipMemcpyDeviceToHost));
   *out = int64_t(v);
   return 0;
 }

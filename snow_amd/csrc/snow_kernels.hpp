@@ -162,24 +162,25 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
   }
 }
 
-// Start/end of every non-empty cell in the sorted key array, and the list of active cell
-// tiles (stamped with this step). Entries of empty cells are left stale; readers validate
-// them with keys[start] == cell. The order of active_list is not deterministic, but nothing
-// computed from it depends on that order.
-__global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict__ keys, uint32_t n,
-                                                     uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend,
+// Particle range [tile_lo, tile_hi) of every non-empty cell tile in the sorted key array, and
+// the list of active cell tiles (stamped with this step). Entries of tiles without particles
+// are left stale; only listed (stamped) tiles are read. The order of active_list is not
+// deterministic, but nothing computed from it depends on that order.
+__global__ void __launch_bounds__(256) k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n,
+                                                     uint32_t* __restrict__ tile_lo, uint32_t* __restrict__ tile_hi,
                                                      const SceneDev* __restrict__ scenes,
                                                      const uint16_t* __restrict__ tile_scene, StepCtl* __restrict__ ctl,
                                                      uint32_t* __restrict__ tile_stamp,
                                                      uint32_t* __restrict__ active_list) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < n;
-  const uint32_t k = in ? keys[i] : 0u;
-  const uint32_t prev = (in && i) ? keys[i - 1] : 0xffffffffu;
-  if (in && (i == 0 || prev != k)) cstart[k] = i;
-  if (in && (i == n - 1 || keys[i + 1] != k)) cend[k] = i + 1;
-  const uint32_t tile = k / TILE_CELLS;
-  const bool start = in && (i == 0 || prev / TILE_CELLS != tile) && scenes[tile_scene[tile]].dead_tile != tile;
+  const uint32_t tile = in ? keys[i] / TILE_CELLS : 0u;
+  const uint32_t prev = (in && i) ? keys[i - 1] / TILE_CELLS : 0xffffffffu;
+  const uint32_t next = (in && i + 1 < n) ? keys[i + 1] / TILE_CELLS : 0xffffffffu;
+  const bool first = in && (i == 0 || prev != tile);
+  if (in && (i + 1 == n || next != tile)) tile_hi[tile] = i + 1;
+  const bool start = first && scenes[tile_scene[tile]].dead_tile != tile;
+  if (first) tile_lo[tile] = i;
   // one atomic per wave for the tile-list append
   const uint64_t ball = __ballot(start);
   if (!ball) return;
@@ -197,14 +198,17 @@ __global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict_
 
 // ---------------------------------------------------------------------------------------
 // P2G on one 4^3 cell tile (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84).
-// A wave takes one cell at a time; lane l is the stencil offset o = (l&3, (l>>2)&3, l>>4)
-// (getIJK order, neighborIndex.include.glsl:6-12), so the 64 lanes cover the particle's 4x4x4
-// stencil and each particle of the cell is a wave-uniform scalar load. The cell's particles are
-// summed in registers in their (stable, sorted) order, then added to the tile's LDS accumulator
-// as 64-bit fixed point: integer addition is associative, so the tile sums are exact and do not
-// depend on the order in which the waves commit. The fixed-point scale is a power of two per
-// tile and quantity, 2^61 / (count * max |per-particle term|), from a vectorised pre-pass over
-// the tile's particles, so no sum can overflow and the resolution is ~2^-40 of the largest term.
+// The tile's particles are contiguous in sorted order (cell by cell); each wave takes an equal
+// contiguous share of them and stages P2G_CHUNK particles at a time into its LDS slot: one lane
+// per particle loads the record and evaluates the particle's 12 weights and 12 gradients once.
+// Then lane l of the wave is the stencil offset o = (l&3, (l>>2)&3, l>>4) (getIJK order,
+// neighborIndex.include.glsl:6-12): for each staged particle the 64 lanes compute the 64 node
+// contributions from broadcast LDS reads and sum them in registers. When the cell changes, the
+// register sums are added to the tile's LDS accumulator as 64-bit fixed point: integer addition
+// is associative, so the tile sums are exact whatever the order of the commits, and the result
+// is bit-reproducible. The fixed-point scale is a power of two per tile and quantity,
+// 2^61 / (count * max |per-particle term|), from a vectorised pre-pass over the tile's
+// particles, so no sum can overflow and the resolution is ~2^-40 of the largest term.
 struct P2GBound { float m, p, f; };
 
 __device__ __forceinline__ P2GBound particle_bound(const float4& r0, const float4& r1, const float4& r2,
@@ -238,82 +242,79 @@ constexpr int P2G_STAGE = 40;        // floats per staged particle
 __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-    const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+    const uint32_t* __restrict__ tile_lo, const uint32_t* __restrict__ tile_hi,
     const float4* __restrict__ rec, float* __restrict__ tpart, StepCtl* __restrict__ ctl,
-    const uint32_t* __restrict__ active_list, uint32_t n, int skip_heavy) {
+    const uint32_t* __restrict__ active_list, int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
-  __shared__ uint32_t s_cs[TILE_CELLS];
-  __shared__ uint32_t s_ce[TILE_CELLS];
   __shared__ unsigned long long s_acc[NQ * HALO_NODES];
   // per-wave staging of up to P2G_CHUNK particles: for each axis the interleaved (w_o, q_o),
-  // o = 0..3 (24 floats), then m, m*v (3), A row-major (9), 3 pad
+  // o = 0..3 (24 floats), then m, m*v (3), -A row-major (9), the local cell index, 2 pad
   __shared__ __attribute__((aligned(16))) float s_stage[P2G_WAVES][P2G_CHUNK * P2G_STAGE];
-  __shared__ uint32_t s_rng[2];
   __shared__ uint32_t s_bnd[3];
   __shared__ uint32_t s_work;
 
   const int t = threadIdx.x;
-  const uint32_t n_active = ctl->n_active;
-  for (;;) {
-  if (t == 0) s_work = atomicAdd(&ctl->p2g_next, 1u);
-  __syncthreads();
-  const uint32_t work = s_work;
-  if (work >= n_active) break;
-  const uint32_t tile = active_list[work];
-  const int sid = tile_scene[tile];
-  const SceneDev& S = scenes[sid];
-  if (t == 0) { s_rng[0] = 0xffffffffu; s_rng[1] = 0u; s_bnd[0] = s_bnd[1] = s_bnd[2] = 0u; }
-  __syncthreads();
-  bool valid = false;
-  if (t < TILE_CELLS) {
-    const uint32_t gc = tile * TILE_CELLS + t;
-    const uint32_t st = cstart[gc];
-    valid = st < n && keys[st] == gc;
-    const uint32_t en = valid ? cend[gc] : 0u;
-    s_cs[t] = valid ? st : 0u;
-    s_ce[t] = en;
-    if (valid) { atomicMin(&s_rng[0], st); atomicMax(&s_rng[1], en); }
-  }
-  if (!__syncthreads_or(valid)) continue;  // listed tiles always hold particles
-  // bound pre-pass over the tile's particles (contiguous in sorted order)
-  const uint32_t r_lo = s_rng[0], r_hi = s_rng[1];
-  {
-    float bm = 0.f, bp = 0.f, bf = 0.f;
-    for (uint32_t i = r_lo + t; i < r_hi; i += NT) {
-      const uint32_t p = vals[i];
-      const P2GBound b = particle_bound(rec[4 * size_t(p)], rec[4 * size_t(p) + 1], rec[4 * size_t(p) + 2],
-                                        rec[4 * size_t(p) + 3], skip_heavy);
-      bm = fmaxf(bm, b.m); bp = fmaxf(bp, b.p); bf = fmaxf(bf, b.f);
-    }
-    // non-negative floats order like their bit patterns
-    atomicMax(&s_bnd[0], __float_as_uint(bm));
-    atomicMax(&s_bnd[1], __float_as_uint(bp));
-    atomicMax(&s_bnd[2], __float_as_uint(bf));
-  }
-  for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0ull;
-  __syncthreads();
-  const uint32_t count = r_hi - r_lo;
-  const int em = fixed_exponent(__uint_as_float(s_bnd[0]), count);
-  const int ep = fixed_exponent(__uint_as_float(s_bnd[1]), count);
-  const int ef = fixed_exponent(__uint_as_float(s_bnd[2]), count);
-  const float sm = ldexpf(1.0f, em), sp = ldexpf(1.0f, ep), sf = ldexpf(1.0f, ef);
-
   const int wave = t >> 6, lane = t & 63;
   const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
-  const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
   float* stage = s_stage[wave];
+  const uint32_t n_active = ctl->n_active;
+  for (;;) {
+    if (t == 0) {
+      s_work = atomicAdd(&ctl->p2g_next, 1u);
+      s_bnd[0] = s_bnd[1] = s_bnd[2] = 0u;
+    }
+    __syncthreads();
+    const uint32_t work = s_work;
+    if (work >= n_active) break;
+    const uint32_t tile = active_list[work];
+    const SceneDev& S = scenes[tile_scene[tile]];
+    const uint32_t r_lo = tile_lo[tile], r_hi = tile_hi[tile];
+    const uint32_t count = r_hi - r_lo;
+    // bound pre-pass over the tile's particles
+    {
+      float bm = 0.f, bp = 0.f, bf = 0.f;
+      for (uint32_t i = r_lo + t; i < r_hi; i += NT) {
+        const uint32_t p = vals[i];
+        const P2GBound b = particle_bound(rec[4 * size_t(p)], rec[4 * size_t(p) + 1], rec[4 * size_t(p) + 2],
+                                          rec[4 * size_t(p) + 3], skip_heavy);
+        bm = fmaxf(bm, b.m); bp = fmaxf(bp, b.p); bf = fmaxf(bf, b.f);
+      }
+      // non-negative floats order like their bit patterns
+      atomicMax(&s_bnd[0], __float_as_uint(bm));
+      atomicMax(&s_bnd[1], __float_as_uint(bp));
+      atomicMax(&s_bnd[2], __float_as_uint(bf));
+    }
+    for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0ull;
+    __syncthreads();
+    const int em = fixed_exponent(__uint_as_float(s_bnd[0]), count);
+    const int ep = fixed_exponent(__uint_as_float(s_bnd[1]), count);
+    const int ef = fixed_exponent(__uint_as_float(s_bnd[2]), count);
+    const float sm = ldexpf(1.0f, em), sp = ldexpf(1.0f, ep), sf = ldexpf(1.0f, ef);
+    const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
 
-  for (int cl = wave; cl < TILE_CELLS; cl += P2G_WAVES) {
-    const uint32_t b = __builtin_amdgcn_readfirstlane(s_cs[cl]);
-    const uint32_t e = __builtin_amdgcn_readfirstlane(s_ce[cl]);
-    if (b >= e) continue;
-    const int cx = cl & 3, cy = (cl >> 2) & 3, cz = cl >> 4;
+    // this wave's contiguous share of the tile's particles
+    const uint32_t share = (count + P2G_WAVES - 1) / P2G_WAVES;
+    const uint32_t w_lo = min(r_hi, r_lo + uint32_t(wave) * share), w_hi = min(r_hi, w_lo + share);
+    int cur = -1;  // local cell of the register sums
     float am = 0.f, apx = 0.f, apy = 0.f, apz = 0.f, afx = 0.f, afy = 0.f, afz = 0.f;
-    for (uint32_t c0 = b; c0 < e; c0 += P2G_CHUNK) {
-      const uint32_t cnt = min(uint32_t(P2G_CHUNK), e - c0);
+    auto commit = [&]() {
+      const int cx = cur & 3, cy = (cur >> 2) & 3, cz = cur >> 4;
+      const int jn = (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
+      atomicAdd(&s_acc[0 * HALO_NODES + jn], (unsigned long long)llrintf(am * sm));
+      atomicAdd(&s_acc[1 * HALO_NODES + jn], (unsigned long long)llrintf(apx * sp));
+      atomicAdd(&s_acc[2 * HALO_NODES + jn], (unsigned long long)llrintf(apy * sp));
+      atomicAdd(&s_acc[3 * HALO_NODES + jn], (unsigned long long)llrintf(apz * sp));
+      atomicAdd(&s_acc[4 * HALO_NODES + jn], (unsigned long long)llrintf(afx * sf));
+      atomicAdd(&s_acc[5 * HALO_NODES + jn], (unsigned long long)llrintf(afy * sf));
+      atomicAdd(&s_acc[6 * HALO_NODES + jn], (unsigned long long)llrintf(afz * sf));
+      am = apx = apy = apz = afx = afy = afz = 0.f;
+    };
+    for (uint32_t c0 = w_lo; c0 < w_hi; c0 += P2G_CHUNK) {
+      const uint32_t cnt = min(uint32_t(P2G_CHUNK), w_hi - c0);
       // stage: lane k < cnt loads particle c0+k and evaluates its weights once
       if (uint32_t(lane) < cnt) {
         const uint32_t p = vals[c0 + lane];
+        const uint32_t cell = keys[c0 + lane] % TILE_CELLS;
         const float4 r0 = rec[4 * size_t(p)];
         const float4 r1 = rec[4 * size_t(p) + 1];
         const float4 r2 = rec[4 * size_t(p) + 2];
@@ -321,10 +322,10 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
         // p2gTransfer.glsl:64-66 skips |m| > 100 (gMassCompute does not)
         const float use = (skip_heavy && fabsf(r0.w) > 100.0f) ? 0.0f : 1.0f;
         const float m = r0.w * use;
+        // every particle of a cell has base = floor(u) = that cell; f = u - base
         const float ux = fmaf(r0.x, inv_h, uo0), uy = fmaf(r0.y, inv_h, uo1), uz = fmaf(r0.z, inv_h, uo2);
         const float fx = ux - floorf(ux), fy = uy - floorf(uy), fz = uz - floorf(uz);
-        float* d = stage + lane * P2G_STAGE;
-        float4* d4 = reinterpret_cast<float4*>(d);
+        float4* d4 = reinterpret_cast<float4*>(stage + lane * P2G_STAGE);
         d4[0] = make_float4(spline_w(0, fx), spline_q(0, fx), spline_w(1, fx), spline_q(1, fx));
         d4[1] = make_float4(spline_w(2, fx), spline_q(2, fx), spline_w(3, fx), spline_q(3, fx));
         d4[2] = make_float4(spline_w(0, fy), spline_q(0, fy), spline_w(1, fy), spline_q(1, fy));
@@ -332,14 +333,19 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
         d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
         d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
         d4[6] = make_float4(m, r1.w * m, r2.w * m, r3.w * m);
-        d4[7] = make_float4(r1.x * use, r1.y * use, r1.z * use, r2.x * use);
-        d4[8] = make_float4(r2.y * use, r2.z * use, r3.x * use, r3.y * use);
-        d4[9] = make_float4(r3.z * use, 0.f, 0.f, 0.f);
+        d4[7] = make_float4(-r1.x * use, -r1.y * use, -r1.z * use, -r2.x * use);
+        d4[8] = make_float4(-r2.y * use, -r2.z * use, -r3.x * use, -r3.y * use);
+        d4[9] = make_float4(-r3.z * use, __uint_as_float(cell), 0.f, 0.f);
       }
       // the wave's own LDS writes complete before its reads (in-order LDS per wave)
       __builtin_amdgcn_wave_barrier();
       for (uint32_t k = 0; k < cnt; ++k) {
         const float* d = stage + k * P2G_STAGE;
+        const int cell = int(__builtin_amdgcn_readfirstlane(__float_as_uint(d[37])));
+        if (cell != cur) {
+          if (cur >= 0) commit();
+          cur = cell;
+        }
         const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
         const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
         const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
@@ -354,29 +360,21 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
         apx = fmaf(u0.y, w, apx);
         apy = fmaf(u0.z, w, apy);
         apz = fmaf(u0.w, w, apz);
-        afx -= u1.x * g0 + u1.y * g1 + u1.z * g2;
-        afy -= u1.w * g0 + u2.x * g1 + u2.y * g2;
-        afz -= u2.z * g0 + u2.w * g1 + a22 * g2;
+        afx = fmaf(u1.x, g0, fmaf(u1.y, g1, fmaf(u1.z, g2, afx)));
+        afy = fmaf(u1.w, g0, fmaf(u2.x, g1, fmaf(u2.y, g2, afy)));
+        afz = fmaf(u2.z, g0, fmaf(u2.w, g1, fmaf(a22, g2, afz)));
       }
       __builtin_amdgcn_wave_barrier();
     }
-    const int jn = (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
-    atomicAdd(&s_acc[0 * HALO_NODES + jn], (unsigned long long)llrintf(am * sm));
-    atomicAdd(&s_acc[1 * HALO_NODES + jn], (unsigned long long)llrintf(apx * sp));
-    atomicAdd(&s_acc[2 * HALO_NODES + jn], (unsigned long long)llrintf(apy * sp));
-    atomicAdd(&s_acc[3 * HALO_NODES + jn], (unsigned long long)llrintf(apz * sp));
-    atomicAdd(&s_acc[4 * HALO_NODES + jn], (unsigned long long)llrintf(afx * sf));
-    atomicAdd(&s_acc[5 * HALO_NODES + jn], (unsigned long long)llrintf(afy * sf));
-    atomicAdd(&s_acc[6 * HALO_NODES + jn], (unsigned long long)llrintf(afz * sf));
-  }
-  __syncthreads();
-  float* out = tpart + size_t(tile) * (NQ * HALO_NODES);
-  for (int i = t; i < NQ * HALO_NODES; i += NT) {
-    const int q = i / HALO_NODES;
-    const int e = q == 0 ? em : (q < 4 ? ep : ef);
-    out[i] = ldexpf(float((long long)s_acc[i]), -e);
-  }
-  __syncthreads();
+    if (cur >= 0) commit();
+    __syncthreads();
+    float* out = tpart + size_t(tile) * (NQ * HALO_NODES);
+    for (int i = t; i < NQ * HALO_NODES; i += NT) {
+      const int q = i / HALO_NODES;
+      const int e = q == 0 ? em : (q < 4 ? ep : ef);
+      out[i] = ldexpf(float((long long)s_acc[i]), -e);
+    }
+    __syncthreads();
   }
 }
 

@@ -284,3 +284,64 @@ def test_mixed_precision_not_worse(oracle_mod):
         errs[flags] = (np.abs(g["x"] - o.x).max(), np.abs(g["fe"] - o.fe).max(), np.abs(g["fp"] - o.fp).max())
     print("\nplain", errs[0], "mixed", errs[FLAG_MIXED])
     assert errs[FLAG_MIXED][0] <= 2 * errs[0][0] + 1e-7
+
+
+def _sweep_small():
+    sc = scenes.parameter_sweep(num_scenes=6, nmin=2000, nmax=20000, dims_choices=(32, 48))
+    return sc
+
+
+def test_graph_replay_matches_eager():
+    """Replays of the captured step equal eager batched steps bit for bit, with eager work in between."""
+    sc = _sweep_small()
+    with SnowContext(sc) as a, SnowContext(sc) as b:
+        a.init()
+        b.init()
+        for _ in range(7):
+            a.step()
+        b.run_graph(2)
+        b.step()
+        b.run_graph(3)
+        b.sync()
+        b.run_graph(1)
+        for i in range(len(sc)):
+            pa, pb = a.read_particles(i), b.read_particles(i)
+            for k in ("x", "v", "fe", "fp"):
+                assert np.array_equal(pa[k], pb[k]), (i, k)
+
+
+def test_autotune_keeps_state_and_results():
+    sc = _sweep_small()
+    with SnowContext(sc) as a, SnowContext(sc) as b:
+        a.init()
+        b.init()
+        for ctx in (a, b):
+            ctx.step()
+        before = b.read_particles(2)
+        shape = b.autotune(steps=2)
+        assert shape["g2p_block"] in (64, 128, 256)
+        after = b.read_particles(2)
+        for k in ("x", "v", "fe", "fp"):
+            assert np.array_equal(before[k], after[k]), k
+        for _ in range(3):
+            a.step()
+        b.run_graph(3)
+        for i in range(len(sc)):
+            pa, pb = a.read_particles(i), b.read_particles(i)
+            for k in ("x", "v", "fe", "fp"):
+                assert np.array_equal(pa[k], pb[k]), (i, k)
+
+
+def test_batched_sweep_matches_oracle(oracle_mod):
+    """Per-scene parameters and dt in one batch: each scene agrees with its own FP64 oracle."""
+    sc = _sweep_small()[:3]
+    with SnowContext(sc) as ctx:
+        ctx.init()
+        ctx.run_graph(10)
+        got = [ctx.read_particles(i) for i in range(len(sc))]
+    for s, g in zip(sc, got):
+        o = oracle_mod.Oracle(s, nthreads=8)
+        o.init()
+        o.step(s.dt, 10)
+        assert np.abs(g["x"] - o.x).max() < 1e-6
+        assert rel_err(g["v"], o.v) < 1e-4
