@@ -38,7 +38,7 @@ BYTES = {
     "#Sim:Transfer to Particles": (192, 24),
 }
 KERNEL_NAMES = {
-    "#Sim:Transfer to Grid": "k_p2g_tile",
+    "#Sim:Transfer to Grid": "k_p2g_chunk",
     "#Sim:Compute Derivatives": "k_grid",
     "#Sim:Transfer to Particles": "k_g2p_advance",
 }

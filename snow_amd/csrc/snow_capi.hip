@@ -84,7 +84,9 @@ struct snow_ctx {
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   uint32_t *d_tile_lo = nullptr, *d_tile_hi = nullptr;
-  float* d_tpart = nullptr;
+  double* d_tpart = nullptr;
+  double* d_hpart = nullptr;
+  double* d_zero_part = nullptr;  // NQ * HALO_NODES zeros
   StepCtl* d_ctl = nullptr;
   uint32_t* d_tile_stamp = nullptr;
   uint32_t* d_active_list = nullptr;
@@ -118,7 +120,8 @@ struct snow_ctx {
 
   // launch shapes (autotuned)
   int g2p_block = 256;
-  int p2g_per_cu = 2;
+  uint32_t total_chunks = 0;       // P2G workgroups (chunks of every scene)
+  uint16_t* d_chunk_scene = nullptr;
   int grid_per_cu = 32;
 };
 
@@ -166,7 +169,8 @@ hipEvent_t take_event(snow_ctx* c) {
     return e;
   }
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  // timing-only events: no system-scope fence (cache writeback) between the timed kernels
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
   return e;
 }
 
@@ -288,7 +292,7 @@ int run_sort(snow_ctx* c, float dt_rigid) {
                                c->key_bits, c->stream));
   }
   if (c->N) {
-    k_tile_bounds<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_keys[1], c->N, c->d_tile_lo, c->d_tile_hi,
+    k_tile_bounds<<<grid_for((c->N + 3) / 4, 256), 256, 0, c->stream>>>(c->d_keys[1], c->N, c->d_tile_lo, c->d_tile_hi,
                                                                c->d_scenes, c->d_tile_scene, c->d_ctl,
                                                                c->d_tile_stamp, c->d_active_list);
     LAUNCHCHK(c);
@@ -298,11 +302,15 @@ int run_sort(snow_ctx* c, float dt_rigid) {
 
 int run_p2g(snow_ctx* c, int mass_only) {
   StageTimer tm(c, ST_P2G);
-  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu * c->p2g_per_cu)));
-  k_p2g_tile<<<blocks, 64 * P2G_WAVES, 0, c->stream>>>(c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1],
-                                                       c->d_tile_lo, c->d_tile_hi, c->d_rec[c->cur], c->d_tpart,
-                                                       c->d_ctl, c->d_active_list, mass_only ? 0 : 1);
-  LAUNCHCHK(c);
+  if (c->total_chunks) {
+    k_p2g_chunk<<<c->total_chunks, 64 * P2G_WAVES, 0, c->stream>>>(
+        c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1], c->d_tile_lo, c->d_tile_hi, c->d_rec[c->cur],
+        c->d_tpart, c->d_hpart, c->d_chunk_scene, mass_only ? 0 : 1);
+    LAUNCHCHK(c);
+    k_fold_partials<<<c->total_chunks, 256, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
+                                                            c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart);
+    LAUNCHCHK(c);
+  }
   return tm.done();
 }
 
@@ -311,8 +319,8 @@ int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
   unsigned long long* cnt = c->d_counts;
   const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles * 8u, unsigned(c->num_cu * c->grid_per_cu)));
 #define GRID_ARGS                                                                                          \
-  c->d_scenes, c->d_tile_scene, c->d_ctl, c->d_active_list, c->d_tile_stamp, c->d_tpart, c->d_g_mmv, c->d_g_f, \
-      c->d_g_node, c->d_cols, dt, cnt
+  c->d_scenes, c->d_tile_scene, c->d_ctl, c->d_active_list, c->d_tile_stamp, c->d_tpart, c->d_zero_part,     \
+      c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_cols, dt, cnt
   if (keep && update)
     k_grid<true, true><<<blocks, 64, 0, c->stream>>>(GRID_ARGS);
   else if (update)
@@ -375,7 +383,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_sort_tmp,
-                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_ctl, c->d_tile_stamp, c->d_active_list, c->d_g_mmv, c->d_g_f, c->d_g_node,
+                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_chunk_scene, c->d_ctl, c->d_tile_stamp, c->d_active_list, c->d_g_mmv, c->d_g_f, c->d_g_node,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -446,7 +454,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   }
 
   // scene table, tile index space
-  uint64_t ct_off = 0, nt_off = 0;
+  uint64_t ct_off = 0, nt_off = 0, chunk_off = 0;
   int64_t node_off = 0, p_off = 0;
   int col_off = 0;
   for (int s = 0; s < num_scenes; ++s) {
@@ -477,6 +485,16 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     S.p_cnt = d.num_particles;
     S.inv_h = 1.0f / d.grid.spacing;
     for (int k = 0; k < 3; ++k) S.uo[k] = -d.grid.origin[k] / d.grid.spacing;
+    // P2G chunk: a power of two from 256 to P2G_SEG_MAX giving at least 64 chunks where it can
+    // (small scenes alone still fill many workgroups, big ones keep few tile splits); the P2G
+    // rounding depends on it, so it depends on the scene's particle count only
+    {
+      uint32_t seg = 256;
+      while (seg < uint32_t(P2G_SEG_MAX) && uint64_t(seg) * 2 * 64 <= uint64_t(d.num_particles)) seg *= 2;
+      S.seg = seg;
+      S.chunk_off = uint32_t(chunk_off);
+      chunk_off += (uint64_t(d.num_particles) + seg - 1) / seg;
+    }
     S.mu0 = d.material.young / (2.0f * (1.0f + d.material.poisson));
     S.lambda0 = d.material.young * d.material.poisson / ((1.0f + d.material.poisson) * (1.0f - 2.0f * d.material.poisson));
     ct_off += nct + 1;
@@ -497,6 +515,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     return fail(nullptr, SNOW_E_CAPACITY, "batch grid too large for 32-bit cell keys");
   }
   c->total_ctiles = uint32_t(ct_off);
+  c->total_chunks = uint32_t(chunk_off);
   c->total_ntiles = uint32_t(nt_off);
   c->total_nodes = node_off;
   {
@@ -544,6 +563,10 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_tile_lo, c->total_ctiles));
   CREATECHK(dalloc(&c->d_tile_hi, c->total_ctiles));
   CREATECHK(dalloc(&c->d_tpart, size_t(c->total_ctiles) * NQ * HALO_NODES));
+  CREATECHK(dalloc(&c->d_hpart, size_t(std::max<uint32_t>(1, c->total_chunks)) * NQ * HALO_NODES));
+  CREATECHK(dalloc(&c->d_chunk_scene, std::max<uint32_t>(1, c->total_chunks)));
+  CREATECHK(dalloc(&c->d_zero_part, size_t(NQ) * HALO_NODES));
+  CREATECHK(hipMemset(c->d_zero_part, 0, sizeof(double) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_ctl, 1));
   CREATECHK(dalloc(&c->d_tile_stamp, c->total_ctiles));
   CREATECHK(dalloc(&c->d_active_list, c->total_ctiles));
@@ -568,6 +591,13 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     CREATECHK(hipMemcpy(c->d_scenes, c->scenes.data(), sizeof(SceneDev) * c->scenes.size(), hipMemcpyHostToDevice));
     CREATECHK(hipMemcpy(c->d_pscene, ps.data(), sizeof(uint16_t) * N, hipMemcpyHostToDevice));
     CREATECHK(hipMemcpy(c->d_tile_scene, ts.data(), sizeof(uint16_t) * ts.size(), hipMemcpyHostToDevice));
+    std::vector<uint16_t> cs(std::max<uint32_t>(1, c->total_chunks), 0);
+    for (int s = 0; s < num_scenes; ++s) {
+      const SceneDev& S = c->scenes[s];
+      const uint32_t nch = uint32_t((uint64_t(S.p_cnt) + S.seg - 1) / S.seg);
+      for (uint32_t k = 0; k < nch; ++k) cs[S.chunk_off + k] = uint16_t(s);
+    }
+    CREATECHK(hipMemcpy(c->d_chunk_scene, cs.data(), sizeof(uint16_t) * cs.size(), hipMemcpyHostToDevice));
     CREATECHK(hipMemcpy(c->d_ntile_scene, nts.data(), sizeof(uint16_t) * nts.size(), hipMemcpyHostToDevice));
     if (!c->cols.empty())
       CREATECHK(hipMemcpy(c->d_cols, c->cols.data(), sizeof(Collider) * c->cols.size(), hipMemcpyHostToDevice));
@@ -746,7 +776,7 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
   return 0;
 }
 
-int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_per_cu, int32_t* grid_per_cu) {
+int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_chunk, int32_t* grid_per_cu) {
   if (!c || steps <= 0) return SNOW_E_ARG;
   if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before autotuning");
   for (const auto& S : c->scenes)
@@ -799,11 +829,10 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_p
     ms = f;
     return 0;
   };
-  // coordinate descent over the three launch shapes
+  // coordinate descent over the launch shapes that leave the results unchanged
   const int g2p_cands[] = {64, 128, 256};
-  const int p2g_cands[] = {1, 2};
   const int grid_cands[] = {8, 32};
-  int best_g = c->g2p_block, best_p = c->p2g_per_cu, best_q = c->grid_per_cu;
+  int best_g = c->g2p_block, best_q = c->grid_per_cu;
   double best = 1e30, t = 0.0;
   for (int g : g2p_cands) {
     c->g2p_block = g;
@@ -811,13 +840,6 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_p
     if (t < best) { best = t; best_g = g; }
   }
   c->g2p_block = best_g;
-  for (int pp : p2g_cands) {
-    if (rc) break;
-    c->p2g_per_cu = pp;
-    if ((rc = trial(t))) break;
-    if (t < best) { best = t; best_p = pp; }
-  }
-  c->p2g_per_cu = best_p;
   for (int q : grid_cands) {
     if (rc) break;
     c->grid_per_cu = q;
@@ -838,7 +860,7 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_p
     c->graph[p] = nullptr;
   }
   if (g2p_block) *g2p_block = c->g2p_block;
-  if (p2g_per_cu) *p2g_per_cu = c->p2g_per_cu;
+  if (p2g_chunk) *p2g_chunk = c->scenes.empty() ? 0 : int32_t(c->scenes[0].seg);
   if (grid_per_cu) *grid_per_cu = c->grid_per_cu;
   return 0;
 }

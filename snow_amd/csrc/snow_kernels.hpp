@@ -39,13 +39,13 @@ struct SceneDev {
   int nt[3];          // node tiles per axis
   uint32_t nt_off;    // global index of the scene's first node tile
   uint32_t dead_tile; // global index of the scene's out-of-grid tile
-  uint32_t pad0;
+  uint32_t seg;       // P2G chunk of the scene, in particles
   int64_t node_off;   // offset of node 0 in the dense node arrays
   int64_t p_off;      // first particle
   int64_t p_cnt;      // particle count
   float mu0, lambda0; // E/(2(1+nu)), E nu/((1+nu)(1-2nu))
   float inv_h;        // 1/h
-  float pad1;
+  uint32_t chunk_off; // global index of the scene's first P2G chunk
   float uo[3];        // -origin/h: grid coordinate u = x/h + uo
   float pad2;
 };
@@ -121,7 +121,7 @@ __device__ __forceinline__ void store_node(float4* __restrict__ g, int64_t n, fl
 struct StepCtl {
   uint32_t step;      // step stamp, incremented by k_step_begin
   uint32_t n_active;  // active cell tiles this step (entries of active_list)
-  uint32_t p2g_next;  // work counter of the persistent P2G kernel
+  uint32_t spare;
   uint32_t pad;
 };
 
@@ -134,14 +134,14 @@ __global__ void __launch_bounds__(256) k_step_begin(StepCtl* __restrict__ ctl, u
   if (t == 0) {
     ctl->step += 1u;
     ctl->n_active = 0u;
-    ctl->p2g_next = 0u;
+    ctl->spare = 0u;
   }
   for (int i = t; i < ncounts; i += blockDim.x) counts[i] = 0ull;
   if (dt_rigid >= 0.0f) {
-    for (int s = 0; s < nscenes; ++s) {
+    for (int s = t; s < nscenes; s += blockDim.x) {
       const SceneDev& S = scenes[s];
       const float dt = dt_rigid > 0.0f ? dt_rigid : S.dt;
-      for (int c = t; c < S.ncol; c += blockDim.x) {
+      for (int c = 0; c < S.ncol; ++c) {
         Collider& C = cols[S.col_off + c];
         C.px += dt * C.vx; C.py += dt * C.vy; C.pz += dt * C.vz;
       }
@@ -166,49 +166,84 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
 // the list of active cell tiles (stamped with this step). Entries of tiles without particles
 // are left stale; only listed (stamped) tiles are read. The order of active_list is not
 // deterministic, but nothing computed from it depends on that order.
+// Each thread handles 4 consecutive keys; its neighbours' keys come from two extra loads.
 __global__ void __launch_bounds__(256) k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n,
                                                      uint32_t* __restrict__ tile_lo, uint32_t* __restrict__ tile_hi,
                                                      const SceneDev* __restrict__ scenes,
                                                      const uint16_t* __restrict__ tile_scene, StepCtl* __restrict__ ctl,
                                                      uint32_t* __restrict__ tile_stamp,
                                                      uint32_t* __restrict__ active_list) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = i < n;
-  const uint32_t tile = in ? keys[i] / TILE_CELLS : 0u;
-  const uint32_t prev = (in && i) ? keys[i - 1] / TILE_CELLS : 0xffffffffu;
-  const uint32_t next = (in && i + 1 < n) ? keys[i + 1] / TILE_CELLS : 0xffffffffu;
-  const bool first = in && (i == 0 || prev != tile);
-  if (in && (i + 1 == n || next != tile)) tile_hi[tile] = i + 1;
-  const bool start = first && scenes[tile_scene[tile]].dead_tile != tile;
-  if (first) tile_lo[tile] = i;
+  const uint32_t i0 = 4u * (blockIdx.x * blockDim.x + threadIdx.x);
+  uint32_t k[4];
+  if (i0 + 4u <= n) {
+    const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+    k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) k[j] = i0 + j < n ? keys[i0 + j] : 0xffffffffu;
+  }
+  const uint32_t prev = (i0 > 0u && i0 <= n) ? keys[i0 - 1u] / TILE_CELLS : 0xffffffffu;
+  const uint32_t next = i0 + 4u < n ? keys[i0 + 4u] / TILE_CELLS : 0xffffffffu;
+  uint32_t tl[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) tl[j] = i0 + j < n ? k[j] / TILE_CELLS : 0xffffffffu;
+  bool start[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t i = i0 + uint32_t(j);
+    const bool in = i < n;
+    const uint32_t before = j == 0 ? prev : tl[j - 1];
+    const uint32_t after = j == 3 ? next : tl[j + 1];
+    const bool first = in && before != tl[j];
+    if (in && after != tl[j]) tile_hi[tl[j]] = i + 1u;
+    if (first) tile_lo[tl[j]] = i;
+    start[j] = first && scenes[tile_scene[tl[j]]].dead_tile != tl[j];
+  }
   // one atomic per wave for the tile-list append
-  const uint64_t ball = __ballot(start);
-  if (!ball) return;
+  uint64_t ball[4];
+  uint32_t total = 0u;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    ball[j] = __ballot(start[j]);
+    total += uint32_t(__popcll(ball[j]));
+  }
+  if (!total) return;
   const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((long long)ball) - 1;
   uint32_t slot0 = 0u;
-  if (lane == leader) slot0 = atomicAdd(&ctl->n_active, uint32_t(__popcll(ball)));
-  slot0 = __shfl(slot0, leader, 64);
-  if (start) {
-    const uint32_t slot = slot0 + uint32_t(__popcll(ball & ((1ull << lane) - 1ull)));
-    active_list[slot] = tile;
-    tile_stamp[tile] = ctl->step;
+  if (lane == 0) slot0 = atomicAdd(&ctl->n_active, total);
+  slot0 = __shfl(slot0, 0, 64);
+  const uint32_t step = ctl->step;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (start[j]) {
+      active_list[slot0 + uint32_t(__popcll(ball[j] & lt))] = tl[j];
+      tile_stamp[tl[j]] = step;
+    }
+    slot0 += uint32_t(__popcll(ball[j]));
   }
 }
 
 // ---------------------------------------------------------------------------------------
-// P2G on one 4^3 cell tile (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84).
-// The tile's particles are contiguous in sorted order (cell by cell); each wave takes an equal
-// contiguous share of them and stages P2G_CHUNK particles at a time into its LDS slot: one lane
-// per particle loads the record and evaluates the particle's 12 weights and 12 gradients once.
-// Then lane l of the wave is the stencil offset o = (l&3, (l>>2)&3, l>>4) (getIJK order,
+// P2G (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84) over chunks of the sorted particles.
+// A scene's particles are contiguous in sorted order; its chunk c (global index chunk_off + c)
+// is the sorted range [p_off + c*seg, p_off + (c+1)*seg), so every workgroup gets about the same
+// number of particles however unevenly they fill the cell tiles, and a scene's chunks are the
+// same alone or in a batch. The chunk falls apart into
+// runs of particles of one 4^3 cell tile (contiguous, cell by cell). For each run the 8 waves
+// take equal contiguous shares and stage P2G_CHUNK particles at a time into their LDS slot: one
+// lane per particle loads the record and evaluates the particle's 12 weights and 12 gradients
+// once. Then lane l of the wave is the stencil offset o = (l&3, (l>>2)&3, l>>4) (getIJK order,
 // neighborIndex.include.glsl:6-12): for each staged particle the 64 lanes compute the 64 node
 // contributions from broadcast LDS reads and sum them in registers. When the cell changes, the
-// register sums are added to the tile's LDS accumulator as 64-bit fixed point: integer addition
-// is associative, so the tile sums are exact whatever the order of the commits, and the result
-// is bit-reproducible. The fixed-point scale is a power of two per tile and quantity,
-// 2^61 / (count * max |per-particle term|), from a vectorised pre-pass over the tile's
-// particles, so no sum can overflow and the resolution is ~2^-40 of the largest term.
+// register sums are added to the run's LDS accumulator as 64-bit fixed point: integer addition
+// is associative, so the run sums are exact whatever the order of the commits, and the result
+// is bit-reproducible. The fixed-point scale is a power of two per run and quantity,
+// 2^61 / (count * max |per-particle term|), from a pre-pass over the run's particles, so no sum
+// can overflow and the resolution is ~2^-40 of the largest term.
+// A run that starts its tile writes the tile partial tpart[tile]; a run that continues a tile
+// begun in an earlier chunk is the chunk's first run and writes hpart[chunk]. The grid merge
+// adds tpart[tile] and the hpart of the tile's later chunks in chunk order.
 struct P2GBound { float m, p, f; };
 
 __device__ __forceinline__ P2GBound particle_bound(const float4& r0, const float4& r1, const float4& r2,
@@ -237,40 +272,60 @@ __device__ __forceinline__ int fixed_exponent(float maxval, uint32_t count) {
 
 constexpr int P2G_CHUNK = 32;        // particles staged per wave at a time
 constexpr int P2G_STAGE = 40;        // floats per staged particle
+constexpr int P2G_SEG_MAX = 2048;    // largest chunk (sizes the LDS run list)
 
-// Persistent: each workgroup pulls active tiles from active_list until the list is exhausted.
-__global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
+__global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     const uint32_t* __restrict__ tile_lo, const uint32_t* __restrict__ tile_hi,
-    const float4* __restrict__ rec, float* __restrict__ tpart, StepCtl* __restrict__ ctl,
-    const uint32_t* __restrict__ active_list, int skip_heavy) {
+    const float4* __restrict__ rec, double* __restrict__ tpart, double* __restrict__ hpart,
+    const uint16_t* __restrict__ chunk_scene, int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   __shared__ unsigned long long s_acc[NQ * HALO_NODES];
   // per-wave staging of up to P2G_CHUNK particles: for each axis the interleaved (w_o, q_o),
-  // o = 0..3 (24 floats), then m, m*v (3), -A row-major (9), the local cell index, 2 pad
+  // o = 0..3 (24 floats), then m, m*v (3), A row-major (9), 3 pad
   __shared__ __attribute__((aligned(16))) float s_stage[P2G_WAVES][P2G_CHUNK * P2G_STAGE];
-  __shared__ uint32_t s_bnd[3];
-  __shared__ uint32_t s_work;
+  __shared__ uint32_t s_run[P2G_SEG_MAX];
+  __shared__ uint32_t s_bnd[2][3];
+  __shared__ uint32_t s_nrun;
 
   const int t = threadIdx.x;
-  const int wave = t >> 6, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int lane = t & 63;
   const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
   float* stage = s_stage[wave];
-  const uint32_t n_active = ctl->n_active;
-  for (;;) {
-    if (t == 0) {
-      s_work = atomicAdd(&ctl->p2g_next, 1u);
-      s_bnd[0] = s_bnd[1] = s_bnd[2] = 0u;
-    }
-    __syncthreads();
-    const uint32_t work = s_work;
-    if (work >= n_active) break;
-    const uint32_t tile = active_list[work];
+  const uint32_t chunk = blockIdx.x;
+  uint32_t lo, hi;
+  {
+    const SceneDev& S = scenes[chunk_scene[chunk]];
+    lo = uint32_t(S.p_off) + (chunk - S.chunk_off) * S.seg;
+    hi = min(uint32_t(S.p_off + S.p_cnt), lo + S.seg);
+  }
+
+  // run starts of the chunk (unordered: every run is processed on its own)
+  if (t == 0) s_nrun = 0u;
+  if (t < 6) (&s_bnd[0][0])[t] = 0u;
+  __syncthreads();
+  for (uint32_t i = lo + t; i < hi; i += NT) {
+    const uint32_t tl = keys[i] / TILE_CELLS;
+    if (i == lo || keys[i - 1] / TILE_CELLS != tl) s_run[atomicAdd(&s_nrun, 1u)] = i;
+  }
+  __syncthreads();
+  const uint32_t nrun = s_nrun;
+
+  for (uint32_t r = 0; r < nrun; ++r) {
+    const uint32_t r_lo = s_run[r];
+    const uint32_t tile = keys[r_lo] / TILE_CELLS;
     const SceneDev& S = scenes[tile_scene[tile]];
-    const uint32_t r_lo = tile_lo[tile], r_hi = tile_hi[tile];
+    if (S.dead_tile == tile) {  // out-of-grid particles (uniform branch); keep the slot rotation
+      if (t < 3) s_bnd[(r & 1u) ^ 1u][t] = 0u;
+      __syncthreads();
+      continue;
+    }
+    const uint32_t r_hi = min(hi, tile_hi[tile]);
     const uint32_t count = r_hi - r_lo;
-    // bound pre-pass over the tile's particles
+    uint32_t* bnd = s_bnd[r & 1u];
+    // bound pre-pass over the run's particles
     {
       float bm = 0.f, bp = 0.f, bf = 0.f;
       for (uint32_t i = r_lo + t; i < r_hi; i += NT) {
@@ -279,20 +334,30 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
                                           rec[4 * size_t(p) + 3], skip_heavy);
         bm = fmaxf(bm, b.m); bp = fmaxf(bp, b.p); bf = fmaxf(bf, b.f);
       }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        bm = fmaxf(bm, __shfl_xor(bm, off, 64));
+        bp = fmaxf(bp, __shfl_xor(bp, off, 64));
+        bf = fmaxf(bf, __shfl_xor(bf, off, 64));
+      }
       // non-negative floats order like their bit patterns
-      atomicMax(&s_bnd[0], __float_as_uint(bm));
-      atomicMax(&s_bnd[1], __float_as_uint(bp));
-      atomicMax(&s_bnd[2], __float_as_uint(bf));
+      if (lane == 0) {
+        atomicMax(&bnd[0], __float_as_uint(bm));
+        atomicMax(&bnd[1], __float_as_uint(bp));
+        atomicMax(&bnd[2], __float_as_uint(bf));
+      }
     }
     for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0ull;
+    // the other bound slot was read by everyone before the previous run's last barrier
+    if (t < 3) s_bnd[(r & 1u) ^ 1u][t] = 0u;
     __syncthreads();
-    const int em = fixed_exponent(__uint_as_float(s_bnd[0]), count);
-    const int ep = fixed_exponent(__uint_as_float(s_bnd[1]), count);
-    const int ef = fixed_exponent(__uint_as_float(s_bnd[2]), count);
+    const int em = fixed_exponent(__uint_as_float(bnd[0]), count);
+    const int ep = fixed_exponent(__uint_as_float(bnd[1]), count);
+    const int ef = fixed_exponent(__uint_as_float(bnd[2]), count);
     const float sm = ldexpf(1.0f, em), sp = ldexpf(1.0f, ep), sf = ldexpf(1.0f, ef);
     const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
 
-    // this wave's contiguous share of the tile's particles
+    // this wave's contiguous share of the run
     const uint32_t share = (count + P2G_WAVES - 1) / P2G_WAVES;
     const uint32_t w_lo = min(r_hi, r_lo + uint32_t(wave) * share), w_hi = min(r_hi, w_lo + share);
     int cur = -1;  // local cell of the register sums
@@ -300,21 +365,23 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
     auto commit = [&]() {
       const int cx = cur & 3, cy = (cur >> 2) & 3, cz = cur >> 4;
       const int jn = (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
+      // the force sums hold +A grad(w); the node force is -A grad(w)
       atomicAdd(&s_acc[0 * HALO_NODES + jn], (unsigned long long)llrintf(am * sm));
       atomicAdd(&s_acc[1 * HALO_NODES + jn], (unsigned long long)llrintf(apx * sp));
       atomicAdd(&s_acc[2 * HALO_NODES + jn], (unsigned long long)llrintf(apy * sp));
       atomicAdd(&s_acc[3 * HALO_NODES + jn], (unsigned long long)llrintf(apz * sp));
-      atomicAdd(&s_acc[4 * HALO_NODES + jn], (unsigned long long)llrintf(afx * sf));
-      atomicAdd(&s_acc[5 * HALO_NODES + jn], (unsigned long long)llrintf(afy * sf));
-      atomicAdd(&s_acc[6 * HALO_NODES + jn], (unsigned long long)llrintf(afz * sf));
+      atomicAdd(&s_acc[4 * HALO_NODES + jn], (unsigned long long)llrintf(-afx * sf));
+      atomicAdd(&s_acc[5 * HALO_NODES + jn], (unsigned long long)llrintf(-afy * sf));
+      atomicAdd(&s_acc[6 * HALO_NODES + jn], (unsigned long long)llrintf(-afz * sf));
       am = apx = apy = apz = afx = afy = afz = 0.f;
     };
     for (uint32_t c0 = w_lo; c0 < w_hi; c0 += P2G_CHUNK) {
-      const uint32_t cnt = min(uint32_t(P2G_CHUNK), w_hi - c0);
+      const uint32_t cnt = __builtin_amdgcn_readfirstlane(min(uint32_t(P2G_CHUNK), w_hi - c0));
       // stage: lane k < cnt loads particle c0+k and evaluates its weights once
+      uint32_t cellv = 0u;
       if (uint32_t(lane) < cnt) {
         const uint32_t p = vals[c0 + lane];
-        const uint32_t cell = keys[c0 + lane] % TILE_CELLS;
+        cellv = keys[c0 + lane] % TILE_CELLS;
         const float4 r0 = rec[4 * size_t(p)];
         const float4 r1 = rec[4 * size_t(p) + 1];
         const float4 r2 = rec[4 * size_t(p) + 2];
@@ -333,19 +400,19 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
         d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
         d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
         d4[6] = make_float4(m, r1.w * m, r2.w * m, r3.w * m);
-        d4[7] = make_float4(-r1.x * use, -r1.y * use, -r1.z * use, -r2.x * use);
-        d4[8] = make_float4(-r2.y * use, -r2.z * use, -r3.x * use, -r3.y * use);
-        d4[9] = make_float4(-r3.z * use, __uint_as_float(cell), 0.f, 0.f);
+        d4[7] = make_float4(r1.x * use, r1.y * use, r1.z * use, r2.x * use);
+        d4[8] = make_float4(r2.y * use, r2.z * use, r3.x * use, r3.y * use);
+        d4[9] = make_float4(r3.z * use, 0.f, 0.f, 0.f);
       }
       // the wave's own LDS writes complete before its reads (in-order LDS per wave)
       __builtin_amdgcn_wave_barrier();
       for (uint32_t k = 0; k < cnt; ++k) {
-        const float* d = stage + k * P2G_STAGE;
-        const int cell = int(__builtin_amdgcn_readfirstlane(__float_as_uint(d[37])));
+        const int cell = int(__builtin_amdgcn_readlane(cellv, int(k)));
         if (cell != cur) {
           if (cur >= 0) commit();
           cur = cell;
         }
+        const float* d = stage + k * P2G_STAGE;
         const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
         const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
         const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
@@ -368,20 +435,51 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_tile(
     }
     if (cur >= 0) commit();
     __syncthreads();
-    float* out = tpart + size_t(tile) * (NQ * HALO_NODES);
+    double* out = (tile_lo[tile] == r_lo ? tpart + size_t(tile) * (NQ * HALO_NODES)
+                                         : hpart + size_t(chunk) * (NQ * HALO_NODES));
     for (int i = t; i < NQ * HALO_NODES; i += NT) {
       const int q = i / HALO_NODES;
       const int e = q == 0 ? em : (q < 4 ? ep : ef);
-      out[i] = ldexpf(float((long long)s_acc[i]), -e);
+      out[i] = ldexp(double((long long)s_acc[i]), -e);  // relative rounding <= 2^-53
     }
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------------------
+// A tile whose particles span several P2G chunks has its first run in tpart[tile] and one
+// continuation partial per later chunk. The block of the tile's last chunk adds the
+// continuations to tpart[tile] in chunk order (double sums of exact fixed-point partials).
+__global__ void __launch_bounds__(256) k_fold_partials(const SceneDev* __restrict__ scenes,
+                                                       const uint16_t* __restrict__ chunk_scene,
+                                                       const uint32_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ tile_lo,
+                                                       const uint32_t* __restrict__ tile_hi,
+                                                       double* __restrict__ tpart,
+                                                       const double* __restrict__ hpart) {
+  const uint32_t chunk = blockIdx.x;
+  const SceneDev& S = scenes[chunk_scene[chunk]];
+  const uint32_t pbase = uint32_t(S.p_off);
+  const uint32_t lo = pbase + (chunk - S.chunk_off) * S.seg;
+  const uint32_t hi = min(uint32_t(S.p_off + S.p_cnt), lo + S.seg);
+  const uint32_t tile = keys[lo] / TILE_CELLS;
+  if (tile == S.dead_tile) return;
+  const uint32_t tlo = tile_lo[tile], thi = tile_hi[tile];
+  if (tlo == lo || thi > hi) return;  // no continuation here, or a later chunk folds
+  const uint32_t c_first = S.chunk_off + (tlo - pbase) / S.seg + 1u;
+  double* dst = tpart + size_t(tile) * (NQ * HALO_NODES);
+  for (int i = threadIdx.x; i < NQ * HALO_NODES; i += blockDim.x) {
+    double acc = dst[i];
+    for (uint32_t c = c_first; c <= chunk; ++c) acc += hpart[size_t(c) * (NQ * HALO_NODES) + i];
+    dst[i] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Merge of the tile partials into node values + grid update (gForceCompute.glsl:57-103).
 // A node n = 4K + r (per axis) receives from cell tiles K (halo index r + 3) and, when r >= 1,
-// K+1 (halo index r - 1): at most 8 tiles, summed in a fixed order with TwoSum. Work items
+// K+1 (halo index r - 1): at most 8 tiles (each with its continuation partials), summed in a
+// fixed order. Work items
 // are (active cell tile k, offset a in {0,1}^3): node tile K = k - a is processed by the
 // block whose k is the first stamped tile among K + b (b in the fixed order 000,100,010,...),
 // so every node tile reached by a particle stencil is processed exactly once.
@@ -397,7 +495,8 @@ __global__ void __launch_bounds__(64) k_grid(const SceneDev* __restrict__ scenes
                                              const StepCtl* __restrict__ ctl,
                                              const uint32_t* __restrict__ active_list,
                                              const uint32_t* __restrict__ tile_stamp,
-                                             const float* __restrict__ tpart,
+                                             const double* __restrict__ tpart,
+                                             const double* __restrict__ zero_part,
                                              float4* __restrict__ g_mmv, float4* __restrict__ g_f,
                                              float4* __restrict__ g_node,
                                              const Collider* __restrict__ cols, float dt_override,
@@ -417,32 +516,40 @@ __global__ void __launch_bounds__(64) k_grid(const SceneDev* __restrict__ scenes
     const int Kx = kx - (a & 1), Ky = ky - ((a >> 1) & 1), Kz = kz - (a >> 2);
     if (Kx < 0 || Ky < 0 || Kz < 0 || Kx >= S.nt[0] || Ky >= S.nt[1] || Kz >= S.nt[2]) continue;
     // owner: first live tile among K + b, b = 0..7 in order
+    bool live[8];
     int owner = -1;
 #pragma unroll
-    for (int bb = 0; bb < 8; ++bb)
-      if (owner < 0 && tile_live(S, tile_stamp, step, Kx + (bb & 1), Ky + ((bb >> 1) & 1), Kz + (bb >> 2))) owner = bb;
+    for (int bb = 0; bb < 8; ++bb) {
+      live[bb] = tile_live(S, tile_stamp, step, Kx + (bb & 1), Ky + ((bb >> 1) & 1), Kz + (bb >> 2));
+      if (owner < 0 && live[bb]) owner = bb;
+    }
     if (owner != a) continue;
 
     const int nx = TILE * Kx + rx, ny = TILE * Ky + ry, nz = TILE * Kz + rz;
     const bool inside = nx < S.D[0] && ny < S.D[1] && nz < S.D[2];
     if (!inside) continue;
-    float hi[NQ], lo[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) { hi[q] = 0.f; lo[q] = 0.f; }
+    // the tile partials are exact fixed-point sums held in double; their sum in double, in a
+    // fixed order, has a relative error of a few 2^-53 before the one rounding to float.
+    // Candidates that do not contribute read a zero block, so all loads issue together.
+    const double* src[8];
 #pragma unroll
     for (int bb = 0; bb < 8; ++bb) {
       const int bx = bb & 1, by = (bb >> 1) & 1, bz = bb >> 2;
-      if ((bx && rx < 1) || (by && ry < 1) || (bz && rz < 1)) continue;
-      if (!tile_live(S, tile_stamp, step, Kx + bx, Ky + by, Kz + bz)) continue;
+      const bool use = live[bb] && !((bx && rx < 1) || (by && ry < 1) || (bz && rz < 1));
       const uint32_t ctile = S.ct_off + uint32_t((Kx + bx) + S.ct[0] * ((Ky + by) + S.ct[1] * (Kz + bz)));
       const int j = (bx ? rx - 1 : rx + 3) + HALO * (by ? ry - 1 : ry + 3) + HALO * HALO * (bz ? rz - 1 : rz + 3);
-      const float* src = tpart + size_t(ctile) * (NQ * HALO_NODES) + j;
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) two_sum_acc(hi[q], lo[q], src[q * HALO_NODES]);
+      src[bb] = use ? tpart + size_t(ctile) * (NQ * HALO_NODES) + j : zero_part;
     }
-    const float m = hi[0] + lo[0];
-    const float px = hi[1] + lo[1], py = hi[2] + lo[2], pz = hi[3] + lo[3];
-    const float fx = hi[4] + lo[4], fy = hi[5] + lo[5], fz = hi[6] + lo[6];
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      acc[q] = src[0][q * HALO_NODES];
+#pragma unroll
+      for (int bb = 1; bb < 8; ++bb) acc[q] += src[bb][q * HALO_NODES];
+    }
+    const float m = float(acc[0]);
+    const float px = float(acc[1]), py = float(acc[2]), pz = float(acc[3]);
+    const float fx = float(acc[4]), fy = float(acc[5]), fz = float(acc[6]);
     const int64_t node = S.node_off + nx + int64_t(S.D[0]) * (ny + int64_t(S.D[1]) * nz);
     if (KEEP) {
       g_mmv[node] = make_float4(px, py, pz, m);

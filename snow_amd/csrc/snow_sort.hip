@@ -1,12 +1,17 @@
 // snow_sort.hip - stable LSD radix sort of (cell key, slot) pairs for gfx950.
 //
-// Each pass sorts by one 8-bit digit:
-//   k_radix_hist     per-block digit histograms (LDS atomics; counts are order-independent)
-//   k_col_*          exclusive offsets from the block-major histogram table (column sums per
-//                    group of blocks, one block for group and digit prefixes, column apply)
-//   k_radix_scatter  stable scatter: a block walks its 4096 keys in 16 rounds of 256 in input
-//                    order; within a wave, keys with equal digits are matched with 8 ballots and
-//                    ranked by lane, waves are ranked in order through LDS counters.
+// The key bits are split into the fewest passes of at most 11 bits (22-bit keys: two passes).
+// Each pass runs three kernels over tiles of 4096 keys:
+//   k_radix_hist     per-tile digit histogram, written digit-major (hist[d][tile]); each thread
+//                    counts 16 consecutive keys and adds runs of equal digits with one LDS atomic
+//                    (nearly sorted input makes long runs)
+//   k_radix_rowscan  one wave per digit: exclusive scan of its row over the tiles, and the digit
+//                    total
+//   k_radix_scatter  each wave owns 1024 consecutive keys of the tile and ranks them in input
+//                    order (keys with equal digits in a round of 64 are matched with ballots,
+//                    running per-wave digit counters in LDS carry the rank across rounds); the
+//                    wave bases follow from the per-wave counts, the tile's row offsets and the
+//                    exclusive scan of the digit totals, which every tile recomputes.
 // Stability makes the result a deterministic function of the input order. No hipMemset and no
 // host synchronisation: the pass sequence can be captured in a HIP graph.
 #include <hip/hip_runtime.h>
@@ -15,162 +20,173 @@
 namespace snow {
 
 namespace {
-constexpr int RADIX_BITS = 8;
-constexpr int BINS = 1 << RADIX_BITS;
+constexpr int MAX_BITS = 11;
+constexpr int MAX_BINS = 1 << MAX_BITS;
 constexpr int SORT_THREADS = 256;
+constexpr int SORT_WAVES = SORT_THREADS / 64;
 constexpr int SORT_ITEMS = 16;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // keys per block
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // keys per tile
+constexpr int WAVE_KEYS = SORT_TILE / SORT_WAVES;      // consecutive keys per wave
 
 __global__ void __launch_bounds__(SORT_THREADS) k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                             int shift, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[BINS];
+                                                             int shift, int bits, uint32_t ntiles,
+                                                             uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[MAX_BINS];
   const int t = threadIdx.x;
-  h[t] = 0u;
+  const int bins = 1 << bits;
+  const uint32_t mask = uint32_t(bins - 1);
+  for (int d = t; d < bins; d += SORT_THREADS) h[d] = 0u;
   __syncthreads();
-  const uint32_t base = blockIdx.x * uint32_t(SORT_TILE);
-  uint32_t kreg[SORT_ITEMS];
+  const uint32_t i0 = blockIdx.x * uint32_t(SORT_TILE) + uint32_t(t) * SORT_ITEMS;
+  uint32_t k[SORT_ITEMS];
+  if (i0 + SORT_ITEMS <= n) {
+    const uint4* p = reinterpret_cast<const uint4*>(keys + i0);
+#pragma unroll
+    for (int q = 0; q < SORT_ITEMS / 4; ++q) {
+      const uint4 v = p[q];
+      k[4 * q] = v.x; k[4 * q + 1] = v.y; k[4 * q + 2] = v.z; k[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; ++r) k[r] = i0 + r < n ? keys[i0 + r] : 0u;
+  }
+  const int cnt_valid = i0 >= n ? 0 : int(min(uint32_t(SORT_ITEMS), n - i0));
+  uint32_t cur = (k[0] >> shift) & mask, run = 0u;
 #pragma unroll
   for (int r = 0; r < SORT_ITEMS; ++r) {
-    const uint32_t i = base + uint32_t(r * SORT_THREADS + t);
-    kreg[r] = i < n ? keys[i] : 0xffffffffu;
+    if (r < cnt_valid) {
+      const uint32_t d = (k[r] >> shift) & mask;
+      if (d != cur) {
+        atomicAdd(&h[cur], run);
+        cur = d;
+        run = 0u;
+      }
+      ++run;
+    }
   }
+  if (run) atomicAdd(&h[cur], run);
+  __syncthreads();
+  for (int d = t; d < bins; d += SORT_THREADS) hist[size_t(d) * ntiles + blockIdx.x] = h[d];
+}
+
+// One wave per digit row: exclusive scan over the tiles in place, row total to totals[d].
+__global__ void __launch_bounds__(SORT_THREADS) k_radix_rowscan(uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                                int bins, uint32_t* __restrict__ totals) {
+  const int lane = threadIdx.x & 63;
+  const int d = int(blockIdx.x) * SORT_WAVES + int(threadIdx.x >> 6);
+  if (d >= bins) return;  // whole wave
+  uint32_t* row = hist + size_t(d) * ntiles;
+  uint32_t carry = 0u;
+  for (uint32_t c = 0; c < ntiles; c += 64) {
+    const uint32_t i = c + uint32_t(lane);
+    const uint32_t v = i < ntiles ? row[i] : 0u;
+    uint32_t s = v;
 #pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r)
-    if (base + uint32_t(r * SORT_THREADS + t) < n) atomicAdd(&h[(kreg[r] >> shift) & (BINS - 1)], 1u);
-  __syncthreads();
-  hist[size_t(blockIdx.x) * BINS + t] = h[t];  // block-major: row = block, column = digit
-}
-
-// Offsets from the block-major histogram: O[b][d] = sum_{d'<d} total(d') + sum_{b'<b} hist[b'][d].
-// Rows are processed in groups of GROUP blocks: column sums per group, one block scanning the
-// group sums and the digit totals, then each group rewrites its rows as offsets.
-constexpr int GROUP = 32;
-
-__global__ void __launch_bounds__(BINS) k_col_sums(const uint32_t* __restrict__ hist, uint32_t nblocks,
-                                                   uint32_t* __restrict__ gsum) {
-  const int d = threadIdx.x;
-  const uint32_t b0 = blockIdx.x * GROUP;
-  uint32_t s = 0u;
-#pragma unroll 8
-  for (int r = 0; r < GROUP; ++r)
-    if (b0 + r < nblocks) s += hist[size_t(b0 + r) * BINS + d];
-  gsum[size_t(blockIdx.x) * BINS + d] = s;
-}
-
-__global__ void __launch_bounds__(BINS) k_col_scan(uint32_t* __restrict__ gsum, uint32_t ngroups) {
-  __shared__ uint32_t s_tot[BINS];
-  const int d = threadIdx.x;
-  uint32_t s = 0u;
-  for (uint32_t g = 0; g < ngroups; ++g) {
-    const uint32_t v = gsum[size_t(g) * BINS + d];
-    gsum[size_t(g) * BINS + d] = s;
-    s += v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(s, off, 64);
+      if (lane >= off) s += y;
+    }
+    if (i < ntiles) row[i] = carry + s - v;
+    carry += __shfl(s, 63, 64);
   }
-  s_tot[d] = s;
-  __syncthreads();
-  // inclusive Hillis-Steele scan of the digit totals (one per thread)
-  for (int off = 1; off < BINS; off <<= 1) {
-    const uint32_t v = d >= off ? s_tot[d - off] : 0u;
-    __syncthreads();
-    s_tot[d] += v;
-    __syncthreads();
-  }
-  const uint32_t dbase = s_tot[d] - s;
-  for (uint32_t g = 0; g < ngroups; ++g) gsum[size_t(g) * BINS + d] += dbase;
+  if (lane == 0) totals[d] = carry;
 }
 
-__global__ void __launch_bounds__(BINS) k_col_apply(uint32_t* __restrict__ hist, uint32_t nblocks,
-                                                    const uint32_t* __restrict__ gsum) {
-  const int d = threadIdx.x;
-  const uint32_t b0 = blockIdx.x * GROUP;
-  uint32_t s = gsum[size_t(blockIdx.x) * BINS + d];
-  for (int r = 0; r < GROUP; ++r) {
-    if (b0 + r >= nblocks) break;
-    const size_t idx = size_t(b0 + r) * BINS + d;
-    const uint32_t v = hist[idx];
-    hist[idx] = s;
-    s += v;
-  }
-}
-
-__global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(const uint32_t* __restrict__ kin,
-                                                                const uint32_t* __restrict__ vin,
-                                                                uint32_t* __restrict__ kout,
-                                                                uint32_t* __restrict__ vout, uint32_t n, int shift,
-                                                                const uint32_t* __restrict__ offsets,
-                                                                uint32_t nblocks) {
-  constexpr int NW = SORT_THREADS / 64;
-  __shared__ uint32_t s_base[BINS];
-  __shared__ uint32_t s_wcnt[NW][BINS];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  s_base[t] = offsets[size_t(blockIdx.x) * BINS + t];
-#pragma unroll
-  for (int w = 0; w < NW; ++w) s_wcnt[w][t] = 0u;
-  __syncthreads();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const uint32_t base = blockIdx.x * uint32_t(SORT_TILE);
-  // all loads of the block's keys and values are issued before the ranking rounds
+__global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, uint32_t n, int shift, int bits, uint32_t ntiles,
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_cnt[SORT_WAVES][MAX_BINS];
+  __shared__ uint32_t s_part[SORT_THREADS];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int bins = 1 << bits;
+  const uint32_t mask = uint32_t(bins - 1);
+  for (int i = t; i < SORT_WAVES * MAX_BINS; i += SORT_THREADS) (&s_cnt[0][0])[i] = 0u;
+  const uint32_t base = blockIdx.x * uint32_t(SORT_TILE) + uint32_t(wave) * WAVE_KEYS + uint32_t(lane);
   uint32_t kreg[SORT_ITEMS], vreg[SORT_ITEMS];
 #pragma unroll
   for (int r = 0; r < SORT_ITEMS; ++r) {
-    const uint32_t i = base + uint32_t(r * SORT_THREADS + t);
+    const uint32_t i = base + uint32_t(r * 64);
     kreg[r] = i < n ? kin[i] : 0u;
     vreg[r] = i < n ? vin[i] : 0u;
   }
+  __syncthreads();
+  // rank within the wave: running per-digit counters, one round of 64 keys at a time
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t loc[SORT_ITEMS];
+  uint32_t* cnt = s_cnt[wave];
 #pragma unroll
   for (int r = 0; r < SORT_ITEMS; ++r) {
-    const uint32_t i = base + uint32_t(r * SORT_THREADS + t);
-    const bool valid = i < n;
-    const uint32_t k = kreg[r];
-    const uint32_t v = vreg[r];
-    const uint32_t d = (k >> shift) & (BINS - 1);
+    const bool valid = base + uint32_t(r * 64) < n;
+    const uint32_t d = (kreg[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < RADIX_BITS; ++b) {
+    for (int b = 0; b < bits; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bb = __ballot(bit);
       peers &= bit ? bb : ~bb;
     }
     const uint32_t rank = __popcll(peers & lt_mask);
-    if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
+    const uint32_t old = valid ? cnt[d] : 0u;
+    // one lane per digit group updates the counter; the wave's LDS accesses complete in order
+    if (valid && rank == 0u) cnt[d] = old + uint32_t(__popcll(peers));
+    loc[r] = old + rank;
+  }
+  __syncthreads();
+  // digit bases: exclusive scan of the totals, plus this tile's row offsets, plus earlier waves
+  const int per = (bins + SORT_THREADS - 1) / SORT_THREADS;
+  const int d0 = t * per;
+  uint32_t part = 0u;
+  for (int j = 0; j < per; ++j)
+    if (d0 + j < bins) part += totals[d0 + j];
+  s_part[t] = part;
+  __syncthreads();
+  for (int off = 1; off < SORT_THREADS; off <<= 1) {
+    const uint32_t y = t >= off ? s_part[t - off] : 0u;
     __syncthreads();
-    if (valid) {
-      uint32_t pos = s_base[d] + rank;
-#pragma unroll
-      for (int w = 0; w < NW; ++w)
-        if (w < wave) pos += s_wcnt[w][d];
-      kout[pos] = k;
-      vout[pos] = v;
-    }
+    s_part[t] += y;
     __syncthreads();
-    {
-      uint32_t add = 0u;
+  }
+  uint32_t run = s_part[t] - part;
+  for (int j = 0; j < per; ++j) {
+    const int d = d0 + j;
+    if (d < bins) {
+      uint32_t acc = run + offsets[size_t(d) * ntiles + blockIdx.x];
+      run += totals[d];
 #pragma unroll
-      for (int w = 0; w < NW; ++w) {
-        add += s_wcnt[w][t];
-        s_wcnt[w][t] = 0u;
+      for (int w = 0; w < SORT_WAVES; ++w) {
+        const uint32_t c = s_cnt[w][d];
+        s_cnt[w][d] = acc;
+        acc += c;
       }
-      s_base[t] += add;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    if (base + uint32_t(r * 64) < n) {
+      const uint32_t d = (kreg[r] >> shift) & mask;
+      const uint32_t pos = cnt[d] + loc[r];
+      kout[pos] = kreg[r];
+      vout[pos] = vreg[r];
+    }
   }
 }
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
 
 struct SortLayout {
-  uint32_t nblocks, ngroups;
-  size_t hist_off, gsum_off, ktmp_off, vtmp_off, total;
+  uint32_t ntiles;
+  size_t hist_off, tot_off, ktmp_off, vtmp_off, total;
 };
 
 SortLayout layout(uint32_t n) {
   SortLayout L;
-  L.nblocks = div_up(n ? n : 1, SORT_TILE);
-  L.ngroups = div_up(L.nblocks, GROUP);
+  L.ntiles = div_up(n ? n : 1, SORT_TILE);
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   L.hist_off = 0;
-  L.gsum_off = al(L.hist_off + sizeof(uint32_t) * BINS * L.nblocks);
-  L.ktmp_off = al(L.gsum_off + sizeof(uint32_t) * BINS * L.ngroups);
+  L.tot_off = al(L.hist_off + sizeof(uint32_t) * MAX_BINS * size_t(L.ntiles));
+  L.ktmp_off = al(L.tot_off + sizeof(uint32_t) * MAX_BINS);
   L.vtmp_off = al(L.ktmp_off + sizeof(uint32_t) * (n ? n : 1));
   L.total = al(L.vtmp_off + sizeof(uint32_t) * (n ? n : 1));
   return L;
@@ -179,6 +195,7 @@ SortLayout layout(uint32_t n) {
 
 // Stable sort of (keys_in, vals_in) by the low end_bit bits of the key into (keys_out,
 // vals_out). With temp == nullptr, returns the temporary storage size in *temp_bytes.
+// Arrays must be 16-byte aligned.
 hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
                             const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit,
                             hipStream_t stream) {
@@ -191,10 +208,12 @@ hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys
   if (n == 0) return hipSuccess;
   char* base = static_cast<char*>(temp);
   uint32_t* hist = reinterpret_cast<uint32_t*>(base + L.hist_off);
-  uint32_t* gsum = reinterpret_cast<uint32_t*>(base + L.gsum_off);
+  uint32_t* totals = reinterpret_cast<uint32_t*>(base + L.tot_off);
   uint32_t* ktmp = reinterpret_cast<uint32_t*>(base + L.ktmp_off);
   uint32_t* vtmp = reinterpret_cast<uint32_t*>(base + L.vtmp_off);
-  const int passes = end_bit <= 0 ? 1 : (end_bit + RADIX_BITS - 1) / RADIX_BITS;
+  const int eb = end_bit < 1 ? 1 : (end_bit > 32 ? 32 : end_bit);
+  const int passes = (eb + MAX_BITS - 1) / MAX_BITS;
+  const int dbits = (eb + passes - 1) / passes;
   // choose the first destination so that the last pass writes keys_out
   const uint32_t* kin = keys_in;
   const uint32_t* vin = vals_in;
@@ -202,12 +221,13 @@ hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys
     const bool to_out = ((passes - 1 - p) % 2) == 0;
     uint32_t* ko = to_out ? keys_out : ktmp;
     uint32_t* vo = to_out ? vals_out : vtmp;
-    const int shift = p * RADIX_BITS;
-    k_radix_hist<<<L.nblocks, SORT_THREADS, 0, stream>>>(kin, n, shift, hist);
-    k_col_sums<<<L.ngroups, BINS, 0, stream>>>(hist, L.nblocks, gsum);
-    k_col_scan<<<1, BINS, 0, stream>>>(gsum, L.ngroups);
-    k_col_apply<<<L.ngroups, BINS, 0, stream>>>(hist, L.nblocks, gsum);
-    k_radix_scatter<<<L.nblocks, SORT_THREADS, 0, stream>>>(kin, vin, ko, vo, n, shift, hist, L.nblocks);
+    const int shift = p * dbits;
+    const int bits = eb - shift < dbits ? eb - shift : dbits;
+    const int bins = 1 << bits;
+    k_radix_hist<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, n, shift, bits, L.ntiles, hist);
+    k_radix_rowscan<<<div_up(bins, SORT_WAVES), SORT_THREADS, 0, stream>>>(hist, L.ntiles, bins, totals);
+    k_radix_scatter<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, vin, ko, vo, n, shift, bits, L.ntiles, hist,
+                                                           totals);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     kin = ko;
