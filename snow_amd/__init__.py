@@ -253,7 +253,7 @@ class SnowContext:
     def autotune(self, steps: int = 5) -> dict:
         g, p, q = C.c_int32(), C.c_int32(), C.c_int32()
         self._chk(self.lib.snow_autotune(self.h, steps, C.byref(g), C.byref(p), C.byref(q)))
-        return {"g2p_block": g.value, "p2g_chunk": p.value, "grid_per_cu": q.value}
+        return {"g2p_block": g.value, "p2g_chunk": p.value, "grid_blocks": q.value}
 
     def sync(self):
         self._chk(self.lib.snow_sync(self.h))

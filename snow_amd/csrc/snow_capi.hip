@@ -87,9 +87,10 @@ struct snow_ctx {
   double* d_tpart = nullptr;
   double* d_hpart = nullptr;
   double* d_zero_part = nullptr;  // NQ * HALO_NODES zeros
+  uint32_t* d_list_ctr = nullptr;  // TILE_LISTS append counters
+  uint32_t* d_tile_list = nullptr; // TILE_LISTS sublists of total_ctiles entries
   StepCtl* d_ctl = nullptr;
   uint32_t* d_tile_stamp = nullptr;
-  uint32_t* d_active_list = nullptr;
   int num_cu = 256;
   float4 *d_g_mmv = nullptr, *d_g_f = nullptr, *d_g_node = nullptr;  // g_node: 2 float4 per node
   Collider* d_cols = nullptr;
@@ -122,7 +123,6 @@ struct snow_ctx {
   int g2p_block = 256;
   uint32_t total_chunks = 0;       // P2G workgroups (chunks of every scene)
   uint16_t* d_chunk_scene = nullptr;
-  int grid_per_cu = 32;
 };
 
 namespace {
@@ -281,7 +281,7 @@ int run_force_prepass(snow_ctx* c) {
 // Step start (stamp, counters, rigid advance when dt_rigid >= 0) + sort + cell bounds.
 int run_sort(snow_ctx* c, float dt_rigid) {
   k_step_begin<<<1, 256, 0, c->stream>>>(c->d_ctl, c->d_counts, c->nscenes + 1, c->d_scenes, c->nscenes, c->d_cols,
-                                         dt_rigid);
+                                         dt_rigid, c->d_list_ctr);
   LAUNCHCHK(c);
   int rc = run_keys(c);
   if (rc) return rc;
@@ -291,12 +291,6 @@ int run_sort(snow_ctx* c, float dt_rigid) {
     HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->N,
                                c->key_bits, c->stream));
   }
-  if (c->N) {
-    k_tile_bounds<<<grid_for((c->N + 3) / 4, 256), 256, 0, c->stream>>>(c->d_keys[1], c->N, c->d_tile_lo, c->d_tile_hi,
-                                                               c->d_scenes, c->d_tile_scene, c->d_ctl,
-                                                               c->d_tile_stamp, c->d_active_list);
-    LAUNCHCHK(c);
-  }
   return tm.done();
 }
 
@@ -304,10 +298,11 @@ int run_p2g(snow_ctx* c, int mass_only) {
   StageTimer tm(c, ST_P2G);
   if (c->total_chunks) {
     k_p2g_chunk<<<c->total_chunks, 64 * P2G_WAVES, 0, c->stream>>>(
-        c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1], c->d_tile_lo, c->d_tile_hi, c->d_rec[c->cur],
-        c->d_tpart, c->d_hpart, c->d_chunk_scene, mass_only ? 0 : 1);
+        c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1], c->d_tile_lo, c->d_tile_hi, c->d_tile_stamp,
+        c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_tpart, c->d_hpart,
+        c->d_chunk_scene, mass_only ? 0 : 1);
     LAUNCHCHK(c);
-    k_fold_partials<<<c->total_chunks, 256, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
+    k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
                                                             c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart);
     LAUNCHCHK(c);
   }
@@ -317,16 +312,17 @@ int run_p2g(snow_ctx* c, int mass_only) {
 int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
   StageTimer tm(c, ST_GRID);
   unsigned long long* cnt = c->d_counts;
-  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles * 8u, unsigned(c->num_cu * c->grid_per_cu)));
+  if (c->total_chunks == 0) return tm.done();
+  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu * 4)));
 #define GRID_ARGS                                                                                          \
-  c->d_scenes, c->d_tile_scene, c->d_ctl, c->d_active_list, c->d_tile_stamp, c->d_tpart, c->d_zero_part,     \
-      c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_cols, dt, cnt
+  c->d_scenes, c->d_tile_scene, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_ctl, c->d_tile_stamp,     \
+      c->d_tpart, c->d_zero_part, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_cols, dt, cnt
   if (keep && update)
-    k_grid<true, true><<<blocks, 64, 0, c->stream>>>(GRID_ARGS);
+    k_grid<true, true><<<blocks, GRID_THREADS, 0, c->stream>>>(GRID_ARGS);
   else if (update)
-    k_grid<false, true><<<blocks, 64, 0, c->stream>>>(GRID_ARGS);
+    k_grid<false, true><<<blocks, GRID_THREADS, 0, c->stream>>>(GRID_ARGS);
   else
-    k_grid<true, false><<<blocks, 64, 0, c->stream>>>(GRID_ARGS);
+    k_grid<true, false><<<blocks, GRID_THREADS, 0, c->stream>>>(GRID_ARGS);
 #undef GRID_ARGS
   LAUNCHCHK(c);
   return tm.done();
@@ -383,7 +379,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_sort_tmp,
-                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_chunk_scene, c->d_ctl, c->d_tile_stamp, c->d_active_list, c->d_g_mmv, c->d_g_f, c->d_g_node,
+                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_ctl, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -559,17 +555,18 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     c->sort_tmp_bytes = tb;
     CREATECHK(hipMalloc(&c->d_sort_tmp, std::max<size_t>(tb, 16)));
   }
-  const size_t ncells = size_t(c->total_ctiles) * TILE_CELLS;
   CREATECHK(dalloc(&c->d_tile_lo, c->total_ctiles));
   CREATECHK(dalloc(&c->d_tile_hi, c->total_ctiles));
   CREATECHK(dalloc(&c->d_tpart, size_t(c->total_ctiles) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_hpart, size_t(std::max<uint32_t>(1, c->total_chunks)) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_chunk_scene, std::max<uint32_t>(1, c->total_chunks)));
   CREATECHK(dalloc(&c->d_zero_part, size_t(NQ) * HALO_NODES));
+  CREATECHK(dalloc(&c->d_list_ctr, size_t(TILE_LISTS) * TILE_LIST_STRIDE));
+  CREATECHK(hipMemset(c->d_list_ctr, 0, sizeof(uint32_t) * TILE_LISTS * TILE_LIST_STRIDE));
+  CREATECHK(dalloc(&c->d_tile_list, size_t(TILE_LISTS) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_zero_part, 0, sizeof(double) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_ctl, 1));
   CREATECHK(dalloc(&c->d_tile_stamp, c->total_ctiles));
-  CREATECHK(dalloc(&c->d_active_list, c->total_ctiles));
   CREATECHK(dalloc(&c->d_g_mmv, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_g_f, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_g_node, 2 * size_t(c->total_nodes)));
@@ -636,7 +633,6 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(hipMemset(c->d_tile_hi, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_ctl, 0, sizeof(StepCtl)));
   CREATECHK(hipMemset(c->d_tile_stamp, 0, sizeof(uint32_t) * c->total_ctiles));
-  CREATECHK(hipMemset(c->d_active_list, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_g_mmv, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_f, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_node, 0, 2 * sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
@@ -776,7 +772,7 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
   return 0;
 }
 
-int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_chunk, int32_t* grid_per_cu) {
+int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_chunk, int32_t* grid_blocks) {
   if (!c || steps <= 0) return SNOW_E_ARG;
   if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before autotuning");
   for (const auto& S : c->scenes)
@@ -829,10 +825,10 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
     ms = f;
     return 0;
   };
-  // coordinate descent over the launch shapes that leave the results unchanged
+  // the G2P block size is the launch shape that leaves the results unchanged; the P2G and grid
+  // widths follow from the particle counts
   const int g2p_cands[] = {64, 128, 256};
-  const int grid_cands[] = {8, 32};
-  int best_g = c->g2p_block, best_q = c->grid_per_cu;
+  int best_g = c->g2p_block;
   double best = 1e30, t = 0.0;
   for (int g : g2p_cands) {
     c->g2p_block = g;
@@ -840,13 +836,6 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
     if (t < best) { best = t; best_g = g; }
   }
   c->g2p_block = best_g;
-  for (int q : grid_cands) {
-    if (rc) break;
-    c->grid_per_cu = q;
-    if ((rc = trial(t))) break;
-    if (t < best) { best = t; best_q = q; }
-  }
-  c->grid_per_cu = best_q;
   c->timers = timers;
   const int rr = restore();
   free_all();
@@ -861,7 +850,7 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
   }
   if (g2p_block) *g2p_block = c->g2p_block;
   if (p2g_chunk) *p2g_chunk = c->scenes.empty() ? 0 : int32_t(c->scenes[0].seg);
-  if (grid_per_cu) *grid_per_cu = c->grid_per_cu;
+  if (grid_blocks) *grid_blocks = int32_t(c->total_chunks);
   return 0;
 }
 

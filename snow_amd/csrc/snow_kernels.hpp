@@ -24,6 +24,7 @@ constexpr int HALO = 7;              // TILE + 3 nodes per edge
 constexpr int HALO_NODES = 343;      // 7^3
 constexpr int NQ = 7;                // m, p.xyz, f.xyz
 constexpr int P2G_WAVES = 8;         // waves per P2G workgroup
+constexpr int P2G_SEG_MAX = 2048;    // largest P2G chunk in particles (sizes the LDS run lists)
 
 struct SceneDev {
   int D[3];
@@ -120,17 +121,25 @@ __device__ __forceinline__ void store_node(float4* __restrict__ g, int64_t n, fl
 // (graph-capturable) while the set of active tiles changes from step to step.
 struct StepCtl {
   uint32_t step;      // step stamp, incremented by k_step_begin
-  uint32_t n_active;  // active cell tiles this step (entries of active_list)
+  uint32_t n_active;  // unused
   uint32_t spare;
   uint32_t pad;
 };
+
+// The list of cell tiles with particles is kept in TILE_LISTS sublists, each with its own
+// append counter (TILE_LIST_STRIDE words apart, so the counters sit in different cache lines):
+// appends from many workgroups then do not serialise on one address.
+constexpr int TILE_LISTS = 64;
+constexpr int TILE_LIST_STRIDE = 32;
 
 // Start of a step: new stamp, cleared counters, and the collider advance of rigidAdvance.glsl:14-18
 // (cx += dt*cv) when dt_rigid >= 0 (0 selects each scene's own dt).
 __global__ void __launch_bounds__(256) k_step_begin(StepCtl* __restrict__ ctl, unsigned long long* __restrict__ counts,
                                                     int ncounts, const SceneDev* __restrict__ scenes, int nscenes,
-                                                    Collider* __restrict__ cols, float dt_rigid) {
+                                                    Collider* __restrict__ cols, float dt_rigid,
+                                                    uint32_t* __restrict__ list_ctr) {
   const int t = threadIdx.x;
+  if (t < TILE_LISTS) list_ctr[t * TILE_LIST_STRIDE] = 0u;
   if (t == 0) {
     ctl->step += 1u;
     ctl->n_active = 0u;
@@ -162,66 +171,50 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
   }
 }
 
-// Particle range [tile_lo, tile_hi) of every non-empty cell tile in the sorted key array, and
-// the list of active cell tiles (stamped with this step). Entries of tiles without particles
-// are left stale; only listed (stamped) tiles are read. The order of active_list is not
-// deterministic, but nothing computed from it depends on that order.
-// Each thread handles 4 consecutive keys; its neighbours' keys come from two extra loads.
-__global__ void __launch_bounds__(256) k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n,
-                                                     uint32_t* __restrict__ tile_lo, uint32_t* __restrict__ tile_hi,
-                                                     const SceneDev* __restrict__ scenes,
-                                                     const uint16_t* __restrict__ tile_scene, StepCtl* __restrict__ ctl,
-                                                     uint32_t* __restrict__ tile_stamp,
-                                                     uint32_t* __restrict__ active_list) {
-  const uint32_t i0 = 4u * (blockIdx.x * blockDim.x + threadIdx.x);
-  uint32_t k[4];
-  if (i0 + 4u <= n) {
-    const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
-    k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
-  } else {
+// Ordered starts of the runs of equal cell tile among the sorted positions [lo, hi), written to
+// runs[0..n); returns n. Block-wide (every thread of the NT-thread block calls it). Position lo
+// always starts a run (whether or not its tile began before lo).
+template <int NT>
+__device__ uint32_t ordered_runs(const uint32_t* __restrict__ keys, uint32_t lo, uint32_t hi,
+                                 uint32_t* __restrict__ runs, uint32_t* __restrict__ s_wsum) {
+  constexpr int PER_MAX = P2G_SEG_MAX / NT;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const uint32_t len = hi - lo;
+  const uint32_t per = (len + NT - 1) / NT;
+  const uint32_t b = lo + uint32_t(t) * per, e = min(hi, b + per);
+  uint32_t flags = 0u, cnt = 0u;
+  uint32_t prevt = (b > lo && b < hi) ? keys[b - 1] / TILE_CELLS : 0xffffffffu;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) k[j] = i0 + j < n ? keys[i0 + j] : 0xffffffffu;
-  }
-  const uint32_t prev = (i0 > 0u && i0 <= n) ? keys[i0 - 1u] / TILE_CELLS : 0xffffffffu;
-  const uint32_t next = i0 + 4u < n ? keys[i0 + 4u] / TILE_CELLS : 0xffffffffu;
-  uint32_t tl[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) tl[j] = i0 + j < n ? k[j] / TILE_CELLS : 0xffffffffu;
-  bool start[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const uint32_t i = i0 + uint32_t(j);
-    const bool in = i < n;
-    const uint32_t before = j == 0 ? prev : tl[j - 1];
-    const uint32_t after = j == 3 ? next : tl[j + 1];
-    const bool first = in && before != tl[j];
-    if (in && after != tl[j]) tile_hi[tl[j]] = i + 1u;
-    if (first) tile_lo[tl[j]] = i;
-    start[j] = first && scenes[tile_scene[tl[j]]].dead_tile != tl[j];
-  }
-  // one atomic per wave for the tile-list append
-  uint64_t ball[4];
-  uint32_t total = 0u;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    ball[j] = __ballot(start[j]);
-    total += uint32_t(__popcll(ball[j]));
-  }
-  if (!total) return;
-  const int lane = threadIdx.x & 63;
-  uint32_t slot0 = 0u;
-  if (lane == 0) slot0 = atomicAdd(&ctl->n_active, total);
-  slot0 = __shfl(slot0, 0, 64);
-  const uint32_t step = ctl->step;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (start[j]) {
-      active_list[slot0 + uint32_t(__popcll(ball[j] & lt))] = tl[j];
-      tile_stamp[tl[j]] = step;
+  for (int k = 0; k < PER_MAX; ++k) {
+    const uint32_t i = b + uint32_t(k);
+    if (i < e) {
+      const uint32_t tl = keys[i] / TILE_CELLS;
+      if (i == lo || tl != prevt) { flags |= 1u << k; ++cnt; }
+      prevt = tl;
     }
-    slot0 += uint32_t(__popcll(ball[j]));
   }
+  // block exclusive scan of the counts
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) s_wsum[wave] = incl;
+  __syncthreads();
+  uint32_t base = 0u, total = 0u;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    const uint32_t v = s_wsum[w];
+    base += w < wave ? v : 0u;
+    total += v;
+  }
+  uint32_t o = base + incl - cnt;
+#pragma unroll
+  for (int k = 0; k < PER_MAX; ++k)
+    if (flags & (1u << k)) runs[o++] = b + uint32_t(k);
+  __syncthreads();
+  return total;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -272,14 +265,14 @@ __device__ __forceinline__ int fixed_exponent(float maxval, uint32_t count) {
 
 constexpr int P2G_CHUNK = 32;        // particles staged per wave at a time
 constexpr int P2G_STAGE = 40;        // floats per staged particle
-constexpr int P2G_SEG_MAX = 2048;    // largest chunk (sizes the LDS run list)
 
 __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-    const uint32_t* __restrict__ tile_lo, const uint32_t* __restrict__ tile_hi,
-    const float4* __restrict__ rec, double* __restrict__ tpart, double* __restrict__ hpart,
-    const uint16_t* __restrict__ chunk_scene, int skip_heavy) {
+    uint32_t* __restrict__ tile_lo, uint32_t* __restrict__ tile_hi, uint32_t* __restrict__ tile_stamp,
+    const StepCtl* __restrict__ ctl, uint32_t* __restrict__ list_ctr, uint32_t* __restrict__ tile_list,
+    uint32_t list_cap, const float4* __restrict__ rec, double* __restrict__ tpart,
+    double* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene, int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   __shared__ unsigned long long s_acc[NQ * HALO_NODES];
   // per-wave staging of up to P2G_CHUNK particles: for each axis the interleaved (w_o, q_o),
@@ -287,7 +280,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   __shared__ __attribute__((aligned(16))) float s_stage[P2G_WAVES][P2G_CHUNK * P2G_STAGE];
   __shared__ uint32_t s_run[P2G_SEG_MAX];
   __shared__ uint32_t s_bnd[2][3];
-  __shared__ uint32_t s_nrun;
+  __shared__ uint32_t s_wsum[P2G_WAVES];
+  __shared__ uint32_t s_edge[2];
 
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -295,26 +289,25 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
   float* stage = s_stage[wave];
   const uint32_t chunk = blockIdx.x;
+  const uint32_t step = ctl->step;
   uint32_t lo, hi;
   {
     const SceneDev& S = scenes[chunk_scene[chunk]];
     lo = uint32_t(S.p_off) + (chunk - S.chunk_off) * S.seg;
-    hi = min(uint32_t(S.p_off + S.p_cnt), lo + S.seg);
+    const uint32_t end = uint32_t(S.p_off + S.p_cnt);
+    hi = min(end, lo + S.seg);
+    // tiles of the particles just before and after the chunk (same scene only)
+    if (t == 0) {
+      s_edge[0] = lo > uint32_t(S.p_off) ? keys[lo - 1] / TILE_CELLS : 0xffffffffu;
+      s_edge[1] = hi < end ? keys[hi] / TILE_CELLS : 0xffffffffu;
+    }
   }
-
-  // run starts of the chunk (unordered: every run is processed on its own)
-  if (t == 0) s_nrun = 0u;
   if (t < 6) (&s_bnd[0][0])[t] = 0u;
-  __syncthreads();
-  for (uint32_t i = lo + t; i < hi; i += NT) {
-    const uint32_t tl = keys[i] / TILE_CELLS;
-    if (i == lo || keys[i - 1] / TILE_CELLS != tl) s_run[atomicAdd(&s_nrun, 1u)] = i;
-  }
-  __syncthreads();
-  const uint32_t nrun = s_nrun;
+  const uint32_t nrun = ordered_runs<NT>(keys, lo, hi, s_run, s_wsum);
 
   for (uint32_t r = 0; r < nrun; ++r) {
     const uint32_t r_lo = s_run[r];
+    const uint32_t r_hi = r + 1 < nrun ? s_run[r + 1] : hi;
     const uint32_t tile = keys[r_lo] / TILE_CELLS;
     const SceneDev& S = scenes[tile_scene[tile]];
     if (S.dead_tile == tile) {  // out-of-grid particles (uniform branch); keep the slot rotation
@@ -322,7 +315,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       __syncthreads();
       continue;
     }
-    const uint32_t r_hi = min(hi, tile_hi[tile]);
+    const bool starts_tile = r > 0 || s_edge[0] != tile;
+    const bool ends_tile = r + 1 < nrun || s_edge[1] != tile;
     const uint32_t count = r_hi - r_lo;
     uint32_t* bnd = s_bnd[r & 1u];
     // bound pre-pass over the run's particles
@@ -435,8 +429,16 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     }
     if (cur >= 0) commit();
     __syncthreads();
-    double* out = (tile_lo[tile] == r_lo ? tpart + size_t(tile) * (NQ * HALO_NODES)
-                                         : hpart + size_t(chunk) * (NQ * HALO_NODES));
+    double* out = starts_tile ? tpart + size_t(tile) * (NQ * HALO_NODES) : hpart + size_t(chunk) * (NQ * HALO_NODES);
+    if (t == 0) {
+      if (starts_tile) {
+        tile_lo[tile] = r_lo;
+        tile_stamp[tile] = step;
+        const uint32_t sub = chunk % TILE_LISTS;
+        tile_list[size_t(sub) * list_cap + atomicAdd(&list_ctr[sub * TILE_LIST_STRIDE], 1u)] = tile;
+      }
+      if (ends_tile) tile_hi[tile] = r_hi;
+    }
     for (int i = t; i < NQ * HALO_NODES; i += NT) {
       const int q = i / HALO_NODES;
       const int e = q == 0 ? em : (q < 4 ? ep : ef);
@@ -450,7 +452,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
 // A tile whose particles span several P2G chunks has its first run in tpart[tile] and one
 // continuation partial per later chunk. The block of the tile's last chunk adds the
 // continuations to tpart[tile] in chunk order (double sums of exact fixed-point partials).
-__global__ void __launch_bounds__(256) k_fold_partials(const SceneDev* __restrict__ scenes,
+__global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restrict__ scenes,
                                                        const uint16_t* __restrict__ chunk_scene,
                                                        const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ tile_lo,
@@ -470,7 +472,15 @@ __global__ void __launch_bounds__(256) k_fold_partials(const SceneDev* __restric
   double* dst = tpart + size_t(tile) * (NQ * HALO_NODES);
   for (int i = threadIdx.x; i < NQ * HALO_NODES; i += blockDim.x) {
     double acc = dst[i];
-    for (uint32_t c = c_first; c <= chunk; ++c) acc += hpart[size_t(c) * (NQ * HALO_NODES) + i];
+    // eight loads in flight at a time; the sum stays in chunk order (adding 0.0 is exact)
+    for (uint32_t c = c_first; c <= chunk; c += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        v[u] = c + u <= chunk ? hpart[size_t(c + u) * (NQ * HALO_NODES) + i] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
     dst[i] = acc;
   }
 }
@@ -478,36 +488,50 @@ __global__ void __launch_bounds__(256) k_fold_partials(const SceneDev* __restric
 // ---------------------------------------------------------------------------------------
 // Merge of the tile partials into node values + grid update (gForceCompute.glsl:57-103).
 // A node n = 4K + r (per axis) receives from cell tiles K (halo index r + 3) and, when r >= 1,
-// K+1 (halo index r - 1): at most 8 tiles (each with its continuation partials), summed in a
-// fixed order. Work items
-// are (active cell tile k, offset a in {0,1}^3): node tile K = k - a is processed by the
-// block whose k is the first stamped tile among K + b (b in the fixed order 000,100,010,...),
-// so every node tile reached by a particle stencil is processed exactly once.
+// K+1 (halo index r - 1): at most 8 tiles, summed in a fixed order. Workgroups take the listed
+// cell tiles k (those with particles) in turn; wave a of the workgroup handles node tile
+// K = k - a, a in {0,1}^3, if k is the first stamped tile among K + b (b in the fixed order
+// 000,100,010,...), so every node tile reached by a particle stencil is processed exactly once.
 __device__ __forceinline__ bool tile_live(const SceneDev& S, const uint32_t* __restrict__ stamp, uint32_t step,
                                           int kx, int ky, int kz) {
   if (kx < 0 || ky < 0 || kz < 0 || kx >= S.ct[0] || ky >= S.ct[1] || kz >= S.ct[2]) return false;
   return stamp[S.ct_off + uint32_t(kx + S.ct[0] * (ky + S.ct[1] * kz))] == step;
 }
 
+constexpr int GRID_THREADS = 512;  // 8 waves: one per candidate node tile
+
 template <bool KEEP, bool UPDATE>
-__global__ void __launch_bounds__(64) k_grid(const SceneDev* __restrict__ scenes,
-                                             const uint16_t* __restrict__ tile_scene,
-                                             const StepCtl* __restrict__ ctl,
-                                             const uint32_t* __restrict__ active_list,
-                                             const uint32_t* __restrict__ tile_stamp,
-                                             const double* __restrict__ tpart,
-                                             const double* __restrict__ zero_part,
-                                             float4* __restrict__ g_mmv, float4* __restrict__ g_f,
-                                             float4* __restrict__ g_node,
-                                             const Collider* __restrict__ cols, float dt_override,
-                                             unsigned long long* __restrict__ active_count) {
+__global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restrict__ scenes,
+                                                       const uint16_t* __restrict__ tile_scene,
+                                                       const uint32_t* __restrict__ list_ctr,
+                                                       const uint32_t* __restrict__ tile_list, uint32_t list_cap,
+                                                       const StepCtl* __restrict__ ctl,
+                                                       const uint32_t* __restrict__ tile_stamp,
+                                                       const double* __restrict__ tpart,
+                                                       const double* __restrict__ zero_part,
+                                                       float4* __restrict__ g_mmv, float4* __restrict__ g_f,
+                                                       float4* __restrict__ g_node,
+                                                       const Collider* __restrict__ cols, float dt_override,
+                                                       unsigned long long* __restrict__ active_count) {
   const uint32_t step = ctl->step;
-  const uint32_t items = ctl->n_active * 8u;
   const int t = threadIdx.x;
-  const int rx = t & 3, ry = (t >> 2) & 3, rz = t >> 4;
-  for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
-    const uint32_t k = active_list[w >> 3];
-    const int a = int(w & 7u);
+  const int a = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int lane = t & 63;
+  const int rx = lane & 3, ry = (lane >> 2) & 3, rz = lane >> 4;
+  // sublist sizes (lane i: sublist i) and their inclusive prefix
+  const uint32_t cnt = lane < TILE_LISTS ? list_ctr[lane * TILE_LIST_STRIDE] : 0u;
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
+    // (uniform values: the loads below become scalar loads)
+    const int sub = __builtin_amdgcn_readfirstlane(__ffsll((long long)__ballot(incl > e)) - 1);
+    const uint32_t first = __builtin_amdgcn_readfirstlane(__shfl(incl - cnt, sub, 64));
+    const uint32_t k = tile_list[size_t(sub) * list_cap + (e - first)];
     const int sid = tile_scene[k];
     const SceneDev& S = scenes[sid];
     const uint32_t kl = k - S.ct_off;
