@@ -184,6 +184,36 @@ class PhysicalScene:
     material: Material = field(default_factory=Material)
     dt: float = 1e-3
 
+    def save(self, filename: str) -> None:
+        """Write the scene in the SNOWSCN1 format read by the C++ driver (snow_amd/csrc/snow_run.cpp):
+        identity F_E / F_P, the grid, material, colliders, dt and the particles' x, v, mass."""
+        g, m = self.grid.to_c(), self.material.to_c()
+        cols = [c.to_c() for c in self.colliders.colliders]
+        ps = self.particles
+        with open(filename, "wb") as f:
+            f.write(b"SNOWSCN1")
+            f.write(np.int64(len(ps)).tobytes())
+            f.write(bytes(g))  # dim[3], origin[3], spacing
+            f.write(bytes(m))
+            f.write(np.int32(len(cols)).tobytes())
+            for c in cols:
+                f.write(bytes(c))
+            f.write(np.float32(self.dt).tobytes())
+            for a in (ps.x, ps.v, ps.mass):
+                f.write(np.ascontiguousarray(a, np.float32).tobytes())
+
+
+def load_run_output(filename: str) -> dict:
+    """Read the SNOWOUT1 file written by the C++ driver: x, v [N,3], fe, fp [N,9]."""
+    with open(filename, "rb") as f:
+        if f.read(8) != b"SNOWOUT1":
+            raise ValueError(f"{filename}: not a SNOWOUT1 file")
+        n = int(np.frombuffer(f.read(8), np.int64)[0])
+        out = {}
+        for k, w in (("x", 3), ("v", 3), ("fe", 9), ("fp", 9)):
+            out[k] = np.frombuffer(f.read(4 * w * n), np.float32).reshape(n, w).copy()
+    return out
+
 
 class SnowContext:
     """Owns one ``snow_ctx`` (one HIP stream) holding a batch of scenes."""

@@ -59,6 +59,19 @@ def build_library(verbose: bool = False) -> str:
     return OUT
 
 
+def build_driver(verbose: bool = False) -> str:
+    """The C++ command-line driver (csrc/snow_run.cpp) over include/snow_time_update.hpp."""
+    out = os.path.join(HERE, "snow_run")
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "snow_run.cpp"),
+           "-L", HERE, "-lsnow_amd", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle() -> str:
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "liboracle.so")
@@ -66,5 +79,6 @@ def build_oracle() -> str:
 
 if __name__ == "__main__":
     print(build_library(verbose="-v" in sys.argv))
+    print(build_driver(verbose="-v" in sys.argv))
     if "--oracle" in sys.argv:
         print(build_oracle())

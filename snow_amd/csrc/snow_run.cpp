@@ -1,0 +1,95 @@
+// snow_run.cpp - command-line driver of the snow step through the C++ step interface
+// (include/snow_time_update.hpp), the way the reference's launchSnow drives ExplicitTimeUpdate
+// (src/snow/utils/launchHelper/explicit.cpp:2-116), without a window or renderer.
+//
+//   snow_run SCENE_FILE STEPS OUT_FILE [flags]
+//
+// SCENE_FILE (little endian): "SNOWSCN1", int64 n, int32 dim[3], float origin[3], float h,
+//   snow_material (10 floats), int32 ncol, ncol x snow_collider (12 floats), float dt,
+//   float x[3n], float v[3n], float mass[n]
+// OUT_FILE: "SNOWOUT1", int64 n, float x[3n], v[3n], fe[9n], fp[9n] after init + STEPS steps.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "snow_time_update.hpp"
+
+namespace {
+
+template <class T>
+bool read_into(FILE* f, T* dst, size_t count) {
+  return std::fread(dst, sizeof(T), count, f) == count;
+}
+
+template <class T>
+bool write_from(FILE* f, const T* src, size_t count) {
+  return std::fwrite(src, sizeof(T), count, f) == count;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    std::fprintf(stderr, "usage: %s SCENE_FILE STEPS OUT_FILE [flags]\n", argv[0]);
+    return 2;
+  }
+  const long steps = std::strtol(argv[2], nullptr, 10);
+  const uint32_t flags = argc > 4 ? uint32_t(std::strtoul(argv[4], nullptr, 0)) : 0u;
+  FILE* f = std::fopen(argv[1], "rb");
+  if (!f) {
+    std::perror(argv[1]);
+    return 2;
+  }
+  char magic[8];
+  int64_t n = 0;
+  snow_scene_desc d;
+  std::memset(&d, 0, sizeof d);
+  int32_t ncol = 0;
+  bool ok = read_into(f, magic, 8) && std::memcmp(magic, "SNOWSCN1", 8) == 0 && read_into(f, &n, 1) &&
+            n >= 0 && n <= (int64_t(1) << 31) && read_into(f, d.grid.dim, 3) && read_into(f, d.grid.origin, 3) &&
+            read_into(f, &d.grid.spacing, 1) && read_into(f, &d.material, 1) && read_into(f, &ncol, 1) &&
+            ncol >= 0 && ncol <= 4096;
+  std::vector<snow_collider> cols(size_t(ncol > 0 ? ncol : 0));
+  const size_t nn = static_cast<size_t>(n);
+  std::vector<float> x(3 * nn), v(3 * nn), m(nn);
+  ok = ok && read_into(f, cols.data(), cols.size()) && read_into(f, &d.dt, 1) && read_into(f, x.data(), x.size()) &&
+       read_into(f, v.data(), v.size()) && read_into(f, m.data(), m.size());
+  std::fclose(f);
+  if (!ok) {
+    std::fprintf(stderr, "%s: not a SNOWSCN1 scene file\n", argv[1]);
+    return 2;
+  }
+  d.num_particles = n;
+  d.x = x.data();
+  d.v = v.data();
+  d.mass = m.data();
+  d.num_colliders = ncol;
+  d.colliders = cols.data();
+
+  try {
+    snow_amd::SnowExplicitTimeUpdate sim(d, flags);
+    sim.init();
+    for (long s = 0; s < steps; ++s) sim.update(d.dt);
+    std::vector<float> ox, ov, ofe, ofp;
+    sim.read_particles(0, &ox, &ov, &ofe, &ofp);
+    FILE* o = std::fopen(argv[3], "wb");
+    if (!o) {
+      std::perror(argv[3]);
+      return 2;
+    }
+    const bool wok = write_from(o, "SNOWOUT1", 8) && write_from(o, &n, 1) && write_from(o, ox.data(), ox.size()) &&
+                     write_from(o, ov.data(), ov.size()) && write_from(o, ofe.data(), ofe.size()) &&
+                     write_from(o, ofp.data(), ofp.size());
+    std::fclose(o);
+    if (!wok) {
+      std::fprintf(stderr, "%s: write failed\n", argv[3]);
+      return 2;
+    }
+    std::printf("snow_run: %lld particles, %ld steps\n", static_cast<long long>(n), steps);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -2,11 +2,12 @@
 //
 // The key bits are split into the fewest passes of at most 11 bits (22-bit keys: two passes).
 // Each pass runs three kernels over tiles of 4096 keys:
-//   k_radix_hist     per-tile digit histogram, written digit-major (hist[d][tile]); each thread
-//                    counts 16 consecutive keys and adds runs of equal digits with one LDS atomic
-//                    (nearly sorted input makes long runs)
-//   k_radix_rowscan  one wave per digit: exclusive scan of its row over the tiles, and the digit
-//                    total
+//   k_radix_hist     per-tile digit histogram, written tile-major (hist[tile][d], one coalesced
+//                    row per tile); each thread counts 16 consecutive keys and adds runs of equal
+//                    digits with one LDS atomic (nearly sorted input makes long runs)
+//   k_radix_colscan  64 digits per workgroup: exclusive scan of each digit's column over the
+//                    tiles (16 groups of tiles per digit, rows read as coalesced 256-byte
+//                    segments), and the digit totals
 //   k_radix_scatter  each wave owns 1024 consecutive keys of the tile and ranks them in input
 //                    order (keys with equal digits in a round of 64 are matched with ballots,
 //                    running per-wave digit counters in LDS carry the rank across rounds); the
@@ -66,30 +67,45 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_hist(const uint32_t* __r
   }
   if (run) atomicAdd(&h[cur], run);
   __syncthreads();
-  for (int d = t; d < bins; d += SORT_THREADS) hist[size_t(d) * ntiles + blockIdx.x] = h[d];
+  for (int d = t; d < bins; d += SORT_THREADS) hist[size_t(blockIdx.x) * bins + d] = h[d];
 }
 
-// One wave per digit row: exclusive scan over the tiles in place, row total to totals[d].
-__global__ void __launch_bounds__(SORT_THREADS) k_radix_rowscan(uint32_t* __restrict__ hist, uint32_t ntiles,
-                                                                int bins, uint32_t* __restrict__ totals) {
-  const int lane = threadIdx.x & 63;
-  const int d = int(blockIdx.x) * SORT_WAVES + int(threadIdx.x >> 6);
-  if (d >= bins) return;  // whole wave
-  uint32_t* row = hist + size_t(d) * ntiles;
-  uint32_t carry = 0u;
-  for (uint32_t c = 0; c < ntiles; c += 64) {
-    const uint32_t i = c + uint32_t(lane);
-    const uint32_t v = i < ntiles ? row[i] : 0u;
-    uint32_t s = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t y = __shfl_up(s, off, 64);
-      if (lane >= off) s += y;
-    }
-    if (i < ntiles) row[i] = carry + s - v;
-    carry += __shfl(s, 63, 64);
+// Exclusive scan of each digit column of the tile-major histogram, in place; column totals to
+// totals[d]. Workgroup g: digits [64g, 64g+64) (lane = digit), COL_PARTS groups of tiles (wave =
+// group).
+constexpr int COL_PARTS = 16;
+__global__ void __launch_bounds__(64 * COL_PARTS) k_radix_colscan(uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                                  int bins, uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_part[COL_PARTS][64];
+  const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
+  const int d = int(blockIdx.x) * 64 + lane;
+  const bool on = d < bins;
+  const uint32_t q = (ntiles + COL_PARTS - 1) / COL_PARTS;
+  const uint32_t b0 = min(ntiles, uint32_t(part) * q), b1 = min(ntiles, b0 + q);
+  uint32_t sum = 0u;
+  if (on) {
+#pragma unroll 8
+    for (uint32_t b = b0; b < b1; ++b) sum += hist[size_t(b) * bins + d];
   }
-  if (lane == 0) totals[d] = carry;
+  s_part[part][lane] = sum;
+  __syncthreads();
+  uint32_t run = 0u, total = 0u;
+#pragma unroll
+  for (int p = 0; p < COL_PARTS; ++p) {
+    const uint32_t v = s_part[p][lane];
+    run += p < part ? v : 0u;
+    total += v;
+  }
+  if (on) {
+#pragma unroll 8
+    for (uint32_t b = b0; b < b1; ++b) {
+      const size_t idx = size_t(b) * bins + d;
+      const uint32_t v = hist[idx];
+      hist[idx] = run;
+      run += v;
+    }
+    if (part == 0) totals[d] = total;
+  }
 }
 
 __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
@@ -151,7 +167,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
   for (int j = 0; j < per; ++j) {
     const int d = d0 + j;
     if (d < bins) {
-      uint32_t acc = run + offsets[size_t(d) * ntiles + blockIdx.x];
+      uint32_t acc = run + offsets[size_t(blockIdx.x) * bins + d];
       run += totals[d];
 #pragma unroll
       for (int w = 0; w < SORT_WAVES; ++w) {
@@ -225,7 +241,7 @@ hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys
     const int bits = eb - shift < dbits ? eb - shift : dbits;
     const int bins = 1 << bits;
     k_radix_hist<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, n, shift, bits, L.ntiles, hist);
-    k_radix_rowscan<<<div_up(bins, SORT_WAVES), SORT_THREADS, 0, stream>>>(hist, L.ntiles, bins, totals);
+    k_radix_colscan<<<div_up(bins, 64), 64 * COL_PARTS, 0, stream>>>(hist, L.ntiles, bins, totals);
     k_radix_scatter<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, vin, ko, vo, n, shift, bits, L.ntiles, hist,
                                                            totals);
     hipError_t e = hipGetLastError();

@@ -146,3 +146,36 @@ def test_import_reference_encodings():
         for r in range(3):
             assert fe[0, c * 3 + r] == F[c, r]
     np.testing.assert_array_equal(fp, np.tile(np.eye(3, dtype=np.float32).reshape(1, 9), (n, 1)))
+
+
+def test_cpp_driver_built_and_checks_input(tmp_path):
+    """The C++ driver over include/snow_time_update.hpp is built and rejects bad input cleanly."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "snow_amd", "snow_run")
+    assert os.access(exe, os.X_OK)
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 2 and "usage" in res.stderr
+    bad = tmp_path / "bad.scn"
+    bad.write_bytes(b"NOTASCENE" + b"\0" * 64)
+    res = subprocess.run([exe, str(bad), "1", str(tmp_path / "o")], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 2 and "SNOWSCN1" in res.stderr
+
+
+def test_scene_file_roundtrip(tmp_path):
+    """PhysicalScene.save writes the SNOWSCN1 layout the C++ driver reads (header sizes, order)."""
+    import numpy as np
+    from snow_amd import scenes
+    sc = scenes.tiny_block(n_side=3, dims=16)
+    p = tmp_path / "t.scn"
+    sc.save(str(p))
+    raw = p.read_bytes()
+    n = len(sc.particles)
+    ncol = len(sc.colliders.colliders)
+    assert raw[:8] == b"SNOWSCN1"
+    assert int(np.frombuffer(raw[8:16], np.int64)[0]) == n
+    assert len(raw) == 8 + 8 + 28 + 40 + 4 + 48 * ncol + 4 + 4 * 7 * n
+    dims = np.frombuffer(raw[16:28], np.int32)
+    assert tuple(dims) == (sc.grid.dimx, sc.grid.dimy, sc.grid.dimz)
+    x = np.frombuffer(raw[len(raw) - 28 * n:len(raw) - 16 * n], np.float32).reshape(n, 3)
+    assert np.array_equal(x, sc.particles.x)

@@ -353,3 +353,26 @@ def test_batched_sweep_matches_oracle(oracle_mod):
         o.step(s.dt, 10)
         assert np.abs(g["x"] - o.x).max() < 1e-6
         assert rel_err(g["v"], o.v) < 1e-4
+
+
+def test_cpp_driver_matches_python_binding(tmp_path, c1_scene):
+    """The C++ step interface (include/snow_time_update.hpp, driven by snow_amd/snow_run) gives the
+    same bits as the ctypes binding for init + 10 steps of the 64^3 snowball drop."""
+    import os
+    import subprocess
+
+    from snow_amd import load_run_output
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "snow_amd", "snow_run")
+    scene_file, out_file = str(tmp_path / "c1.scn"), str(tmp_path / "c1.out")
+    c1_scene.save(scene_file)
+    print(f"\n[cpp driver] {exe} {scene_file} 10", flush=True)
+    res = subprocess.run([exe, scene_file, "10", out_file], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    got = load_run_output(out_file)
+    with SnowContext(c1_scene) as ctx:
+        ctx.init()
+        for _ in range(10):
+            ctx.step(c1_scene.dt)
+        ref = ctx.read_particles()
+    for k in ("x", "v", "fe", "fp"):
+        assert np.array_equal(got[k], ref[k]), k
