@@ -13,6 +13,11 @@ FP32 form's particle-steps/s. The same line also reports
 Prints ONE JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-errors] [--no-sweep]
+                    [--no-second-form]
+
+roofline.traffic is the HBM traffic per launch of the dominant kernel measured with rocprofv3
+FETCH_SIZE / WRITE_SIZE counters (calibrated against a plain copy), read from the committed summary
+profiles/snow_step_two_snowballs_1M_128_hbm_traffic.md; null when the summary has no entry for it.
 """
 from __future__ import annotations
 
@@ -42,6 +47,24 @@ KERNEL_NAMES = {
     "#Sim:Compute Derivatives": "k_grid",
     "#Sim:Transfer to Particles": "k_g2p_advance",
 }
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "snow_step_two_snowballs_1M_128_hbm_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3 counter summary
+    (lines "traffic_bytes_per_launch[<kernel>]: <bytes>"), or None."""
+    import re
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                m = re.match(r"\s*traffic_bytes_per_launch\[(\w+)\]:\s*(\d+)", line)
+                if m and m.group(1) == kernel:
+                    return int(m.group(2))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(scene, budget_s: float = 12.0):
@@ -162,6 +185,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-second-form", action="store_true", help="skip the mixed (or plain) second form")
     ap.add_argument("--only-sweep", action="store_true", help="run only the batched 64-scene sweep")
     ap.add_argument("--error-steps", type=int, default=10)
     ap.add_argument("--sweep-steps", type=int, default=100)
@@ -236,23 +260,24 @@ def main():
         "config": {"workload": "two_snowballs_128^3", "particles": n, "grid": [128, 128, 128], "dt": dt,
                    "form": args.form, "active_nodes": active},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom)},
         "step_roofline_frac": (296 * n + 92 * active) * args.steps / el / (HBM_PEAK_GBS * 1e9),
         "kernels": kernels,
     }
     ctx.close()
 
-    other = FLAG_MIXED if args.form == "plain" else 0
-    with SnowContext(scene, other) as ctx2:
-        ctx2.init()
-        for _ in range(args.warmup):
-            ctx2.step(dt)
-        el2 = timed_steps(ctx2, args.steps, dt)
-        fp64 = ctx2.fp64_particles() if other else 0
-    out["forms"] = {args.form: {"value": value, "unit": "particle-steps/s"},
-                    ("mixed" if other else "plain"): {"value": n * args.steps / el2, "unit": "particle-steps/s",
-                                                      "fp64_particles_last_step": fp64}}
-    print("[bench] second form timed", file=sys.stderr, flush=True)
+    out["forms"] = {args.form: {"value": value, "unit": "particle-steps/s"}}
+    if not args.no_second_form:
+        other = FLAG_MIXED if args.form == "plain" else 0
+        with SnowContext(scene, other) as ctx2:
+            ctx2.init()
+            for _ in range(args.warmup):
+                ctx2.step(dt)
+            el2 = timed_steps(ctx2, args.steps, dt)
+            fp64 = ctx2.fp64_particles() if other else 0
+        out["forms"]["mixed" if other else "plain"] = {"value": n * args.steps / el2, "unit": "particle-steps/s",
+                                                       "fp64_particles_last_step": fp64}
+        print("[bench] second form timed", file=sys.stderr, flush=True)
     if not args.no_errors:
         out["errors"] = {"steps": args.error_steps, "reference": "FP64 CPU oracle",
                          "plain": error_report(scene, 0, args.error_steps),

@@ -382,3 +382,78 @@ def test_fp32_baseline_tracks_fp64(oracle_mod):
         o.init()
         o.step(1e-4, 10)
     assert np.abs(o64.x - o32.x).max() < 1e-6
+
+
+# ---- particleAdvance.glsl: deformation update with the singular-value clamp --------------------
+def affine_grid_scene(oracle_mod, L, crit_comp=2.5e-2, crit_stretch=7.5e-3, fe=None, fp=None):
+    """Particles well inside the grid, v* = L x_i on every node (v_old = v*), physical gradient:
+    the cubic B-spline reproduces affine fields, so the G2P velocity gradient is exactly L."""
+    mat = Material(young=1.4e5, poisson=0.2, hardening=10.0, crit_comp=crit_comp, crit_stretch=crit_stretch,
+                   flip_alpha=0.0, gravity=(0.0, 0.0, 0.0), grad_scale=0.0)
+    sc = small_scene(n_side=3, dims=16, v=(0.0, 0.0, 0.0), material=mat)
+    n = len(sc.particles)
+    if fe is not None:
+        sc.particles.fe = np.tile(np.asarray(fe, np.float32).T.reshape(1, 9), (n, 1))
+    if fp is not None:
+        sc.particles.fp = np.tile(np.asarray(fp, np.float32).T.reshape(1, 9), (n, 1))
+    o = oracle_mod.Oracle(sc)
+    D, h = sc.grid.dimx, sc.grid.h
+    ii, jj, kk = np.meshgrid(np.arange(D), np.arange(D), np.arange(D), indexing="ij")
+    xn = np.stack([ii.transpose(2, 1, 0).ravel(), jj.transpose(2, 1, 0).ravel(), kk.transpose(2, 1, 0).ravel()], 1)
+    xn = xn * h + np.array([sc.grid.x_off, sc.grid.y_off, sc.grid.z_off])
+    vn = xn @ np.asarray(L, np.float64).T
+    o.gm[:] = 1.0
+    o.gmv[:] = vn
+    o.gvstar[:] = vn
+    return sc, o
+
+
+def as_mats(a):
+    return np.array(a, np.float64).reshape(-1, 3, 3).transpose(0, 2, 1)  # column-major storage (a copy)
+
+
+def test_advance_elastic_update_keeps_fp(oracle_mod):
+    """Singular values stay inside [1-theta_c, 1+theta_s]: F_E <- (I + dt L) F_E and F_P unchanged
+    (up to the 1e-5 rounding of V and the small off-diagonals of the QR factor S)."""
+    L = np.array([[0.3, 0.1, 0.0], [-0.2, 0.1, 0.05], [0.0, 0.1, -0.2]])
+    fp0 = np.diag([1.002, 0.999, 1.0])
+    sc, o = affine_grid_scene(oracle_mod, L, fp=fp0)
+    fe0 = as_mats(o.fe)
+    dt = float(np.float32(1e-3))
+    o.g2p_advance(dt)
+    expect = (np.eye(3) + dt * L)[None] @ fe0
+    np.testing.assert_allclose(as_mats(o.fe), expect, atol=5e-5)
+    np.testing.assert_allclose(as_mats(o.fp), np.broadcast_to(fp0, expect.shape), atol=5e-5)
+    assert np.abs(np.linalg.det(as_mats(o.fp)) - np.linalg.det(fp0)).max() < 1e-4
+
+
+def test_advance_clamp_moves_stretch_into_fp(oracle_mod):
+    """A stretch beyond 1+theta_s: F_E's singular values end clamped to [1-theta_c, 1+theta_s] and
+    the product F_E F_P still equals (I + dt L) F_E0 F_P0 (plastic flow conserves the total F)."""
+    L = np.diag([20.0, -40.0, 0.0])
+    theta_c, theta_s = 2.5e-2, 7.5e-3
+    sc, o = affine_grid_scene(oracle_mod, L, crit_comp=theta_c, crit_stretch=theta_s)
+    F0 = as_mats(o.fe) @ as_mats(o.fp)
+    dt = float(np.float32(1e-3))
+    o.g2p_advance(dt)
+    fe, fp = as_mats(o.fe), as_mats(o.fp)
+    s = np.linalg.svd(fe, compute_uv=False)
+    assert s.max() <= 1 + theta_s + 1e-5 and s.min() >= 1 - theta_c - 1e-5
+    # the unclamped singular values (1.02 / 0.96) were really clamped
+    assert abs(s.max() - (1 + theta_s)) < 1e-4 and abs(s.min() - (1 - theta_c)) < 1e-4
+    np.testing.assert_allclose(fe @ fp, (np.eye(3) + dt * L)[None] @ F0, atol=1e-4)
+    # the plastic part carries the excess: det F_P = det F / det F_E
+    np.testing.assert_allclose(np.linalg.det(fp), np.linalg.det(fe @ fp) / np.linalg.det(fe), rtol=1e-6)
+
+
+def test_advance_rigid_rotation_rate_keeps_singular_values(oracle_mod):
+    """A skew L (rotation rate): F_E's singular values are unchanged to first order and F_P stays."""
+    w = 0.5
+    L = np.array([[0.0, -w, 0.0], [w, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    sc, o = affine_grid_scene(oracle_mod, L, fe=np.diag([1.005, 0.99, 1.0]))
+    s0 = np.linalg.svd(as_mats(o.fe), compute_uv=False)
+    dt = float(np.float32(1e-4))
+    o.g2p_advance(dt)
+    s1 = np.linalg.svd(as_mats(o.fe), compute_uv=False)
+    np.testing.assert_allclose(s1, s0, atol=5e-5)
+    np.testing.assert_allclose(as_mats(o.fp), np.broadcast_to(np.eye(3), s1.shape[:1] + (3, 3)), atol=5e-5)
