@@ -3,7 +3,7 @@
  *
  * GL-free restatement of the reference's simulation-step classes:
  *   TimeUpdate { virtual void init() = 0; virtual void update(double dt) = 0; }
- *       (src/snow/simulation/timeUpdate/timeUpdate.hpp:10-18)
+ *       (src/snow/simulation/timeUpdate.hpp:10-18)
  *   ExplicitTimeUpdate::init / update
  *       (src/snow/simulation/timeUpdate/explicittimeupdate.cpp:4-63)
  *   physicEngine / MPMPhysicEngine { bool init(); void update(double); }
