@@ -186,7 +186,7 @@ class PhysicalScene:
 
     def save(self, filename: str) -> None:
         """Write the scene in the SNOWSCN1 format read by the C++ driver (snow_amd/csrc/snow_run.cpp):
-        identity F_E / F_P, the grid, material, colliders, dt and the particles' x, v, mass."""
+        the grid, material, colliders, dt and the particles' x, v, mass, F_E, F_P."""
         g, m = self.grid.to_c(), self.material.to_c()
         cols = [c.to_c() for c in self.colliders.colliders]
         ps = self.particles
@@ -199,7 +199,7 @@ class PhysicalScene:
             for c in cols:
                 f.write(bytes(c))
             f.write(np.float32(self.dt).tobytes())
-            for a in (ps.x, ps.v, ps.mass):
+            for a in (ps.x, ps.v, ps.mass, ps.fe, ps.fp):
                 f.write(np.ascontiguousarray(a, np.float32).tobytes())
 
 

@@ -92,7 +92,9 @@ struct snow_ctx {
   StepCtl* d_ctl = nullptr;
   uint32_t* d_tile_stamp = nullptr;
   int num_cu = 256;
-  float4 *d_g_mmv = nullptr, *d_g_f = nullptr, *d_g_node = nullptr;  // g_node: 2 float4 per node
+  float4 *d_g_mmv = nullptr, *d_g_f = nullptr;
+  float2* d_g_node = nullptr;  // 3 float2 per node
+  float* d_g_mass = nullptr;   // node mass of the last grid update
   Collider* d_cols = nullptr;
   unsigned long long* d_counts = nullptr;  // [nscenes] active nodes, then [1] fp64 count
   int* d_bad = nullptr;
@@ -316,7 +318,7 @@ int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
   const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu * 4)));
 #define GRID_ARGS                                                                                          \
   c->d_scenes, c->d_tile_scene, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_ctl, c->d_tile_stamp,     \
-      c->d_tpart, c->d_zero_part, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_cols, dt, cnt
+      c->d_tpart, c->d_zero_part, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass, c->d_cols, dt, cnt
   if (keep && update)
     k_grid<true, true><<<blocks, GRID_THREADS, 0, c->stream>>>(GRID_ARGS);
   else if (update)
@@ -379,7 +381,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_sort_tmp,
-                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_ctl, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node,
+                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_ctl, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -569,7 +571,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_tile_stamp, c->total_ctiles));
   CREATECHK(dalloc(&c->d_g_mmv, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_g_f, size_t(c->total_nodes)));
-  CREATECHK(dalloc(&c->d_g_node, 2 * size_t(c->total_nodes)));
+  CREATECHK(dalloc(&c->d_g_node, 3 * size_t(c->total_nodes)));
+  CREATECHK(dalloc(&c->d_g_mass, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_cols, std::max<size_t>(1, c->cols.size())));
   CREATECHK(dalloc(&c->d_counts, size_t(c->nscenes + 1)));
   CREATECHK(dalloc(&c->d_bad, 1));
@@ -635,7 +638,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(hipMemset(c->d_tile_stamp, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_g_mmv, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_f, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
-  CREATECHK(hipMemset(c->d_g_node, 0, 2 * sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
+  CREATECHK(hipMemset(c->d_g_node, 0, 3 * sizeof(float2) * std::max<int64_t>(1, c->total_nodes)));
+  CREATECHK(hipMemset(c->d_g_mass, 0, sizeof(float) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_counts, 0, sizeof(unsigned long long) * (c->nscenes + 1)));
   CREATECHK(hipDeviceSynchronize());
 #undef CREATECHK
@@ -865,10 +869,14 @@ int snow_stage_rigid_advance(snow_ctx* c, double dt) {
 int snow_stage_grid_reset(snow_ctx* c) {
   if (!c) return SNOW_E_ARG;
   StageTimer tm(c, ST_RESET);
-  const int64_t words = c->total_nodes * 4;
-  for (int k = 0; k < 3; ++k) {
-    uint32_t* p = reinterpret_cast<uint32_t*>(k == 0 ? c->d_g_mmv : (k == 1 ? c->d_g_f : c->d_g_node));
-    const int64_t w = k == 2 ? 2 * words : words;
+  // g_mmv, g_f: 4 words per node; g_node: 6 words per node; g_mass: 1 word per node
+  const int64_t nodes = c->total_nodes;
+  for (int k = 0; k < 4; ++k) {
+    uint32_t* p = k == 0 ? reinterpret_cast<uint32_t*>(c->d_g_mmv)
+                         : k == 1 ? reinterpret_cast<uint32_t*>(c->d_g_f)
+                                  : k == 2 ? reinterpret_cast<uint32_t*>(c->d_g_node)
+                                           : reinterpret_cast<uint32_t*>(c->d_g_mass);
+    const int64_t w = nodes * (k < 2 ? 4 : (k == 2 ? 6 : 1));
     if (w) {
       k_fill_u32<<<grid_for(w, 256), 256, 0, c->stream>>>(p, 0u, w);
       LAUNCHCHK(c);
@@ -900,7 +908,7 @@ int snow_stage_grid_update(snow_ctx* c, double dt) {
   StageTimer tm(c, ST_GRID);
   if (c->total_nodes) {
     k_grid_update_dense<<<grid_for(c->total_nodes, 256), 256, 0, c->stream>>>(
-        c->d_scenes, c->nscenes, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_cols, fdt, c->total_nodes);
+        c->d_scenes, c->nscenes, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass, c->d_cols, fdt, c->total_nodes);
     LAUNCHCHK(c);
   }
   return tm.done();
@@ -1004,7 +1012,8 @@ int snow_read_grid(snow_ctx* c, int32_t s, float* m, float* mv, float* f, float*
   // With keep off, the mass is read from the node records.
   k_grid_export<<<grid_for(nn, 256), 256, 0, c->stream>>>(c->d_scenes, s, c->d_tile_stamp, c->d_ctl,
                                                           c->grid_state == 2,
-                                                          c->d_g_mmv, c->d_g_f, c->d_g_node, c->keep_valid ? 0 : 1,
+                                                          c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
+                                                          c->keep_valid ? 0 : 1,
                                                           om, omv, of, ovs);
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1040,7 +1049,8 @@ int snow_write_grid(snow_ctx* c, int32_t s, const float* m, const float* mv, con
   HIPCHK(c, hipMemcpy(c->d_stage, vs.data(), sizeof(float4) * nn, hipMemcpyHostToDevice));
   if (nn) {
     k_grid_import<<<grid_for(nn, 256), 256, 0, c->stream>>>(c->d_scenes, s, c->d_g_mmv,
-                                                            static_cast<const float4*>(c->d_stage), c->d_g_node);
+                                                            static_cast<const float4*>(c->d_stage), c->d_g_node,
+                                                            c->d_g_mass);
     LAUNCHCHK(c);
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));

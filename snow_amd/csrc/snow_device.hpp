@@ -100,29 +100,34 @@ template <class R>
 __device__ __forceinline__ void jacobi_conj(int x, int y, int z, R& s00, R& s11, R& s22, R& s01,
                                             R& s02, R& s12, R q[4]) {
   const R GAMMA = R(5.828427124), CSTAR = R(0.923879532), SSTAR = R(0.3826834323);
-  R ch = R(2) * (s00 - s11), ch2 = ch * ch;
-  R sh = s01, sh2 = sh * sh;
+  // cos(2t), sin(2t) of the fixed angle pi/8 branch, evaluated as the reference does
+  const R ASTAR = (CSTAR * CSTAR - SSTAR * SSTAR) / (CSTAR * CSTAR + SSTAR * SSTAR);
+  const R BSTAR = (R(2) * SSTAR * CSTAR) / (CSTAR * CSTAR + SSTAR * SSTAR);
+  const R ch = R(2) * (s00 - s11), sh = s01;
+  const R ch2 = ch * ch, sh2 = sh * sh;
   const bool flag = GAMMA * sh2 < ch2;
   const R w = rsq(ch2 + sh2);
-  ch = flag ? w * ch : CSTAR; ch2 = ch * ch;
-  sh = flag ? w * sh : SSTAR; sh2 = sh * sh;
-  const R scale = recip(ch2 + sh2);
-  const R a = (ch2 - sh2) * scale;
-  const R b = (R(2) * sh * ch) * scale;
+  // the reference normalises (ch, sh) with w and then divides by ch^2 + sh^2 of the normalised
+  // pair (SVD.include.glsl:83-92); with w^2 = 1 / (ch^2 + sh^2) both steps fold into one scale
+  const R w2 = w * w;
+  const R chn = flag ? w * ch : CSTAR;
+  const R shn = flag ? w * sh : SSTAR;
+  const R a = flag ? (ch2 - sh2) * w2 : ASTAR;
+  const R b = flag ? R(2) * (sh * ch) * w2 : BSTAR;
   const R a2 = a * a, b2 = b * b, ab = a * b;
   // S[1][0]=s01, S[2][0]=s02, S[2][1]=s12, S[1][2]=s12, S[0][2]=s02
   const R n0 = a2 * s00 + R(2) * ab * s01 + b2 * s11;
   const R n2 = a * s02 + b * s12;
   const R n3 = (a2 - b2) * s01 + ab * (s11 - s00);
-  const R n4 = b2 * s00 - R(2) * ab * s01 + a2 * s11;
+  const R n4 = (s00 + s11) - n0;  // the rotation keeps the trace of the (0,1) block
   const R n5 = a * s12 - b * s02;
   const R n8 = s22;
   // S = mat3(n4, n5, n3, n5, n8, n2, n3, n2, n0)
   s00 = n4; s11 = n8; s22 = n0; s01 = n5; s02 = n3; s12 = n2;
-  const R t1 = sh * q[1], t2 = sh * q[2], t3 = sh * q[3];
+  const R t1 = shn * q[1], t2 = shn * q[2], t3 = shn * q[3];
   const R tmp[3] = {t1, t2, t3};
-  const R shq = sh * q[0];
-  q[0] *= ch; q[1] *= ch; q[2] *= ch; q[3] *= ch;
+  const R shq = shn * q[0];
+  q[0] *= chn; q[1] *= chn; q[2] *= chn; q[3] *= chn;
   q[z + 1] += shq;
   q[0] -= tmp[z];
   q[x + 1] += tmp[y];

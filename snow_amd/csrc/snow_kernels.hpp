@@ -7,7 +7,7 @@
 //    fe/fp    : 9 SoA float planes, plane e = c*3+r (column-major 3x3)
 //    rho0,pid : rest density (pv.w of the reference), original index
 //  grid, dense per scene (node i + j*Dx + k*Dx*Dy):
-//    g_node   : 2 x float4 per node: (v*.x v*.y v*.z u.x | u.y u.z m 0), u = v* - alpha p/m,
+//    g_node   : 3 x float2 per node: (v*.x v*.y | v*.z u.x | u.y u.z), u = v* - alpha p/m,
 //               the gForceCompute output gvn and the FLIP/PIC blend term read together by G2P
 //    g_mmv,g_f: float4 (p, m), (f, 0)    - gv / gf of the reference (stage API, debug)
 //  cell tiles: 4x4x4 cells per tile over the shifted cell range c = base + 2, base in [-2, D]
@@ -109,11 +109,14 @@ __global__ void __launch_bounds__(256) k_keys(const SceneDev* __restrict__ scene
 
 
 // Node record written by the grid update and read by G2P: v* (gvn of gForceCompute.glsl) and the
-// FLIP/PIC term u = v* - alpha p/m of g2pTransfer.glsl:100, packed as two aligned float4.
-__device__ __forceinline__ void store_node(float4* __restrict__ g, int64_t n, float vx, float vy, float vz, float ux,
-                                           float uy, float uz, float m) {
-  g[2 * n] = make_float4(vx, vy, vz, ux);
-  g[2 * n + 1] = make_float4(uy, uz, m, 0.0f);
+// FLIP/PIC term u = v* - alpha p/m of g2pTransfer.glsl:100, packed as three float2 (24 bytes;
+// a row of 4 nodes along x is 96 contiguous bytes). The node mass is kept apart in g_mass.
+__device__ __forceinline__ void store_node(float2* __restrict__ g, float* __restrict__ g_mass, int64_t n, float vx,
+                                           float vy, float vz, float ux, float uy, float uz, float m) {
+  g[3 * n] = make_float2(vx, vy);
+  g[3 * n + 1] = make_float2(vz, ux);
+  g[3 * n + 2] = make_float2(uy, uz);
+  g_mass[n] = m;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -510,7 +513,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
                                                        const double* __restrict__ tpart,
                                                        const double* __restrict__ zero_part,
                                                        float4* __restrict__ g_mmv, float4* __restrict__ g_f,
-                                                       float4* __restrict__ g_node,
+                                                       float2* __restrict__ g_node, float* __restrict__ g_mass,
                                                        const Collider* __restrict__ cols, float dt_override,
                                                        unsigned long long* __restrict__ active_count) {
   const uint32_t step = ctl->step;
@@ -594,7 +597,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
         by = vy - S.alpha * (py / m);
         bz = vz - S.alpha * (pz / m);
       }
-      store_node(g_node, node, vx, vy, vz, bx, by, bz, m);
+      store_node(g_node, g_mass, node, vx, vy, vz, bx, by, bz, m);
     }
     if (active_count && m > 0.0f) atomicAdd(active_count + sid, 1ull);
   }
@@ -605,7 +608,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
 __global__ void __launch_bounds__(256) k_grid_update_dense(const SceneDev* __restrict__ scenes, int nscenes,
                                                            const float4* __restrict__ g_mmv,
                                                            const float4* __restrict__ g_f,
-                                                           float4* __restrict__ g_node,
+                                                           float2* __restrict__ g_node, float* __restrict__ g_mass,
                                                            const Collider* __restrict__ cols, float dt_override,
                                                            int64_t total_nodes) {
   const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -629,13 +632,14 @@ __global__ void __launch_bounds__(256) k_grid_update_dense(const SceneDev* __res
     by = vy - S.alpha * (mm.y / m);
     bz = vz - S.alpha * (mm.z / m);
   }
-  store_node(g_node, node, vx, vy, vz, bx, by, bz, m);
+  store_node(g_node, g_mass, node, vx, vy, vz, bx, by, bz, m);
 }
 
 // From dense (m, p) and a host-written v* (float4 (v*, -) per node of one scene): the G2P inputs.
 __global__ void __launch_bounds__(256) k_grid_import(const SceneDev* __restrict__ scenes, int sid,
                                                      const float4* __restrict__ g_mmv,
-                                                     const float4* __restrict__ vstar, float4* __restrict__ g_node) {
+                                                     const float4* __restrict__ vstar, float2* __restrict__ g_node,
+                                                     float* __restrict__ g_mass) {
   const SceneDev& S = scenes[sid];
   const int64_t nn = int64_t(S.D[0]) * S.D[1] * S.D[2];
   const int64_t l = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -645,10 +649,10 @@ __global__ void __launch_bounds__(256) k_grid_import(const SceneDev* __restrict_
   const float m = mm.w;
   if (m > 0.0f) {
     const float4 vs = vstar[l];
-    store_node(g_node, node, vs.x, vs.y, vs.z, vs.x - S.alpha * (mm.x / m), vs.y - S.alpha * (mm.y / m),
+    store_node(g_node, g_mass, node, vs.x, vs.y, vs.z, vs.x - S.alpha * (mm.x / m), vs.y - S.alpha * (mm.y / m),
                vs.z - S.alpha * (mm.z / m), m);
   } else {
-    store_node(g_node, node, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, m);  // gvn = 0 where m <= 0 (gForceCompute.glsl:251-253)
+    store_node(g_node, g_mass, node, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, m);  // gvn = 0 where m <= 0 (gForceCompute.glsl:251-253)
   }
 }
 
@@ -657,7 +661,8 @@ __global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict_
                                                      const uint32_t* __restrict__ tile_stamp,
                                                      const StepCtl* __restrict__ ctl, int dense_valid,
                                                      const float4* __restrict__ g_mmv, const float4* __restrict__ g_f,
-                                                     const float4* __restrict__ g_node, int mass_from_node,
+                                                     const float2* __restrict__ g_node,
+                                                     const float* __restrict__ g_mass, int mass_from_node,
                                                      float* __restrict__ out_m,
                                                      float* __restrict__ out_mv, float* __restrict__ out_f,
                                                      float* __restrict__ out_vs) {
@@ -679,8 +684,10 @@ __global__ void __launch_bounds__(256) k_grid_export(const SceneDev* __restrict_
   const int64_t node = S.node_off + l;
   const float4 mm = live ? g_mmv[node] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 ff = live ? g_f[node] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 vs = live ? g_node[2 * node] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float mass = mass_from_node ? (live ? g_node[2 * node + 1].z : 0.f) : mm.w;
+  const float2 v01 = live ? g_node[3 * node] : make_float2(0.f, 0.f);
+  const float2 v2u = live ? g_node[3 * node + 1] : make_float2(0.f, 0.f);
+  const float4 vs = make_float4(v01.x, v01.y, v2u.x, 0.f);
+  const float mass = mass_from_node ? (live ? g_mass[node] : 0.f) : mm.w;
   if (out_m) out_m[l] = mass;
   if (out_mv) { out_mv[3 * l] = mm.x; out_mv[3 * l + 1] = mm.y; out_mv[3 * l + 2] = mm.z; }
   if (out_f) { out_f[3 * l] = ff.x; out_f[3 * l + 1] = ff.y; out_f[3 * l + 2] = ff.z; }
@@ -708,18 +715,18 @@ __device__ __forceinline__ int axis_weights(float u, float w[4], float q[4]) {
 // and the m_i > 0 test of g2pTransfer.glsl:99 needs no branch. Each (y,z) row of the stencil is
 // reduced along x first (separable weights); interior particles (whole stencil in the grid)
 // take an unchecked path with one row address per (y,z) and immediate x offsets.
-__device__ __forceinline__ void g2p_row(const float4* __restrict__ rowp, const float wx[4], const float qx[4],
+__device__ __forceinline__ void g2p_row(const float2* __restrict__ rowp, const float wx[4], const float qx[4],
                                         float wyz, float qyz, float wqz, float vpn[3], float dvp[9], int xlo,
                                         int xhi) {
   float sq0 = 0.f, sq1 = 0.f, sq2 = 0.f, sw0 = 0.f, sw1 = 0.f, sw2 = 0.f, fw0 = 0.f, fw1 = 0.f, fw2 = 0.f;
 #pragma unroll
   for (int ox = 0; ox < 4; ++ox) {
     if (ox < xlo || ox > xhi) continue;
-    const float4 a = rowp[2 * ox];
-    const float4 b = rowp[2 * ox + 1];
-    sq0 = fmaf(a.x, qx[ox], sq0); sq1 = fmaf(a.y, qx[ox], sq1); sq2 = fmaf(a.z, qx[ox], sq2);
-    sw0 = fmaf(a.x, wx[ox], sw0); sw1 = fmaf(a.y, wx[ox], sw1); sw2 = fmaf(a.z, wx[ox], sw2);
-    fw0 = fmaf(a.w, wx[ox], fw0); fw1 = fmaf(b.x, wx[ox], fw1); fw2 = fmaf(b.y, wx[ox], fw2);
+    const float2 n0 = rowp[3 * ox], n1 = rowp[3 * ox + 1], n2 = rowp[3 * ox + 2];
+    // v* = (n0.x, n0.y, n1.x), u = (n1.y, n2.x, n2.y)
+    sq0 = fmaf(n0.x, qx[ox], sq0); sq1 = fmaf(n0.y, qx[ox], sq1); sq2 = fmaf(n1.x, qx[ox], sq2);
+    sw0 = fmaf(n0.x, wx[ox], sw0); sw1 = fmaf(n0.y, wx[ox], sw1); sw2 = fmaf(n1.x, wx[ox], sw2);
+    fw0 = fmaf(n1.y, wx[ox], fw0); fw1 = fmaf(n2.x, wx[ox], fw1); fw2 = fmaf(n2.y, wx[ox], fw2);
   }
   vpn[0] = fmaf(fw0, wyz, vpn[0]); vpn[1] = fmaf(fw1, wyz, vpn[1]); vpn[2] = fmaf(fw2, wyz, vpn[2]);
   dvp[0] = fmaf(sq0, wyz, dvp[0]); dvp[1] = fmaf(sq1, wyz, dvp[1]); dvp[2] = fmaf(sq2, wyz, dvp[2]);
@@ -728,7 +735,7 @@ __device__ __forceinline__ void g2p_row(const float4* __restrict__ rowp, const f
 }
 
 __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, float z,
-                                           const float4* __restrict__ g_node, float vpn[3], float dvp[9],
+                                           const float2* __restrict__ g_node, float vpn[3], float dvp[9],
                                            bool compensated) {
   vpn[0] = vpn[1] = vpn[2] = 0.f;
 #pragma unroll
@@ -756,8 +763,9 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
         for (int ox = 0; ox < 4; ++ox) {
           const int ix = bx + ox - 1;
           if (ix < 0 || ix >= D0) continue;
-          const float4 a = g_node[2 * (row + ix)];
-          const float4 b = g_node[2 * (row + ix) + 1];
+          const float2 n0 = g_node[3 * (row + ix)], n1 = g_node[3 * (row + ix) + 1], n2 = g_node[3 * (row + ix) + 2];
+          const float4 a = make_float4(n0.x, n0.y, n1.x, n1.y);
+          const float4 b = make_float4(n2.x, n2.y, 0.f, 0.f);
           const float w = wx[ox] * wyz;
           const float g0 = qx[ox] * wyz, g1 = wx[ox] * qyz, g2 = wx[ox] * wqz;
           const float c[12] = {a.w * w, b.x * w, b.y * w, a.x * g0, a.y * g0, a.z * g0,
@@ -774,8 +782,8 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
 #pragma unroll
     for (int k = 0; k < 9; ++k) dvp[k] += ld[k];
   } else if (interior) {
-    const float4* p0 = g_node + 2 * (S.node_off + (bx - 1) + int64_t(D0) * ((by - 1) + int64_t(D1) * (bz - 1)));
-    const int64_t sy = 2 * int64_t(D0), sz = 2 * int64_t(D0) * D1;
+    const float2* p0 = g_node + 3 * (S.node_off + (bx - 1) + int64_t(D0) * ((by - 1) + int64_t(D1) * (bz - 1)));
+    const int64_t sy = 3 * int64_t(D0), sz = 3 * int64_t(D0) * D1;
 #pragma unroll 1
     for (int oz = 0; oz < 4; ++oz)
 #pragma unroll
@@ -789,7 +797,7 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
       for (int oy = 0; oy < 4; ++oy) {
         const int iy = by + oy - 1;
         if (iy < 0 || iy >= D1) continue;
-        const float4* rowp = g_node + 2 * (S.node_off + (bx - 1) + int64_t(D0) * (iy + int64_t(D1) * iz));
+        const float2* rowp = g_node + 3 * (S.node_off + (bx - 1) + int64_t(D0) * (iy + int64_t(D1) * iz));
         g2p_row(rowp, wx, qx, wy[oy] * wz[oz], qy[oy] * wz[oz], wy[oy] * qz[oz], vpn, dvp, xlo, xhi);
       }
     }
@@ -808,10 +816,10 @@ __device__ __forceinline__ void deform_update(const float dvp[9], float dt, cons
 #pragma unroll
   for (int i = 0; i < 9; ++i) { L[i] = R(dt) * R(dvp[i]); FE[i] = R(FEf[i]); FP[i] = R(FPf[i]); }
   L[0] += R(1); L[4] += R(1); L[8] += R(1);
-  R FEn[9], FEFP[9], Fn[9];
+  // F_E' = (I + dt L) F_E and F' = (I + dt L) F_E F_P = F_E' F_P (particleAdvance.glsl:95-96)
+  R FEn[9], Fn[9];
   mul(L, FE, FEn);
-  mul(FE, FP, FEFP);
-  mul(L, FEFP, Fn);
+  mul(FEn, FP, Fn);
   R W[9], S[9], V[9];
   svd3(FEn, W, S, V);
   const R lb = R(1) - R(crit_c), ub = R(1) + R(crit_s);
@@ -872,7 +880,7 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
     const float* __restrict__ rho_in, const uint32_t* __restrict__ pid_in, float4* __restrict__ rec_out,
     float* __restrict__ fe_out, float* __restrict__ fp_out, float* __restrict__ rho_out,
     uint32_t* __restrict__ pid_out, uint32_t* __restrict__ keys_next, uint32_t* __restrict__ vals_next,
-    const float4* __restrict__ g_node, const Collider* __restrict__ cols,
+    const float2* __restrict__ g_node, const Collider* __restrict__ cols,
     float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -937,7 +945,7 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
 // the advance alone consumes them in place (g2pTransfer.glsl / particleAdvance.glsl).
 __global__ void __launch_bounds__(256) k_g2p_only(const SceneDev* __restrict__ scenes,
                                                   const uint16_t* __restrict__ pscene,
-                                                  const float4* __restrict__ rec, const float4* __restrict__ g_node,
+                                                  const float4* __restrict__ rec, const float2* __restrict__ g_node,
                                                   float* __restrict__ aux,
                                                   uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

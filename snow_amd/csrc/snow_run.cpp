@@ -6,7 +6,7 @@
 //
 // SCENE_FILE (little endian): "SNOWSCN1", int64 n, int32 dim[3], float origin[3], float h,
 //   snow_material (10 floats), int32 ncol, ncol x snow_collider (12 floats), float dt,
-//   float x[3n], float v[3n], float mass[n]
+//   float x[3n], float v[3n], float mass[n], float fe[9n], float fp[9n] (column-major 3x3)
 // OUT_FILE: "SNOWOUT1", int64 n, float x[3n], v[3n], fe[9n], fp[9n] after init + STEPS steps.
 #include <cstdio>
 #include <cstdlib>
@@ -52,9 +52,10 @@ int main(int argc, char** argv) {
             ncol >= 0 && ncol <= 4096;
   std::vector<snow_collider> cols(size_t(ncol > 0 ? ncol : 0));
   const size_t nn = static_cast<size_t>(n);
-  std::vector<float> x(3 * nn), v(3 * nn), m(nn);
+  std::vector<float> x(3 * nn), v(3 * nn), m(nn), fe(9 * nn), fp(9 * nn);
   ok = ok && read_into(f, cols.data(), cols.size()) && read_into(f, &d.dt, 1) && read_into(f, x.data(), x.size()) &&
-       read_into(f, v.data(), v.size()) && read_into(f, m.data(), m.size());
+       read_into(f, v.data(), v.size()) && read_into(f, m.data(), m.size()) && read_into(f, fe.data(), fe.size()) &&
+       read_into(f, fp.data(), fp.size());
   std::fclose(f);
   if (!ok) {
     std::fprintf(stderr, "%s: not a SNOWSCN1 scene file\n", argv[1]);
@@ -64,6 +65,8 @@ int main(int argc, char** argv) {
   d.x = x.data();
   d.v = v.data();
   d.mass = m.data();
+  d.fe = fe.data();
+  d.fp = fp.data();
   d.num_colliders = ncol;
   d.colliders = cols.data();
 

@@ -174,8 +174,10 @@ def test_scene_file_roundtrip(tmp_path):
     ncol = len(sc.colliders.colliders)
     assert raw[:8] == b"SNOWSCN1"
     assert int(np.frombuffer(raw[8:16], np.int64)[0]) == n
-    assert len(raw) == 8 + 8 + 28 + 40 + 4 + 48 * ncol + 4 + 4 * 7 * n
+    assert len(raw) == 8 + 8 + 28 + 40 + 4 + 48 * ncol + 4 + 4 * 25 * n
     dims = np.frombuffer(raw[16:28], np.int32)
     assert tuple(dims) == (sc.grid.dimx, sc.grid.dimy, sc.grid.dimz)
-    x = np.frombuffer(raw[len(raw) - 28 * n:len(raw) - 16 * n], np.float32).reshape(n, 3)
+    x = np.frombuffer(raw[len(raw) - 100 * n:len(raw) - 88 * n], np.float32).reshape(n, 3)
     assert np.array_equal(x, sc.particles.x)
+    fp = np.frombuffer(raw[len(raw) - 36 * n:], np.float32).reshape(n, 9)
+    assert np.array_equal(fp, sc.particles.fp)
