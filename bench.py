@@ -8,12 +8,17 @@ FP32 form's particle-steps/s. The same line also reports
   * errors         - max/RMS error of x, v, F_E, F_P and grid m, mv against the FP64 CPU oracle
                      after a few steps, for both forms,
   * forms.batched  - the 64-scene parameter sweep as one batch (captured step graph, autotuned
-                     launch shapes) against the same scenes stepped one context at a time,
+                     launch shapes) against the same scenes run with the single-scene path, one
+                     scene after another and all interleaved on their own streams,
+  * forms.stress_slab - the 2^24-particle slab impact on 256^3 (plastic clamp active), plain and
+                     mixed throughput, and both forms' errors against the FP64 oracle on a 2^18-
+                     particle version,
+  * forms.large_batch - (--large) 512 small scenes plus one 2^25-particle 512^3 scene in one graph,
   * cpu_baseline   - the FP32 CPU restatement of the step on the host cores.
 Prints ONE JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-errors] [--no-sweep]
-                    [--no-second-form]
+                    [--no-second-form] [--no-stress] [--large]
 
 roofline.traffic is the HBM traffic per launch of the dominant kernel measured with rocprofv3
 FETCH_SIZE / WRITE_SIZE counters (calibrated against a plain copy), read from the committed summary
@@ -99,34 +104,46 @@ def timed_steps(ctx, steps: int, dt=None, graph: bool = False) -> float:
     return time.perf_counter() - t0
 
 
-def error_report(scene, flags, steps: int) -> dict:
-    """Max / RMS error of every quantity against the FP64 oracle after `steps` steps."""
+def error_reports(scene, forms: dict, steps: int) -> dict:
+    """Max / RMS error of every quantity against the FP64 oracle after `steps` steps, for each
+    {name: flags} form; also the fraction of particles whose F_E ends at the singular-value clamp
+    (oracle) and, for forms with FLAG_MIXED, the fraction that took the FP64 path in the last step."""
     import numpy as np
     from oracle import pyoracle
-    from snow_amd import FLAG_KEEP_GRID, SnowContext
-    with SnowContext(scene, flags | FLAG_KEEP_GRID) as ctx:
-        ctx.init()
-        for _ in range(steps):
-            ctx.step(scene.dt)
-        p = ctx.read_particles()
-        g = ctx.read_grid(fields=("m", "mv"))
+    from snow_amd import FLAG_KEEP_GRID, FLAG_MIXED, SnowContext
     o = pyoracle.Oracle(scene, nthreads=min(16, os.cpu_count() or 1))
     o.init()
     o.step(scene.dt, steps)
-    out = {}
-    for name, a, b in (("x", p["x"], o.x), ("v", p["v"], o.v), ("F_E", p["fe"], o.fe), ("F_P", p["fp"], o.fp),
-                       ("grid_m", g["m"], o.gm), ("grid_mv", g["mv"], o.gmv)):
-        d = np.asarray(a, np.float64) - b
-        if d.ndim > 1 and name in ("F_E", "F_P"):
-            d = np.sqrt((d ** 2).sum(-1))  # Frobenius per particle
-        elif d.ndim > 1:
-            d = np.sqrt((d ** 2).sum(-1))
-        out[name] = {"max": float(np.abs(d).max()), "rms": float(np.sqrt((d ** 2).mean()))}
+    mat = scene.material
+    sv = np.linalg.svd(o.fe.reshape(-1, 3, 3), compute_uv=False)
+    at_clamp = (np.abs(sv - (1 - mat.crit_comp)) < 1e-6) | (np.abs(sv - (1 + mat.crit_stretch)) < 1e-6)
+    out = {"steps": steps, "particles": len(scene.particles), "reference": "FP64 CPU oracle",
+           "clamped_fraction": float(at_clamp.any(-1).mean())}
+    for form, flags in forms.items():
+        with SnowContext(scene, flags | FLAG_KEEP_GRID) as ctx:
+            ctx.init()
+            for _ in range(steps):
+                ctx.step(scene.dt)
+            p = ctx.read_particles()
+            g = ctx.read_grid(fields=("m", "mv"))
+            fp64 = ctx.fp64_particles() if flags & FLAG_MIXED else None
+        rep = {}
+        for name, a, b in (("x", p["x"], o.x), ("v", p["v"], o.v), ("F_E", p["fe"], o.fe), ("F_P", p["fp"], o.fp),
+                           ("grid_m", g["m"], o.gm), ("grid_mv", g["mv"], o.gmv)):
+            d = np.asarray(a, np.float64) - b
+            if d.ndim > 1:
+                d = np.sqrt((d ** 2).sum(-1))  # vector norm, Frobenius norm for F_E / F_P
+            rep[name] = {"max": float(np.abs(d).max()), "rms": float(np.sqrt((d ** 2).mean()))}
+        if fp64 is not None:
+            rep["fp64_fraction_last_step"] = fp64 / len(scene.particles)
+        out[form] = rep
     return out
 
 
 def batched_sweep(steps: int) -> dict:
-    """64 scenes in one batch (graph + autotune) vs the same scenes one context at a time."""
+    """64 scenes in one batch (graph + autotune) vs the same scenes run with the single-scene path:
+    one context per scene, (a) one scene after another for all steps and (b) all contexts stepped
+    together, each on its own stream."""
     from snow_amd import SnowContext, scenes
     sc = scenes.parameter_sweep()
     n = sum(len(s.particles) for s in sc)
@@ -142,37 +159,75 @@ def batched_sweep(steps: int) -> dict:
         c.step(c.scenes[0].dt)
     for c in ctxs:
         c.sync()
+    # (a) scene by scene
+    t0 = time.perf_counter()
+    for c in ctxs:
+        dt = c.scenes[0].dt
+        for _ in range(steps):
+            c.step(dt)
+        c.sync()
+    el_seq = time.perf_counter() - t0
+    # (b) all scenes' contexts interleaved (concurrent streams)
     t0 = time.perf_counter()
     for _ in range(steps):
         for c in ctxs:
             c.step(c.scenes[0].dt)
     for c in ctxs:
         c.sync()
-    el_s = time.perf_counter() - t0
+    el_con = time.perf_counter() - t0
     for c in ctxs:
         c.close()
-    single = n * steps / el_s
-    return {"value": batched, "scene_by_scene": single, "speedup": batched / single, "scenes": len(sc),
-            "particles": n, "steps": steps, "launch_shape": shape,
+    single = n * steps / el_seq
+    concurrent = n * steps / el_con
+    return {"value": batched, "scene_by_scene": single, "speedup": batched / single,
+            "scene_by_scene_concurrent_streams": concurrent, "speedup_vs_concurrent_streams": batched / concurrent,
+            "scenes": len(sc), "particles": n, "steps": steps, "launch_shape": shape,
             "grids": sorted({s.grid.dimx for s in sc}), "unit": "particle-steps/s"}
 
 
-def copy_calibration(mib: int) -> None:
-    """One device-to-device hipMemcpy of `mib` MiB (HIP's copy kernel): the reference point for the
-    FETCH_SIZE / WRITE_SIZE counters of the traffic measurement."""
-    import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
-    nbytes = mib << 20
-    a, b = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
-    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
-    for _ in range(3):
-        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
-    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+def stress_slab(steps: int, error_steps: int) -> dict:
+    """The mixed-precision stress scene: a 2^24-particle slab on 256^3 dropped on a box, hit by a
+    sphere at 10 m/s, physical weight gradient (1/h) so that particles reach the plastic clamp.
+    The impactor starts in contact with the slab. Throughput of the plain and mixed forms at full
+    size; errors of both against the FP64 oracle on the same geometry with 2^18 particles on 64^3
+    (a size the oracle finishes in seconds)."""
+    from snow_amd import FLAG_MIXED, SnowContext, scenes
+    sc = scenes.slab_impact(grad_scale=0.0, impactor_y=0.8)
+    n = len(sc.particles)
+    out = {"workload": "slab_impact_256^3", "particles": n, "steps": steps, "unit": "particle-steps/s",
+           "note": "impactor starts in contact; timed after 40 steps (1 ms) of impact"}
+    for name, flags in (("plain", 0), ("mixed", FLAG_MIXED)):
+        with SnowContext(sc, flags) as ctx:
+            ctx.init()
+            for _ in range(40):
+                ctx.step(sc.dt)
+            el = timed_steps(ctx, steps, sc.dt)
+            out[name] = n * steps / el
+            if flags:
+                out["fp64_fraction_last_step"] = ctx.fp64_particles() / n
+    small = scenes.slab_impact(n_particles=1 << 18, dims=64, grad_scale=0.0, impactor_y=0.8)
+    small.dt = 1e-4
+    out["errors"] = error_reports(small, {"plain": 0, "mixed": FLAG_MIXED}, error_steps)
+    out["errors"]["grid"] = 64
+    return out
+
+
+def large_batch(steps: int) -> dict:
+    """512 small scenes (1k-64k particles, grids 16^3-64^3) plus one 2^25-particle 512^3 scene in one
+    step graph; throughput of the whole batch and of its two parts run alone."""
+    from snow_amd import SnowContext, scenes
+    sc = scenes.large_batch()
+    parts = {"all": sc, "small_scenes": sc[:-1], "large_scene": sc[-1:]}
+    out = {"scenes": len(sc), "unit": "particle-steps/s", "steps": steps}
+    for name, part in parts.items():
+        n = sum(len(s.particles) for s in part)
+        with SnowContext(part) as ctx:
+            ctx.init()
+            ctx.run_graph(3)
+            el = timed_steps(ctx, steps, graph=True)
+        out[name] = {"particles": n, "value": n * steps / el}
+        print(f"[bench] large batch: {name} done", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -187,6 +242,10 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-second-form", action="store_true", help="skip the mixed (or plain) second form")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 2^24-particle slab stress scene")
+    ap.add_argument("--stress-steps", type=int, default=10)
+    ap.add_argument("--large", action="store_true", help="also run the 512+1-scene batch (2^25 + ~8M particles)")
+    ap.add_argument("--large-steps", type=int, default=10)
     ap.add_argument("--only-sweep", action="store_true", help="run only the batched 64-scene sweep")
     ap.add_argument("--error-steps", type=int, default=10)
     ap.add_argument("--sweep-steps", type=int, default=100)
@@ -287,13 +346,17 @@ def main():
                                                        "fp64_particles_last_step": fp64}
         print("[bench] second form timed", file=sys.stderr, flush=True)
     if not args.no_errors:
-        out["errors"] = {"steps": args.error_steps, "reference": "FP64 CPU oracle",
-                         "plain": error_report(scene, 0, args.error_steps),
-                         "mixed": error_report(scene, FLAG_MIXED, args.error_steps)}
+        out["errors"] = error_reports(scene, {"plain": 0, "mixed": FLAG_MIXED}, args.error_steps)
         print("[bench] error report done", file=sys.stderr, flush=True)
     if not args.no_sweep:
         out["forms"]["batched"] = batched_sweep(args.sweep_steps)
         print("[bench] batched sweep done", file=sys.stderr, flush=True)
+    if not args.no_stress:
+        out["forms"]["stress_slab"] = stress_slab(args.stress_steps, args.error_steps)
+        print("[bench] stress slab done", file=sys.stderr, flush=True)
+    if args.large:
+        out["forms"]["large_batch"] = large_batch(args.large_steps)
+        print("[bench] large batch done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_budget)

@@ -118,8 +118,9 @@ def two_snowballs(n_particles=1 << 20, seed=1002, grad_scale=1.0, dims=128) -> P
                          snow_material(grad_scale=grad_scale), dt=5e-5)
 
 
-def slab_impact(n_particles=1 << 24, seed=1003, grad_scale=1.0, dims=256) -> PhysicalScene:
-    """Slab dropped onto a box collider, hit by a sphere impactor at 10 m/s."""
+def slab_impact(n_particles=1 << 24, seed=1003, grad_scale=1.0, dims=256, impactor_y=0.85) -> PhysicalScene:
+    """Slab [0.2,0.8]x[0.5,0.7]x[0.2,0.8] dropped onto a box collider, hit by a sphere impactor of
+    radius 0.1 (centre height impactor_y; 0.8 starts it in contact with the slab) at 10 m/s."""
     h = 1.0 / dims
     lo, hi = (0.2, 0.5, 0.2), (0.8, 0.7, 0.8)
     x = rejection_sample(seed, 0, n_particles, lo, hi, lambda p: np.ones(len(p), bool))
@@ -127,7 +128,7 @@ def slab_impact(n_particles=1 << 24, seed=1003, grad_scale=1.0, dims=256) -> Phy
     m = rho0 * (0.6 * 0.2 * 0.6) / len(x)
     cols = [ground(),
             Collider((0.5, 0.175, 0.5), BOX, normal=(0.2, 0.125, 0.2), friction=0.2),
-            Collider((0.5, 0.85, 0.5), SPHERE, velocity=(0.0, -10.0, 0.0), radius=0.1, friction=0.2)]
+            Collider((0.5, impactor_y, 0.5), SPHERE, velocity=(0.0, -10.0, 0.0), radius=0.1, friction=0.2)]
     ps = ParticleSystem(x=x.astype(np.float32), mass=np.full(len(x), m, np.float32))
     return PhysicalScene(ps, Grid(dims, dims, dims, h, 0.0, 0.0, 0.0), CollisionObjects(cols),
                          snow_material(grad_scale=grad_scale), dt=2.5e-5)
@@ -179,3 +180,15 @@ def tiny_block(n_side=6, dims=16, seed=7, velocity=(0.5, -1.0, 0.25), grad_scale
     ps = ParticleSystem(x=x.astype(np.float32), v=v.astype(np.float32), mass=np.full(len(x), m, np.float32))
     return PhysicalScene(ps, Grid(dims, dims, dims, h, 0.0, 0.0, 0.0), CollisionObjects([ground()]),
                          material or snow_material(grad_scale=grad_scale), dt=1e-4)
+
+
+def large_batch(num_small=512, seed=5000, nmin=1024, nmax=65536, dims_choices=(16, 24, 32, 48, 64),
+                large_particles=1 << 25, large_dims=512, grad_scale=1.0) -> list:
+    """Many tiny-to-medium scenes (log-uniform 1k-64k particles, grids 16^3-64^3) plus one large
+    slab-impact scene (2^25 particles on 512^3, dt 1.25e-5), all in one batch."""
+    small = parameter_sweep(num_scenes=num_small, seed=seed, nmin=nmin, nmax=nmax, dims_choices=dims_choices,
+                            grad_scale=grad_scale)
+    big = slab_impact(n_particles=large_particles, seed=seed + 7, grad_scale=grad_scale, dims=large_dims)
+    big.dt = 1.25e-5
+    return small + [big]
+
