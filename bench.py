@@ -260,7 +260,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding of the "
-              f"snow step is design-only, see DESIGN.md)", file=sys.stderr)
+              f"snow step is a design only)", file=sys.stderr)
         sys.exit(2)
 
     from snow_amd import FLAG_MIXED, SnowContext, scenes

@@ -192,6 +192,20 @@ const char* snow_stage_name(int32_t stage);
 /* Accumulated milliseconds and call counts per stage since the last reset. */
 int snow_stage_times(snow_ctx* ctx, double* total_ms, int64_t* calls);
 int snow_reset_stage_times(snow_ctx* ctx);
+/* Per-stage statistics of the timed calls since the last reset, in ns: max, min, mean, sample
+ * standard deviation, count (stats::max/min/avg/stdev, statistics.hpp:8-31). Arrays of
+ * snow_num_stages() entries; any pointer may be NULL. */
+int snow_stage_stats(snow_ctx* ctx, double* max_ns, double* min_ns, double* avg_ns, double* stdev_ns,
+                     int64_t* calls);
+/* Write the statistics in the reference's benchstats.csv layout (Benchmarker::write,
+ * benchmarker.cpp:97-120): "Name:;Max:;Min:;Avg:;Stdev:;#Calls:" rows, stages with calls only. */
+int snow_write_benchstats(snow_ctx* ctx, const char* path);
+/* Zero-copy view of one scene's particles for rendering: *device_ptr points at a device array of
+ * *count float4 (x, y, z, m), *stride_bytes apart - the vec4 (x, m) the reference's point renderer
+ * reads from posB (particlesystem.cpp:4-13, pPoints.vert:2-12). Particles are in the order of the
+ * last sort, not creation order. Valid until the next step or state write on the context. */
+int snow_positions_view(snow_ctx* ctx, int32_t scene, const float** device_ptr, int64_t* stride_bytes,
+                        int64_t* count);
 /* Number of grid nodes of a scene with mass > 0 after the last P2G (for byte counts). */
 int snow_active_nodes(snow_ctx* ctx, int32_t scene, int64_t* out);
 /* Particles (all scenes) that took the FP64 SVD path in the last step (SNOW_FLAG_MIXED). */

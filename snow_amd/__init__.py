@@ -344,6 +344,26 @@ class SnowContext:
     def enable_stage_timers(self, on: bool = True):
         self._chk(self.lib.snow_enable_stage_timers(self.h, 1 if on else 0))
 
+    def stage_stats(self) -> dict:
+        """Per-stage {name: (max_ns, min_ns, avg_ns, stdev_ns, calls)} of the timed calls."""
+        ns = self.lib.snow_num_stages()
+        a = [np.zeros(ns, np.float64) for _ in range(4)]
+        calls = np.zeros(ns, np.int64)
+        self._chk(self.lib.snow_stage_stats(self.h, *(x.ctypes.data_as(C.POINTER(C.c_double)) for x in a),
+                                            calls.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {self.lib.snow_stage_name(i).decode(): (a[0][i], a[1][i], a[2][i], a[3][i], int(calls[i]))
+                for i in range(ns)}
+
+    def write_benchstats(self, path: str = "benchstats.csv"):
+        """Per-stage times in the reference's benchstats.csv layout."""
+        self._chk(self.lib.snow_write_benchstats(self.h, path.encode()))
+
+    def positions_view(self, scene: int = 0) -> tuple:
+        """(device address, stride in bytes, count) of the scene's float4 (x, y, z, m) particle data."""
+        p, stride, n = C.c_void_p(), C.c_int64(), C.c_int64()
+        self._chk(self.lib.snow_positions_view(self.h, scene, C.byref(p), C.byref(stride), C.byref(n)))
+        return p.value, stride.value, n.value
+
     def stage_times(self) -> dict:
         ns = self.lib.snow_num_stages()
         ms, calls = np.zeros(ns, np.float64), np.zeros(ns, np.int64)

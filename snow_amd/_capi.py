@@ -96,6 +96,9 @@ SIGNATURES = [
     ("snow_stage_name", C.c_char_p, [C.c_int32]),
     ("snow_stage_times", C.c_int, [C.c_void_p, f64p, i64p]),
     ("snow_reset_stage_times", C.c_int, [C.c_void_p]),
+    ("snow_stage_stats", C.c_int, [C.c_void_p, f64p, f64p, f64p, f64p, i64p]),
+    ("snow_write_benchstats", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("snow_positions_view", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), i64p, i64p]),
     ("snow_active_nodes", C.c_int, [C.c_void_p, C.c_int32, i64p]),
     ("snow_fp64_particles", C.c_int, [C.c_void_p, i64p]),
 ]

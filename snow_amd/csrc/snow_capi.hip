@@ -117,6 +117,7 @@ struct snow_ctx {
   std::vector<Pending> pending;
   double stage_ms[ST_COUNT] = {0};
   int64_t stage_calls[ST_COUNT] = {0};
+  double stage_min_ms[ST_COUNT] = {0}, stage_max_ms[ST_COUNT] = {0}, stage_sq_ms[ST_COUNT] = {0};
 
   hipGraph_t graph[2] = {nullptr, nullptr};      // batched step starting from buffer parity 0 / 1
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -197,7 +198,11 @@ int resolve_timers(snow_ctx* c) {
   for (auto& p : c->pending) {
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, p.a, p.b));
-    c->stage_ms[p.st] += ms;
+    const double d = ms;
+    if (c->stage_calls[p.st] == 0 || d < c->stage_min_ms[p.st]) c->stage_min_ms[p.st] = d;
+    if (c->stage_calls[p.st] == 0 || d > c->stage_max_ms[p.st]) c->stage_max_ms[p.st] = d;
+    c->stage_ms[p.st] += d;
+    c->stage_sq_ms[p.st] += d * d;
     c->stage_calls[p.st] += 1;
     c->ev_free.push_back(p.a);
     c->ev_free.push_back(p.b);
@@ -1153,7 +1158,56 @@ int snow_reset_stage_times(snow_ctx* c) {
   if (!c) return SNOW_E_ARG;
   int rc = resolve_timers(c);
   if (rc) return rc;
-  for (int i = 0; i < ST_COUNT; ++i) { c->stage_ms[i] = 0; c->stage_calls[i] = 0; }
+  for (int i = 0; i < ST_COUNT; ++i) {
+    c->stage_ms[i] = c->stage_sq_ms[i] = c->stage_min_ms[i] = c->stage_max_ms[i] = 0;
+    c->stage_calls[i] = 0;
+  }
+  return 0;
+}
+int snow_stage_stats(snow_ctx* c, double* max_ns, double* min_ns, double* avg_ns, double* stdev_ns,
+                     int64_t* calls) {
+  if (!c) return SNOW_E_ARG;
+  int rc = resolve_timers(c);
+  if (rc) return rc;
+  for (int i = 0; i < ST_COUNT; ++i) {
+    const int64_t n = c->stage_calls[i];
+    const double avg = n ? c->stage_ms[i] / double(n) : 0.0;
+    // sample standard deviation, as stats::stdev (statistics.hpp:9-15)
+    const double var = n > 1 ? std::max(0.0, (c->stage_sq_ms[i] - double(n) * avg * avg) / double(n - 1)) : 0.0;
+    if (max_ns) max_ns[i] = c->stage_max_ms[i] * 1e6;
+    if (min_ns) min_ns[i] = c->stage_min_ms[i] * 1e6;
+    if (avg_ns) avg_ns[i] = avg * 1e6;
+    if (stdev_ns) stdev_ns[i] = std::sqrt(var) * 1e6;
+    if (calls) calls[i] = n;
+  }
+  return 0;
+}
+int snow_write_benchstats(snow_ctx* c, const char* path) {
+  if (!c || !path) return SNOW_E_ARG;
+  double mx[ST_COUNT], mn[ST_COUNT], av[ST_COUNT], sd[ST_COUNT];
+  int64_t n[ST_COUNT];
+  int rc = snow_stage_stats(c, mx, mn, av, sd, n);
+  if (rc) return rc;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(c, SNOW_E_ARG, std::string("cannot open ") + path);
+  // Benchmarker::write layout (benchmarker.cpp:97-120), times in ns
+  std::fprintf(f, "BENCHMARKER STATS: (all times in ns)\n\n");
+  std::fprintf(f, "Name:;Max:;Min:;Avg:;Stdev:;#Calls:\n");
+  for (int i = 0; i < ST_COUNT; ++i) {
+    if (!n[i]) continue;
+    std::fprintf(f, "%s;%llu;%llu;%g;%g;%lld;\n", kStageNames[i], static_cast<unsigned long long>(std::llround(mx[i])),
+                 static_cast<unsigned long long>(std::llround(mn[i])), av[i], sd[i], static_cast<long long>(n[i]));
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? 0 : fail(c, SNOW_E_ARG, std::string("write failed: ") + path);
+}
+int snow_positions_view(snow_ctx* c, int32_t scene, const float** device_ptr, int64_t* stride_bytes,
+                        int64_t* count) {
+  if (!c || !device_ptr || scene < 0 || scene >= c->nscenes) return SNOW_E_ARG;
+  const SceneDev& S = c->scenes[scene];
+  *device_ptr = reinterpret_cast<const float*>(c->d_rec[c->cur] + 4 * size_t(S.p_off));
+  if (stride_bytes) *stride_bytes = int64_t(4 * sizeof(float4));
+  if (count) *count = S.p_cnt;
   return 0;
 }
 int snow_active_nodes(snow_ctx* c, int32_t s, int64_t* out) {

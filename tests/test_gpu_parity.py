@@ -376,3 +376,46 @@ def test_cpp_driver_matches_python_binding(tmp_path, c1_scene):
         ref = ctx.read_particles()
     for k in ("x", "v", "fe", "fp"):
         assert np.array_equal(got[k], ref[k]), k
+
+
+def test_benchstats_csv_layout(tmp_path, c1_scene):
+    """Stage timers give per-stage statistics, written in the reference's benchstats.csv layout."""
+    with SnowContext(c1_scene) as ctx:
+        ctx.init()
+        ctx.enable_stage_timers(True)
+        ctx.reset_stage_times()
+        for _ in range(5):
+            ctx.step(c1_scene.dt)
+        stats = ctx.stage_stats()
+        path = str(tmp_path / "benchstats.csv")
+        ctx.write_benchstats(path)
+    lines = open(path).read().splitlines()
+    assert lines[0] == "BENCHMARKER STATS: (all times in ns)" and lines[1] == ""
+    assert lines[2] == "Name:;Max:;Min:;Avg:;Stdev:;#Calls:"
+    rows = {r.split(";")[0]: r.split(";") for r in lines[3:]}
+    assert "#Sim:Transfer to Grid" in rows and "#Sim:Transfer to Particles" in rows
+    for name, r in rows.items():
+        mx, mn, avg, sd, calls = float(r[1]), float(r[2]), float(r[3]), float(r[4]), int(r[5])
+        assert calls == 5 and mx >= avg >= mn > 0 and sd >= 0, name
+        assert stats[name][4] == 5
+
+
+def test_positions_view_is_the_particle_state(c1_scene):
+    """The zero-copy view holds (x, y, z, m) of every particle of the scene (in sorted order)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    with SnowContext(c1_scene) as ctx:
+        ctx.init()
+        for _ in range(3):
+            ctx.step(c1_scene.dt)
+        ctx.sync()
+        ptr, stride, n = ctx.positions_view(0)
+        assert n == len(c1_scene.particles) and stride == 64
+        raw = np.empty((n, 16), np.float32)
+        assert hip.hipMemcpy(raw.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(n * stride), 2) == 0
+        p = ctx.read_particles()
+    got = raw[:, :4]
+    want = np.concatenate([p["x"], c1_scene.particles.mass[:, None]], 1).astype(np.float32)
+    order_g = np.lexsort(got.T[::-1])
+    order_w = np.lexsort(want.T[::-1])
+    assert np.array_equal(got[order_g], want[order_w])
