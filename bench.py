@@ -230,6 +230,24 @@ def large_batch(steps: int) -> dict:
     return out
 
 
+def copy_calibration(mib: int) -> None:
+    """One device-to-device hipMemcpy of `mib` MiB (HIP's copy kernel): the reference point for the
+    FETCH_SIZE / WRITE_SIZE counters of the traffic measurement."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
