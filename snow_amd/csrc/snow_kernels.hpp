@@ -808,6 +808,82 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
   return true;
 }
 
+// Wave-cooperative form of the interior gather. The 64 particles of a wave are consecutive in
+// sorted order, so their stencils cover a small box of nodes: the wave copies that box (at most
+// G2P_BOX nodes) from the node records into its LDS slot once, and every lane then gathers its
+// 64 nodes from LDS. This moves the 64 x 24 bytes per particle off the vector L1 / address path.
+// Waves that are not full, mix scenes, have a particle near the grid boundary or need a larger
+// box use g2p_gather. The sums are the same as g2p_gather's, in the same order.
+constexpr int G2P_BOX = 384;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, float x, float y, float z,
+                                                const float2* __restrict__ g_node, float2* box, float vpn[3],
+                                                float dvp[9]) {
+  const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
+  const int bx = int(floorf(ux)), by = int(floorf(uy)), bz = int(floorf(uz));
+  const int D0 = S.D[0], D1 = S.D[1], D2 = S.D[2];
+  const bool interior = in_stencil_range(S, ux, uy, uz) && bx >= 1 && by >= 1 && bz >= 1 && bx + 2 < D0 &&
+                        by + 2 < D1 && bz + 2 < D2;
+  const uint64_t act = __ballot(true);
+  const int sid0 = __builtin_amdgcn_readfirstlane(sid);
+  const bool full = act == ~0ull;
+  if (!full || __ballot(interior && sid == sid0) != act) {
+    g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
+    return;
+  }
+  // node box [xmin, xmin+ex) x [ymin, ymin+ey) x [zmin, zmin+ez)
+  const int xmin = __builtin_amdgcn_readfirstlane(wave_min_i(bx)) - 1;
+  const int ymin = __builtin_amdgcn_readfirstlane(wave_min_i(by)) - 1;
+  const int zmin = __builtin_amdgcn_readfirstlane(wave_min_i(bz)) - 1;
+  const int ex = __builtin_amdgcn_readfirstlane(wave_max_i(bx)) + 3 - xmin;
+  const int ey = __builtin_amdgcn_readfirstlane(wave_max_i(by)) + 3 - ymin;
+  const int ez = __builtin_amdgcn_readfirstlane(wave_max_i(bz)) + 3 - zmin;
+  const int vol = ex * ey * ez;
+  if (vol > G2P_BOX) {
+    g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  for (int idx = lane; idx < vol; idx += 64) {
+    const int ix = idx % ex, iy = (idx / ex) % ey, iz = idx / (ex * ey);
+    const int64_t node = S.node_off + (xmin + ix) + int64_t(D0) * ((ymin + iy) + int64_t(D1) * (zmin + iz));
+    const float2 a = g_node[3 * node], b = g_node[3 * node + 1], c = g_node[3 * node + 2];
+    box[3 * idx] = a;
+    box[3 * idx + 1] = b;
+    box[3 * idx + 2] = c;
+  }
+  // the wave's own LDS writes complete before its reads (in-order LDS per wave)
+  __builtin_amdgcn_wave_barrier();
+  float wx[4], wy[4], wz[4], qx[4], qy[4], qz[4];
+  axis_weights(ux, wx, qx);
+  axis_weights(uy, wy, qy);
+  axis_weights(uz, wz, qz);
+  vpn[0] = vpn[1] = vpn[2] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dvp[k] = 0.f;
+  const float2* p0 = box + 3 * ((bx - 1 - xmin) + ex * ((by - 1 - ymin) + ey * (bz - 1 - zmin)));
+  const int sy = 3 * ex, sz = 3 * ex * ey;
+#pragma unroll 1
+  for (int oz = 0; oz < 4; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < 4; ++oy)
+      g2p_row(p0 + oz * sz + oy * sy, wx, qx, wy[oy] * wz[oz], qy[oy] * wz[oz], wy[oy] * qz[oz], vpn, dvp, 0, 3);
+  const float gs = S.gscale;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dvp[k] *= gs;  // weight gradient scale (1 in the reference)
+}
+
 // particleAdvance.glsl:225-283 in precision R: F_E, F_P update with the singular-value clamp.
 template <class R>
 __device__ __forceinline__ void deform_update(const float dvp[9], float dt, const float FEf[9], const float FPf[9],
@@ -882,9 +958,11 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
     uint32_t* __restrict__ pid_out, uint32_t* __restrict__ keys_next, uint32_t* __restrict__ vals_next,
     const float2* __restrict__ g_node, const Collider* __restrict__ cols,
     float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count) {
+  __shared__ float2 s_box[4][3 * G2P_BOX];  // one node box per wave (blocks of up to 256 threads)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const SceneDev& S = scenes[pscene[i]];
+  const int sid = pscene[i];
+  const SceneDev& S = scenes[sid];
   const float dt = dt_override > 0.0f ? dt_override : S.dt;
   const uint32_t p = vals[i];
   const float4 r0 = rec_in[4 * size_t(p)];
@@ -898,7 +976,10 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
   const uint32_t id = pid_in[p];
 
   float vpn[3], dvp[9];
-  g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, COMP);
+  if (COMP)
+    g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, true);
+  else
+    g2p_gather_wave(S, sid, r0.x, r0.y, r0.z, g_node, s_box[threadIdx.x >> 6], vpn, dvp);
 
   float FEo[9], FPo[9];
   bool wide = false;
