@@ -174,69 +174,24 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
   }
 }
 
-// Ordered starts of the runs of equal cell tile among the sorted positions [lo, hi), written to
-// runs[0..n); returns n. Block-wide (every thread of the NT-thread block calls it). Position lo
-// always starts a run (whether or not its tile began before lo).
-template <int NT>
-__device__ uint32_t ordered_runs(const uint32_t* __restrict__ keys, uint32_t lo, uint32_t hi,
-                                 uint32_t* __restrict__ runs, uint32_t* __restrict__ s_wsum) {
-  constexpr int PER_MAX = P2G_SEG_MAX / NT;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint32_t len = hi - lo;
-  const uint32_t per = (len + NT - 1) / NT;
-  const uint32_t b = lo + uint32_t(t) * per, e = min(hi, b + per);
-  uint32_t flags = 0u, cnt = 0u;
-  uint32_t prevt = (b > lo && b < hi) ? keys[b - 1] / TILE_CELLS : 0xffffffffu;
-#pragma unroll
-  for (int k = 0; k < PER_MAX; ++k) {
-    const uint32_t i = b + uint32_t(k);
-    if (i < e) {
-      const uint32_t tl = keys[i] / TILE_CELLS;
-      if (i == lo || tl != prevt) { flags |= 1u << k; ++cnt; }
-      prevt = tl;
-    }
-  }
-  // block exclusive scan of the counts
-  uint32_t incl = cnt;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += y;
-  }
-  if (lane == 63) s_wsum[wave] = incl;
-  __syncthreads();
-  uint32_t base = 0u, total = 0u;
-#pragma unroll
-  for (int w = 0; w < NT / 64; ++w) {
-    const uint32_t v = s_wsum[w];
-    base += w < wave ? v : 0u;
-    total += v;
-  }
-  uint32_t o = base + incl - cnt;
-#pragma unroll
-  for (int k = 0; k < PER_MAX; ++k)
-    if (flags & (1u << k)) runs[o++] = b + uint32_t(k);
-  __syncthreads();
-  return total;
-}
-
 // ---------------------------------------------------------------------------------------
 // P2G (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84) over chunks of the sorted particles.
 // A scene's particles are contiguous in sorted order; its chunk c (global index chunk_off + c)
 // is the sorted range [p_off + c*seg, p_off + (c+1)*seg), so every workgroup gets about the same
 // number of particles however unevenly they fill the cell tiles, and a scene's chunks are the
-// same alone or in a batch. The chunk falls apart into
-// runs of particles of one 4^3 cell tile (contiguous, cell by cell). For each run the 8 waves
-// take equal contiguous shares and stage P2G_CHUNK particles at a time into their LDS slot: one
-// lane per particle loads the record and evaluates the particle's 12 weights and 12 gradients
-// once. Then lane l of the wave is the stencil offset o = (l&3, (l>>2)&3, l>>4) (getIJK order,
-// neighborIndex.include.glsl:6-12): for each staged particle the 64 lanes compute the 64 node
-// contributions from broadcast LDS reads and sum them in registers. When the cell changes, the
-// register sums are added to the run's LDS accumulator as 64-bit fixed point: integer addition
-// is associative, so the run sums are exact whatever the order of the commits, and the result
-// is bit-reproducible. The fixed-point scale is a power of two per run and quantity,
-// 2^61 / (count * max |per-particle term|), from a pre-pass over the run's particles, so no sum
-// can overflow and the resolution is ~2^-40 of the largest term.
+// same alone or in a batch.
+// The workgroup first reads the chunk's keys and slots once, all loads in flight together: it
+// keeps the slots and local cells in LDS, marks where the cell tile changes (the chunk falls
+// apart into runs of one tile, contiguous and cell by cell) and takes the fixed-point bound of
+// the whole chunk from the particle records. Then, run by run, the 8 waves take equal contiguous
+// shares and stage P2G_CHUNK particles at a time into their LDS slot: one lane per particle
+// evaluates the particle's 12 weights and 12 gradients once, from records loaded one batch ahead.
+// The node sums of a cell are products of small matrices over the cell's particles, done on the
+// matrix cores (below). When the cell changes they are added to the run's LDS accumulator as
+// 64-bit fixed point: integer addition is associative, so the run sums are exact whatever the
+// order of the commits, and the result is bit-reproducible. The fixed-point scale is a power of
+// two per chunk and quantity, 2^FIX_BITS / (count * max |per-particle term|), so no sum can
+// reach 2^FIX_BITS and the resolution is ~2^-39 of the largest term for a 2048-particle chunk.
 // A run that starts its tile writes the tile partial tpart[tile]; a run that continues a tile
 // begun in an earlier chunk is the chunk's first run and writes hpart[chunk]. The grid merge
 // adds tpart[tile] and the hpart of the tile's later chunks in chunk order.
@@ -256,18 +211,30 @@ __device__ __forceinline__ P2GBound particle_bound(const float4& r0, const float
   return b;
 }
 
-// exponent e with count*maxval*2^e <= 2^61 (clamped to the float range)
+constexpr int FIX_BITS = 50;  // |fixed-point sums| < 2^FIX_BITS
+
+// exponent e with count*maxval*2^e <= 2^FIX_BITS (clamped to the float range)
 __device__ __forceinline__ int fixed_exponent(float maxval, uint32_t count) {
   if (!(maxval > 0.0f)) return 0;
   const float bound = maxval * float(count);
   int ex = 0;
   frexpf(bound, &ex);  // bound < 2^ex
-  const int e = 61 - ex;
+  const int e = FIX_BITS - ex;
   return e > 126 ? 126 : (e < -126 ? -126 : e);
 }
 
+// Round-to-nearest-even of x to an integer, for |x| < 2^51: x converts to double exactly, and
+// adding 1.5*2^52 puts the rounded value into the low mantissa bits of a fixed binade.
+__device__ __forceinline__ long long fixed_round(float x) {
+  const double magic = 6755399441055744.0;  // 1.5 * 2^52
+  return __double_as_longlong(double(x) + magic) - __double_as_longlong(magic);
+}
+
 constexpr int P2G_CHUNK = 32;        // particles staged per wave at a time
-constexpr int P2G_STAGE = 40;        // floats per staged particle
+constexpr int P2G_STAGE = 44;        // floats per staged particle
+constexpr uint8_t P2G_DEAD = 0xff;   // local-cell mark of out-of-grid particles
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
@@ -277,156 +244,269 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     uint32_t list_cap, const float4* __restrict__ rec, double* __restrict__ tpart,
     double* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene, int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
+  constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
   __shared__ unsigned long long s_acc[NQ * HALO_NODES];
-  // per-wave staging of up to P2G_CHUNK particles: for each axis the interleaved (w_o, q_o),
-  // o = 0..3 (24 floats), then m, m*v (3), A row-major (9), 3 pad
+  // per-wave staging of P2G_CHUNK particles: for each axis the interleaved (w_o, q_o), o = 0..3
+  // (24 floats), then (m, m*v) (4), then the rows of A as (A_c0, A_c1, A_c2, cell) for c = 0..2
+  // and a zero row (0, 0, 0, cell) (16); cell = local cell of the particle, as bits
   __shared__ __attribute__((aligned(16))) float s_stage[P2G_WAVES][P2G_CHUNK * P2G_STAGE];
-  __shared__ uint32_t s_run[P2G_SEG_MAX];
-  __shared__ uint32_t s_bnd[2][3];
-  __shared__ uint32_t s_wsum[P2G_WAVES];
+  __shared__ uint32_t s_val[P2G_SEG_MAX];        // slots of the chunk's particles
+  __shared__ uint8_t s_cell[P2G_SEG_MAX];        // their local cells (P2G_DEAD: out of grid)
+  __shared__ uint16_t s_run[P2G_SEG_MAX + 1];    // run starts (chunk positions), then the end
+  __shared__ uint32_t s_flag[P2G_SEG_MAX / 32];  // run-start bits
+  __shared__ float s_wmax[P2G_WAVES][3];
+  __shared__ uint32_t s_nrun;
   __shared__ uint32_t s_edge[2];
 
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int lane = t & 63;
-  const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
+  // Matrix-core roles of the lane (v_mfma_f32_16x16x4f32: lane l supplies A[l%16][l/16] and
+  // B[l/16][l%16] and receives D[4(l/16) + r][l%16], r = 0..3). Rows are (quantity, o_x) =
+  // 4*quantity + o_x, columns (o_y, o_z) = o_y + 4*o_z, and K runs over 4 particles.
+  const int kp = lane >> 4;         // particle of the group whose operands the lane supplies
+  const int rq = (lane >> 2) & 3;   // A operand row block (quantity); B operand column o_z
+  const int rx = lane & 3;          // A operand row o_x; B operand column o_y
   float* stage = s_stage[wave];
   const uint32_t chunk = blockIdx.x;
   const uint32_t step = ctl->step;
-  uint32_t lo, hi;
-  {
-    const SceneDev& S = scenes[chunk_scene[chunk]];
-    lo = uint32_t(S.p_off) + (chunk - S.chunk_off) * S.seg;
-    const uint32_t end = uint32_t(S.p_off + S.p_cnt);
-    hi = min(end, lo + S.seg);
-    // tiles of the particles just before and after the chunk (same scene only)
-    if (t == 0) {
-      s_edge[0] = lo > uint32_t(S.p_off) ? keys[lo - 1] / TILE_CELLS : 0xffffffffu;
-      s_edge[1] = hi < end ? keys[hi] / TILE_CELLS : 0xffffffffu;
-    }
+  const SceneDev& S = scenes[chunk_scene[chunk]];
+  const uint32_t lo = uint32_t(S.p_off) + (chunk - S.chunk_off) * S.seg;
+  const uint32_t end = uint32_t(S.p_off + S.p_cnt);
+  const uint32_t hi = min(end, lo + S.seg);
+  const uint32_t len = hi - lo;
+  if (t == 0) {  // tiles of the particles just before and after the chunk (same scene only)
+    s_edge[0] = lo > uint32_t(S.p_off) ? keys[lo - 1] / TILE_CELLS : 0xffffffffu;
+    s_edge[1] = hi < end ? keys[hi] / TILE_CELLS : 0xffffffffu;
   }
-  if (t < 6) (&s_bnd[0][0])[t] = 0u;
-  const uint32_t nrun = ordered_runs<NT>(keys, lo, hi, s_run, s_wsum);
 
-  for (uint32_t r = 0; r < nrun; ++r) {
-    const uint32_t r_lo = s_run[r];
-    const uint32_t r_hi = r + 1 < nrun ? s_run[r + 1] : hi;
-    const uint32_t tile = keys[r_lo] / TILE_CELLS;
-    const SceneDev& S = scenes[tile_scene[tile]];
-    if (S.dead_tile == tile) {  // out-of-grid particles (uniform branch); keep the slot rotation
-      if (t < 3) s_bnd[(r & 1u) ^ 1u][t] = 0u;
-      __syncthreads();
-      continue;
-    }
-    const bool starts_tile = r > 0 || s_edge[0] != tile;
-    const bool ends_tile = r + 1 < nrun || s_edge[1] != tile;
-    const uint32_t count = r_hi - r_lo;
-    uint32_t* bnd = s_bnd[r & 1u];
-    // bound pre-pass over the run's particles
-    {
-      float bm = 0.f, bp = 0.f, bf = 0.f;
-      for (uint32_t i = r_lo + t; i < r_hi; i += NT) {
-        const uint32_t p = vals[i];
-        const P2GBound b = particle_bound(rec[4 * size_t(p)], rec[4 * size_t(p) + 1], rec[4 * size_t(p) + 2],
-                                          rec[4 * size_t(p) + 3], skip_heavy);
-        bm = fmaxf(bm, b.m); bp = fmaxf(bp, b.p); bf = fmaxf(bf, b.f);
-      }
+  // ---- prologue: slots, cells, run starts and the chunk's fixed-point bound
+  {
+    float bm = 0.f, bp = 0.f, bf = 0.f;
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        bm = fmaxf(bm, __shfl_xor(bm, off, 64));
-        bp = fmaxf(bp, __shfl_xor(bp, off, 64));
-        bf = fmaxf(bf, __shfl_xor(bf, off, 64));
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t o = uint32_t(t + NT * j);
+      bool flag = false;
+      if (o < len) {
+        const uint32_t key = keys[lo + o];
+        const uint32_t tl = key / TILE_CELLS;
+        flag = o == 0 || tl != keys[lo + o - 1] / TILE_CELLS;
+        const uint32_t p = vals[lo + o];
+        const bool dead = tl == S.dead_tile;
+        s_val[o] = p;
+        s_cell[o] = dead ? P2G_DEAD : uint8_t(key % TILE_CELLS);
+        if (!dead) {
+          const P2GBound b = particle_bound(rec[4 * size_t(p)], rec[4 * size_t(p) + 1], rec[4 * size_t(p) + 2],
+                                            rec[4 * size_t(p) + 3], skip_heavy);
+          bm = fmaxf(bm, b.m); bp = fmaxf(bp, b.p); bf = fmaxf(bf, b.f);
+        }
       }
-      // non-negative floats order like their bit patterns
+      const unsigned long long bits = __ballot(flag);
       if (lane == 0) {
-        atomicMax(&bnd[0], __float_as_uint(bm));
-        atomicMax(&bnd[1], __float_as_uint(bp));
-        atomicMax(&bnd[2], __float_as_uint(bf));
+        const uint32_t w = uint32_t(NT * j + 64 * wave) / 32u;
+        s_flag[w] = uint32_t(bits);
+        s_flag[w + 1] = uint32_t(bits >> 32);
       }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      bm = fmaxf(bm, __shfl_xor(bm, off, 64));
+      bp = fmaxf(bp, __shfl_xor(bp, off, 64));
+      bf = fmaxf(bf, __shfl_xor(bf, off, 64));
+    }
+    if (lane == 0) {
+      s_wmax[wave][0] = bm;
+      s_wmax[wave][1] = bp;
+      s_wmax[wave][2] = bf;
     }
     for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0ull;
-    // the other bound slot was read by everyone before the previous run's last barrier
-    if (t < 3) s_bnd[(r & 1u) ^ 1u][t] = 0u;
-    __syncthreads();
-    const int em = fixed_exponent(__uint_as_float(bnd[0]), count);
-    const int ep = fixed_exponent(__uint_as_float(bnd[1]), count);
-    const int ef = fixed_exponent(__uint_as_float(bnd[2]), count);
-    const float sm = ldexpf(1.0f, em), sp = ldexpf(1.0f, ep), sf = ldexpf(1.0f, ef);
-    const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
+  }
+  __syncthreads();
+  if (wave == 0) {  // ordered run starts from the bit mask, one 32-position word per lane
+    uint32_t word = s_flag[lane];
+    const uint32_t cnt = __popc(word);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
+    }
+    uint32_t o = incl - cnt;
+    while (word) {
+      const int b = __ffs(word) - 1;
+      word &= word - 1u;
+      s_run[o++] = uint16_t(32 * lane + b);
+    }
+    if (lane == 63) {
+      s_nrun = incl;
+      s_run[incl] = uint16_t(len);
+    }
+  }
+  float bm = 0.f, bp = 0.f, bf = 0.f;
+#pragma unroll
+  for (int w = 0; w < P2G_WAVES; ++w) {
+    bm = fmaxf(bm, s_wmax[w][0]);
+    bp = fmaxf(bp, s_wmax[w][1]);
+    bf = fmaxf(bf, s_wmax[w][2]);
+  }
+  const int em = fixed_exponent(bm, len), ep = fixed_exponent(bp, len), ef = fixed_exponent(bf, len);
+  const float sm = ldexpf(1.0f, em), sp = ldexpf(1.0f, ep), sf = ldexpf(1.0f, ef);
+  const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
+  __syncthreads();
+  const uint32_t nrun = s_nrun;
 
-    // this wave's contiguous share of the run
-    const uint32_t share = (count + P2G_WAVES - 1) / P2G_WAVES;
-    const uint32_t w_lo = min(r_hi, r_lo + uint32_t(wave) * share), w_hi = min(r_hi, w_lo + share);
-    int cur = -1;  // local cell of the register sums
-    float am = 0.f, apx = 0.f, apy = 0.f, apz = 0.f, afx = 0.f, afy = 0.f, afz = 0.f;
+  // ---- the wave's batches over all runs, with the records of the next batch loaded ahead
+  // cursor: run nr, batch [nc, min(nc + P2G_CHUNK, ne)) of the wave's share [., ne) of run nr
+  uint32_t nr = 0, nc = 0, ne = 0;
+  auto seek = [&](uint32_t r) {  // first run >= r, not out of grid, with a non-empty share
+    for (; r < nrun; ++r) {
+      const uint32_t r_lo = s_run[r], r_hi = s_run[r + 1];
+      if (s_cell[r_lo] == P2G_DEAD) continue;
+      const uint32_t sh = (r_hi - r_lo + P2G_WAVES - 1) / P2G_WAVES;
+      nc = min(r_hi, r_lo + uint32_t(wave) * sh);
+      ne = min(r_hi, nc + sh);
+      if (nc < ne) break;
+    }
+    nr = __builtin_amdgcn_readfirstlane(r);
+    nc = __builtin_amdgcn_readfirstlane(nc);
+    ne = __builtin_amdgcn_readfirstlane(ne);
+  };
+  float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0;
+  uint32_t qcell = 0u;
+  auto fetch = [&]() {
+    if (nr < nrun && uint32_t(lane) < min(uint32_t(P2G_CHUNK), ne - nc)) {
+      const uint32_t p = s_val[nc + lane];
+      qcell = s_cell[nc + lane];
+      q0 = rec[4 * size_t(p)];
+      q1 = rec[4 * size_t(p) + 1];
+      q2 = rec[4 * size_t(p) + 2];
+      q3 = rec[4 * size_t(p) + 3];
+    }
+  };
+  seek(0);
+  fetch();
+
+  for (uint32_t r = 0; r < nrun; ++r) {
+    const uint32_t r_lo = s_run[r], r_hi = s_run[r + 1];
+    if (s_cell[r_lo] == P2G_DEAD) continue;  // out-of-grid particles (uniform)
+    const uint32_t tile = keys[lo + r_lo] / TILE_CELLS;
+    const bool starts_tile = r > 0 || s_edge[0] != tile;
+    const bool ends_tile = r + 1 < nrun || s_edge[1] != tile;
+
+    // Per cell the 64 node sums are small matrix products over the cell's particles p:
+    //   mass/momentum  D1[(q, o_x), (o_y, o_z)] = sum_p (mq_p wx_p[o_x]) (wy_p[o_y] wz_p[o_z]),
+    //                  mq = (m, m v);
+    //   A grad(w)      D2[(c, o_x), (o_y, o_z)] = sum_p sum_t A_p[c][t] gx_t[o_x] gyz_t[o_y, o_z],
+    //                  (gx, gyz)_0 = (qx, wy wz), _1 = (wx, qy wz), _2 = (wx, wy qz),
+    // which is A_p (qx wy wz, wx qy wz, wx wy qz)^T, A grad(w) of p2gTransfer.glsl:109-141.
+    // Each group of 4 staged particles is one K=4 step of D1 and three (t = 0,1,2) of D2, on
+    // the matrix cores. A group whose particles span several cells is applied once per cell,
+    // with the B operands of the other cells' particles set to zero.
+    int cur = -1;  // local cell of the accumulators
+    f32x4 acc1 = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
+    const float s1 = kp == 0 ? sm : sp;
     auto commit = [&]() {
       const int cx = cur & 3, cy = (cur >> 2) & 3, cz = cur >> 4;
-      const int jn = (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
-      // the force sums hold +A grad(w); the node force is -A grad(w)
-      atomicAdd(&s_acc[0 * HALO_NODES + jn], (unsigned long long)llrintf(am * sm));
-      atomicAdd(&s_acc[1 * HALO_NODES + jn], (unsigned long long)llrintf(apx * sp));
-      atomicAdd(&s_acc[2 * HALO_NODES + jn], (unsigned long long)llrintf(apy * sp));
-      atomicAdd(&s_acc[3 * HALO_NODES + jn], (unsigned long long)llrintf(apz * sp));
-      atomicAdd(&s_acc[4 * HALO_NODES + jn], (unsigned long long)llrintf(-afx * sf));
-      atomicAdd(&s_acc[5 * HALO_NODES + jn], (unsigned long long)llrintf(-afy * sf));
-      atomicAdd(&s_acc[6 * HALO_NODES + jn], (unsigned long long)llrintf(-afz * sf));
-      am = apx = apy = apz = afx = afy = afz = 0.f;
-    };
-    for (uint32_t c0 = w_lo; c0 < w_hi; c0 += P2G_CHUNK) {
-      const uint32_t cnt = __builtin_amdgcn_readfirstlane(min(uint32_t(P2G_CHUNK), w_hi - c0));
-      // stage: lane k < cnt loads particle c0+k and evaluates its weights once
-      uint32_t cellv = 0u;
-      if (uint32_t(lane) < cnt) {
-        const uint32_t p = vals[c0 + lane];
-        cellv = keys[c0 + lane] % TILE_CELLS;
-        const float4 r0 = rec[4 * size_t(p)];
-        const float4 r1 = rec[4 * size_t(p) + 1];
-        const float4 r2 = rec[4 * size_t(p) + 2];
-        const float4 r3 = rec[4 * size_t(p) + 3];
-        // p2gTransfer.glsl:64-66 skips |m| > 100 (gMassCompute does not)
-        const float use = (skip_heavy && fabsf(r0.w) > 100.0f) ? 0.0f : 1.0f;
-        const float m = r0.w * use;
-        // every particle of a cell has base = floor(u) = that cell; f = u - base
-        const float ux = fmaf(r0.x, inv_h, uo0), uy = fmaf(r0.y, inv_h, uo1), uz = fmaf(r0.z, inv_h, uo2);
-        const float fx = ux - floorf(ux), fy = uy - floorf(uy), fz = uz - floorf(uz);
-        float4* d4 = reinterpret_cast<float4*>(stage + lane * P2G_STAGE);
-        d4[0] = make_float4(spline_w(0, fx), spline_q(0, fx), spline_w(1, fx), spline_q(1, fx));
-        d4[1] = make_float4(spline_w(2, fx), spline_q(2, fx), spline_w(3, fx), spline_q(3, fx));
-        d4[2] = make_float4(spline_w(0, fy), spline_q(0, fy), spline_w(1, fy), spline_q(1, fy));
-        d4[3] = make_float4(spline_w(2, fy), spline_q(2, fy), spline_w(3, fy), spline_q(3, fy));
-        d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
-        d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
-        d4[6] = make_float4(m, r1.w * m, r2.w * m, r3.w * m);
-        d4[7] = make_float4(r1.x * use, r1.y * use, r1.z * use, r2.x * use);
-        d4[8] = make_float4(r2.y * use, r2.z * use, r3.x * use, r3.y * use);
-        d4[9] = make_float4(r3.z * use, 0.f, 0.f, 0.f);
+      // D row block kp is quantity kp (D1) and force row kp (D2); r = o_x; column (rx, rq)
+      const int jn = cx + HALO * (cy + rx) + HALO * HALO * (cz + rq);
+      unsigned long long* a1 = s_acc + kp * HALO_NODES + jn;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) atomicAdd(a1 + i, (unsigned long long)fixed_round(acc1[i] * s1));
+      if (kp < 3) {
+        // the sums hold +A grad(w); the node force is -A grad(w)
+        unsigned long long* a2 = s_acc + (4 + kp) * HALO_NODES + jn;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) atomicAdd(a2 + i, (unsigned long long)fixed_round(-acc2[i] * sf));
       }
+      acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
+      acc2 = f32x4{0.f, 0.f, 0.f, 0.f};
+    };
+    while (nr == r) {  // the wave's batches of this run
+      const uint32_t cnt = min(uint32_t(P2G_CHUNK), ne - nc);
+      // stage: lane k < cnt evaluates the weights of its prefetched particle; the lanes
+      // cnt <= k < P2G_CHUNK write zero records, which contribute nothing to a group
+      const uint32_t cellv = uint32_t(lane) < cnt ? qcell : 0xffffffffu;
+      if (lane < P2G_CHUNK) {
+        float4* d4 = reinterpret_cast<float4*>(stage + lane * P2G_STAGE);
+        if (uint32_t(lane) < cnt) {
+          // p2gTransfer.glsl:64-66 skips |m| > 100 (gMassCompute does not)
+          const float use = (skip_heavy && fabsf(q0.w) > 100.0f) ? 0.0f : 1.0f;
+          const float m = q0.w * use;
+          // every particle of a cell has base = floor(u) = that cell; f = u - base
+          const float ux = fmaf(q0.x, inv_h, uo0), uy = fmaf(q0.y, inv_h, uo1), uz = fmaf(q0.z, inv_h, uo2);
+          const float fx = ux - floorf(ux), fy = uy - floorf(uy), fz = uz - floorf(uz);
+          const float cb = __uint_as_float(cellv);
+          d4[0] = make_float4(spline_w(0, fx), spline_q(0, fx), spline_w(1, fx), spline_q(1, fx));
+          d4[1] = make_float4(spline_w(2, fx), spline_q(2, fx), spline_w(3, fx), spline_q(3, fx));
+          d4[2] = make_float4(spline_w(0, fy), spline_q(0, fy), spline_w(1, fy), spline_q(1, fy));
+          d4[3] = make_float4(spline_w(2, fy), spline_q(2, fy), spline_w(3, fy), spline_q(3, fy));
+          d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
+          d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
+          d4[6] = make_float4(m, q1.w * m, q2.w * m, q3.w * m);
+          d4[7] = make_float4(q1.x * use, q1.y * use, q1.z * use, cb);
+          d4[8] = make_float4(q2.x * use, q2.y * use, q2.z * use, cb);
+          d4[9] = make_float4(q3.x * use, q3.y * use, q3.z * use, cb);
+          d4[10] = make_float4(0.f, 0.f, 0.f, cb);
+        } else {
+          const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+          const float4 zc = make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
+#pragma unroll
+          for (int i = 0; i < 7; ++i) d4[i] = z;
+#pragma unroll
+          for (int i = 7; i < 11; ++i) d4[i] = zc;
+        }
+      }
+      // advance the cursor and load the next batch's records while this one is summed
+      if (nc + P2G_CHUNK < ne) nc += P2G_CHUNK;
+      else seek(nr + 1);
+      fetch();
       // the wave's own LDS writes complete before its reads (in-order LDS per wave)
       __builtin_amdgcn_wave_barrier();
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const int cell = int(__builtin_amdgcn_readlane(cellv, int(k)));
-        if (cell != cur) {
+      const uint32_t ngroups = (cnt + 3u) >> 2;
+      for (uint32_t g = 0; g < ngroups; ++g) {
+        const uint32_t k0 = 4u * g, kl = min(k0 + 3u, cnt - 1u);
+        const int c_first = __builtin_amdgcn_readlane(int(cellv), int(k0));
+        const int c_last = __builtin_amdgcn_readlane(int(cellv), int(kl));
+        const float* d = stage + (k0 + uint32_t(kp)) * P2G_STAGE;
+        const float2 xq = reinterpret_cast<const float2*>(d)[rx];
+        const float2 yq = reinterpret_cast<const float2*>(d + 8)[rx];
+        const float2 zq = reinterpret_cast<const float2*>(d + 16)[rq];
+        const float mq = d[24 + rq];
+        const float4 ar = reinterpret_cast<const float4*>(d + 28)[rq];
+        const float a1 = mq * xq.x;
+        const float a20 = ar.x * xq.y, a21 = ar.y * xq.x, a22 = ar.z * xq.x;
+        const float b0 = yq.x * zq.x, b1 = yq.y * zq.x, b2 = yq.x * zq.y;
+        if (c_first != cur) {
           if (cur >= 0) commit();
-          cur = cell;
+          cur = c_first;
         }
-        const float* d = stage + k * P2G_STAGE;
-        const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
-        const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
-        const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
-        const float4 u0 = reinterpret_cast<const float4*>(d)[6];
-        const float4 u1 = reinterpret_cast<const float4*>(d)[7];
-        const float4 u2 = reinterpret_cast<const float4*>(d)[8];
-        const float a22 = d[36];
-        const float wyz = wqy.x * wqz.x;
-        const float w = wqx.x * wyz;
-        const float g0 = wqx.y * wyz, g1 = wqy.y * (wqx.x * wqz.x), g2 = wqz.y * (wqx.x * wqy.x);
-        am = fmaf(u0.x, w, am);
-        apx = fmaf(u0.y, w, apx);
-        apy = fmaf(u0.z, w, apy);
-        apz = fmaf(u0.w, w, apz);
-        afx = fmaf(u1.x, g0, fmaf(u1.y, g1, fmaf(u1.z, g2, afx)));
-        afy = fmaf(u1.w, g0, fmaf(u2.x, g1, fmaf(u2.y, g2, afy)));
-        afz = fmaf(u2.z, g0, fmaf(u2.w, g1, fmaf(a22, g2, afz)));
+        if (c_last == c_first) {
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc1, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a20, b0, acc2, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a21, b1, acc2, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a22, b2, acc2, 0, 0, 0);
+        } else {
+          // the group's cells are non-decreasing from c_first to c_last: one pass per cell
+          const int pc = int(__float_as_uint(ar.w));
+          for (;;) {
+            const bool in = pc == cur;
+            const float m0 = in ? b0 : 0.f, m1 = in ? b1 : 0.f, m2 = in ? b2 : 0.f;
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, m0, acc1, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a20, m0, acc2, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a21, m1, acc2, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a22, m2, acc2, 0, 0, 0);
+            if (cur == c_last) break;
+            commit();
+            // next cell: that of the first particle of the group beyond the current one
+            int nxt = c_last;
+#pragma unroll
+            for (int i = 2; i >= 1; --i) {
+              const int ci = __builtin_amdgcn_readlane(int(cellv), int(min(k0 + uint32_t(i), kl)));
+              nxt = ci > cur ? ci : nxt;
+            }
+            cur = nxt;
+          }
+        }
       }
       __builtin_amdgcn_wave_barrier();
     }
@@ -435,17 +515,20 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     double* out = starts_tile ? tpart + size_t(tile) * (NQ * HALO_NODES) : hpart + size_t(chunk) * (NQ * HALO_NODES);
     if (t == 0) {
       if (starts_tile) {
-        tile_lo[tile] = r_lo;
+        tile_lo[tile] = lo + r_lo;
         tile_stamp[tile] = step;
         const uint32_t sub = chunk % TILE_LISTS;
         tile_list[size_t(sub) * list_cap + atomicAdd(&list_ctr[sub * TILE_LIST_STRIDE], 1u)] = tile;
       }
-      if (ends_tile) tile_hi[tile] = r_hi;
+      if (ends_tile) tile_hi[tile] = lo + r_hi;
     }
+    // write the run sums out and clear the accumulator for the next run
     for (int i = t; i < NQ * HALO_NODES; i += NT) {
       const int q = i / HALO_NODES;
       const int e = q == 0 ? em : (q < 4 ? ep : ef);
-      out[i] = ldexp(double((long long)s_acc[i]), -e);  // relative rounding <= 2^-53
+      const unsigned long long v = s_acc[i];
+      s_acc[i] = 0ull;
+      out[i] = ldexp(double((long long)v), -e);  // relative rounding <= 2^-53
     }
     __syncthreads();
   }
