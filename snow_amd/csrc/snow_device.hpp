@@ -250,12 +250,12 @@ __device__ __forceinline__ void svd3(const R* A, R* W, R* S, R* V) {
 
 // Force matrix A with f_i = -A grad(w_ip) (p2gTransfer.glsl:109-141), times grad_scale:
 // A = gs * (m/rho0) * (2 mu(J_P) (F_E - R_E) F_E^T + lambda(J_P) (J_E - 1) J_E I),
-// R_E = W V^T rounded to 1e-9 (p2gTransfer.glsl:112-116). Zero when rho0 <= 0.
+// R_E = W V^T rounded to 1e-9 (p2gTransfer.glsl:112-116), from given SVD factors W, V of F_E.
+// Zero when rho0 <= 0.
 template <class R>
-__device__ __forceinline__ void force_matrix(const R* FE, const R* FP, R m, R rho0, R mu0, R lambda0,
-                                             R xi, R gs, R* Aout) {
-  R W[9], S[9], V[9], Rot[9];
-  svd3(FE, W, S, V);
+__device__ __forceinline__ void force_matrix_wv(const R* FE, const R* FP, const R* W, const R* V, R m, R rho0,
+                                                R mu0, R lambda0, R xi, R gs, R* Aout) {
+  R Rot[9];
   mul_bt(W, V, Rot);
 #pragma unroll
   for (int i = 0; i < 9; ++i) Rot[i] = round_to(Rot[i], R(1e9), R(1e-9));
@@ -270,6 +270,15 @@ __device__ __forceinline__ void force_matrix(const R* FE, const R* FP, R m, R rh
   const R k = rho0 > R(0) ? gs * (m / rho0) : R(0);
 #pragma unroll
   for (int i = 0; i < 9; ++i) Aout[i] = k * (R(2) * mu * P[i] + ((i == 0 || i == 4 || i == 8) ? lterm : R(0)));
+}
+
+// The same with the SVD of F_E computed here (SVD.include.glsl via p2gTransfer.glsl:109-111).
+template <class R>
+__device__ __forceinline__ void force_matrix(const R* FE, const R* FP, R m, R rho0, R mu0, R lambda0,
+                                             R xi, R gs, R* Aout) {
+  R W[9], S[9], V[9];
+  svd3(FE, W, S, V);
+  force_matrix_wv(FE, FP, W, V, m, rho0, mu0, lambda0, xi, gs, Aout);
 }
 
 // ---- colliders (gForceCompute.glsl:40-93, particleAdvance.glsl:203-216) ----------------

@@ -970,7 +970,8 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
 // particleAdvance.glsl:225-283 in precision R: F_E, F_P update with the singular-value clamp.
 template <class R>
 __device__ __forceinline__ void deform_update(const float dvp[9], float dt, const float FEf[9], const float FPf[9],
-                                              float crit_c, float crit_s, float FEo[9], float FPo[9]) {
+                                              float crit_c, float crit_s, float FEo[9], float FPo[9], R Wo[9],
+                                              R Vo[9]) {
   R L[9], FE[9], FP[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) { L[i] = R(dt) * R(dvp[i]); FE[i] = R(FEf[i]); FP[i] = R(FPf[i]); }
@@ -1000,6 +1001,8 @@ __device__ __forceinline__ void deform_update(const float dvp[9], float dt, cons
   for (int i = 0; i < 9; ++i) {
     FEo[i] = float(round_to(Fe2[i], R(1e9), R(1e-9)));
     FPo[i] = float(round_to(Fp2[i], R(1e9), R(1e-9)));
+    Wo[i] = W[i];
+    Vo[i] = V[i];
   }
 }
 
@@ -1030,7 +1033,7 @@ __device__ __forceinline__ bool needs_fp64(const float dvp[9], float dt, const f
 // Fused G2P + particle advance (g2pTransfer.glsl:53-143, particleAdvance.glsl:75-217).
 // Thread i handles sorted position i: it reads the particle from its previous slot
 // vals[i] (buffers *_in), writes the new state to slot i (buffers *_out), computes the next
-// step's force matrix (second SVD, of the stored F_E) and the next step's cell key. The
+// step's force matrix (from the SVD of the F_E update) and the next step's cell key. The
 // permutation of the sort is thereby applied for free by the writes of this kernel.
 template <bool MIXED, bool COMP>
 __global__ void __launch_bounds__(256) k_g2p_advance(
@@ -1064,14 +1067,26 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
   else
     g2p_gather_wave(S, sid, r0.x, r0.y, r0.z, g_node, s_box[threadIdx.x >> 6], vpn, dvp);
 
-  float FEo[9], FPo[9];
+  // F_E, F_P update; the next step's force matrix (p2gTransfer.glsl:109-141, once per particle)
+  // takes R_E = W V^T from the factors of this update's SVD: the stored F_E is W S' V^T with
+  // S' the clamped triangular factor, so its own SVD gives the same W V^T.
+  float FEo[9], FPo[9], A[9];
   bool wide = false;
   if (MIXED) wide = needs_fp64(dvp, dt, FE, S.crit_comp, S.crit_stretch);
   if (wide) {
-    deform_update<double>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo);
+    double W[9], V[9], FEd[9], FPd[9], Ad[9];
+    deform_update<double>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) { FEd[e] = FEo[e]; FPd[e] = FPo[e]; }
+    force_matrix_wv<double>(FEd, FPd, W, V, double(r0.w), double(rho0), double(S.mu0), double(S.lambda0),
+                            double(S.hardening), double(S.gscale), Ad);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) A[e] = float(Ad[e]);
     if (fp64_count) atomicAdd(fp64_count, 1ull);
   } else {
-    deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo);
+    float W[9], V[9];
+    deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
+    force_matrix_wv<float>(FEo, FPo, W, V, r0.w, rho0, S.mu0, S.lambda0, S.hardening, S.gscale, A);
   }
 
   // velocity, collisions at the pre-advection position, advection (particleAdvance.glsl:318-359)
@@ -1079,19 +1094,6 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
   collide(r0.x, r0.y, r0.z, vx, vy, vz, cols + S.col_off, S.ncol);
   const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
 
-  // next step's force matrix (p2gTransfer.glsl:109-141, once per particle)
-  float A[9];
-  if (MIXED && wide) {
-    double FEd[9], FPd[9], Ad[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) { FEd[e] = FEo[e]; FPd[e] = FPo[e]; }
-    force_matrix<double>(FEd, FPd, double(r0.w), double(rho0), double(S.mu0), double(S.lambda0),
-                         double(S.hardening), double(S.gscale), Ad);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) A[e] = float(Ad[e]);
-  } else {
-    force_matrix<float>(FEo, FPo, r0.w, rho0, S.mu0, S.lambda0, S.hardening, S.gscale, A);
-  }
   // rec rows: A row r = (A[r], A[3+r], A[6+r]) from the column-major A
   rec_out[4 * size_t(i)] = make_float4(x, y, z, r0.w);
   rec_out[4 * size_t(i) + 1] = make_float4(A[0], A[3], A[6], vx);
@@ -1146,13 +1148,13 @@ __global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict
   for (int k = 0; k < 3; ++k) vpn[k] = aux[size_t(k) * n + i];
 #pragma unroll
   for (int k = 0; k < 9; ++k) dvp[k] = aux[size_t(3 + k) * n + i];
-  float FEo[9], FPo[9];
-  deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo);
+  float FEo[9], FPo[9], W[9], V[9];
+  deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
   float vx = r1.w * S.alpha + vpn[0], vy = r2.w * S.alpha + vpn[1], vz = r3.w * S.alpha + vpn[2];
   collide(r0.x, r0.y, r0.z, vx, vy, vz, cols + S.col_off, S.ncol);
   const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
   float A[9];
-  force_matrix<float>(FEo, FPo, r0.w, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
+  force_matrix_wv<float>(FEo, FPo, W, V, r0.w, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
   rec[4 * size_t(i)] = make_float4(x, y, z, r0.w);
   rec[4 * size_t(i) + 1] = make_float4(A[0], A[3], A[6], vx);
   rec[4 * size_t(i) + 2] = make_float4(A[1], A[4], A[7], vy);
