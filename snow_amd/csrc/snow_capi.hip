@@ -90,6 +90,7 @@ struct snow_ctx {
   uint32_t* d_list_ctr = nullptr;  // TILE_LISTS append counters
   uint32_t* d_tile_list = nullptr; // TILE_LISTS sublists of total_ctiles entries
   StepCtl* d_ctl = nullptr;
+  uint32_t* d_pbound = nullptr;    // [nscenes][2 parities][PB_SLOTS][PB_STRIDE] fixed-point bounds of the next P2G
   uint32_t* d_tile_stamp = nullptr;
   int num_cu = 256;
   float4 *d_g_mmv = nullptr, *d_g_f = nullptr;
@@ -105,6 +106,7 @@ struct snow_ctx {
   int cur = 0;
   bool keys_valid = false;
   bool force_valid = false;
+  bool bound_valid = false;  // the next P2G's bound slots hold the bound of the current records
   bool initialized = false;
   int grid_state = 0;   // 0 nothing, 1 tiled (p2g/step), 2 dense (host write / dense update)
   bool keep_valid = false;
@@ -262,6 +264,28 @@ int ensure_stage(snow_ctx* c, size_t bytes) {
   return 0;
 }
 
+// Clears every bound slot (a writer outside the fused step then publishes the bound of the current
+// records alone, whatever earlier steps left there).
+int clear_bounds(snow_ctx* c) {
+  const uint32_t w = uint32_t(c->nscenes) * 2u * PB_SLOTS * PB_STRIDE;
+  k_fill_u32<<<grid_for(w, 256), 256, 0, c->stream>>>(c->d_pbound, 0u, int64_t(w));
+  LAUNCHCHK(c);
+  return 0;
+}
+
+// Bound of the current records into the slots of step parity (step + ahead) & 1.
+int publish_bounds(snow_ctx* c, uint32_t ahead, int skip_heavy) {
+  int rc;
+  if ((rc = clear_bounds(c))) return rc;
+  if (c->N) {
+    k_particle_bounds<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->d_ctl,
+                                                                  c->d_pbound, ahead, skip_heavy, c->N);
+    LAUNCHCHK(c);
+  }
+  c->bound_valid = true;
+  return 0;
+}
+
 int run_keys(snow_ctx* c) {
   if (c->keys_valid) return 0;
   if (c->N) {
@@ -275,13 +299,16 @@ int run_keys(snow_ctx* c) {
 
 int run_force_prepass(snow_ctx* c) {
   if (c->force_valid) return 0;
+  int rc;
+  if ((rc = clear_bounds(c))) return rc;
   if (c->N) {
     const int k = c->cur;
     k_force_prepass<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[k], c->d_fe[k],
-                                                                 c->d_fp[k], c->d_rho[k], c->N);
+                                                                 c->d_fp[k], c->d_rho[k], c->N, c->d_ctl, c->d_pbound);
     LAUNCHCHK(c);
   }
   c->force_valid = true;
+  c->bound_valid = true;
   return 0;
 }
 
@@ -302,17 +329,22 @@ int run_sort(snow_ctx* c, float dt_rigid) {
 }
 
 int run_p2g(snow_ctx* c, int mass_only) {
+  if (!c->bound_valid) {  // records no writer published (after the step start: this step's parity)
+    const int rc = publish_bounds(c, 0u, mass_only ? 0 : 1);
+    if (rc) return rc;
+  }
   StageTimer tm(c, ST_P2G);
   if (c->total_chunks) {
     k_p2g_chunk<<<c->total_chunks, 64 * P2G_WAVES, 0, c->stream>>>(
         c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1], c->d_tile_lo, c->d_tile_hi, c->d_tile_stamp,
         c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_tpart, c->d_hpart,
-        c->d_chunk_scene, mass_only ? 0 : 1);
+        c->d_chunk_scene, c->d_pbound, mass_only ? 0 : 1);
     LAUNCHCHK(c);
     k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
                                                             c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart);
     LAUNCHCHK(c);
   }
+  c->bound_valid = false;  // consumed; the slots of the other parity are cleared for the writers
   return tm.done();
 }
 
@@ -345,7 +377,7 @@ int run_g2p_advance(snow_ctx* c, float dt) {
 #define G2P_ARGS                                                                                        \
   c->d_scenes, c->d_pscene, c->d_vals[1], c->d_rec[a], c->d_fe[a], c->d_fp[a], c->d_rho[a], c->d_pid[a], \
       c->d_rec[b], c->d_fe[b], c->d_fp[b], c->d_rho[b], c->d_pid[b], c->d_keys[0], c->d_vals[0],         \
-      c->d_g_node, c->d_cols, dt, c->N, f64
+      c->d_g_node, c->d_cols, dt, c->N, f64, c->d_ctl, c->d_pbound
     const unsigned bs = unsigned(c->g2p_block);
     const unsigned g = grid_for(c->N, bs);
     if (mixed && comp) k_g2p_advance<true, true><<<g, bs, 0, c->stream>>>(G2P_ARGS);
@@ -358,6 +390,7 @@ int run_g2p_advance(snow_ctx* c, float dt) {
   c->cur = b;
   c->keys_valid = true;
   c->force_valid = true;
+  c->bound_valid = true;
   c->aux_valid = false;
   return tm.done();
 }
@@ -386,7 +419,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_sort_tmp,
-                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_ctl, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
+                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -573,6 +606,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_tile_list, size_t(TILE_LISTS) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_zero_part, 0, sizeof(double) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_ctl, 1));
+  CREATECHK(dalloc(&c->d_pbound, size_t(c->nscenes) * 2 * PB_SLOTS * PB_STRIDE));
   CREATECHK(dalloc(&c->d_tile_stamp, c->total_ctiles));
   CREATECHK(dalloc(&c->d_g_mmv, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_g_f, size_t(c->total_nodes)));
@@ -640,6 +674,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(hipMemset(c->d_tile_lo, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_tile_hi, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_ctl, 0, sizeof(StepCtl)));
+  CREATECHK(hipMemset(c->d_pbound, 0, sizeof(uint32_t) * c->nscenes * 2 * PB_SLOTS * PB_STRIDE));
   CREATECHK(hipMemset(c->d_tile_stamp, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_g_mmv, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_f, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
@@ -673,10 +708,13 @@ int snow_stage_grid_mass(snow_ctx* c) {
   if (!c) return SNOW_E_ARG;
   int rc;
   // A = 0 during the mass pass: zero the force part of the records (gMassCompute only adds m)
+  if ((rc = clear_bounds(c))) return rc;
   if (c->N) {
-    k_zero_force<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_rec[c->cur], c->N);
+    k_zero_force<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->N,
+                                                             c->d_ctl, c->d_pbound);
     LAUNCHCHK(c);
   }
+  c->bound_valid = true;
   c->force_valid = false;
   if ((rc = run_sort(c, -1.0f))) return rc;
   if ((rc = run_p2g(c, 1))) return rc;
@@ -729,9 +767,11 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
   for (const auto& S : c->scenes)
     if (!(S.dt > 0.0f)) return fail(c, SNOW_E_ARG, "every scene needs dt > 0 for snow_run_graph");
   int rc;
-  // prerequisites of the captured step, run eagerly: force matrices and cell keys of the state
+  // prerequisites of the captured step, run eagerly: force matrices, cell keys and the P2G bound
+  // of the state (the next step start makes parity step + 1 the current one)
   if ((rc = run_force_prepass(c))) return rc;
   if ((rc = run_keys(c))) return rc;
+  if (!c->bound_valid && (rc = publish_bounds(c, 1u, 1))) return rc;
   if (!c->gexec[0]) {
     // One graph per buffer parity: the step reads buffers `cur` and writes `1 - cur`. Every launch
     // in it has fixed arguments (per-step counters live in the device control block), and the
@@ -774,6 +814,7 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
   if (steps > 0) {
     c->keys_valid = true;
     c->force_valid = true;
+    c->bound_valid = true;
     c->aux_valid = false;
     c->grid_state = 1;
     c->keep_valid = (c->flags & SNOW_FLAG_KEEP_GRID) != 0;
@@ -816,6 +857,7 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
     c->cur = k;
     c->keys_valid = true;
     c->force_valid = true;
+    c->bound_valid = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
   };
@@ -938,16 +980,20 @@ int snow_stage_particle_advance(snow_ctx* c, double dt) {
   const float fdt = float(dt);
   if (!(fdt > 0.0f)) return fail(c, SNOW_E_ARG, "dt must be > 0");
   if (!c->aux_valid) return fail(c, SNOW_E_STATE, "particle advance needs a G2P result (snow_stage_g2p)");
+  int rc;
+  if ((rc = clear_bounds(c))) return rc;
   StageTimer tm(c, ST_ADVANCE);
   const int k = c->cur;
   if (c->N) {
     k_advance_only<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[k], c->d_fe[k],
                                                                 c->d_fp[k], c->d_rho[k], c->d_aux, c->d_cols, fdt,
-                                                                c->d_keys[0], c->d_vals[0], c->N);
+                                                                c->d_keys[0], c->d_vals[0], c->N, c->d_ctl,
+                                                                c->d_pbound);
     LAUNCHCHK(c);
   }
   c->keys_valid = true;
   c->force_valid = true;
+  c->bound_valid = true;
   c->aux_valid = false;
   return tm.done();
 }
@@ -1000,6 +1046,7 @@ int snow_write_particles(snow_ctx* c, int32_t s, const float* x, const float* v,
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (mask & 1) c->keys_valid = false;
   if (mask & (4 | 8 | 16)) c->force_valid = false;
+  c->bound_valid = false;
   if (rho0) c->initialized = true;
   return 0;
 }

@@ -181,17 +181,18 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
 // number of particles however unevenly they fill the cell tiles, and a scene's chunks are the
 // same alone or in a batch.
 // The workgroup first reads the chunk's keys and slots once, all loads in flight together: it
-// keeps the slots and local cells in LDS, marks where the cell tile changes (the chunk falls
-// apart into runs of one tile, contiguous and cell by cell) and takes the fixed-point bound of
-// the whole chunk from the particle records. Then, run by run, the 8 waves take equal contiguous
+// keeps the slots and local cells in LDS and marks where the cell tile changes (the chunk falls
+// apart into runs of one tile, contiguous and cell by cell); the fixed-point bound is the scene's,
+// published by the writers of the records (below). Then, run by run, the 8 waves take equal contiguous
 // shares and stage P2G_CHUNK particles at a time into their LDS slot: one lane per particle
 // evaluates the particle's 12 weights and 12 gradients once, from records loaded one batch ahead.
 // The node sums of a cell are products of small matrices over the cell's particles, done on the
 // matrix cores (below). When the cell changes they are added to the run's LDS accumulator as
 // 64-bit fixed point: integer addition is associative, so the run sums are exact whatever the
 // order of the commits, and the result is bit-reproducible. The fixed-point scale is a power of
-// two per chunk and quantity, 2^FIX_BITS / (count * max |per-particle term|), so no sum can
-// reach 2^FIX_BITS and the resolution is ~2^-39 of the largest term for a 2048-particle chunk.
+// two per chunk and quantity, 2^FIX_BITS / (count * max |per-particle term| of the scene), so no
+// sum can reach 2^FIX_BITS and the resolution is ~2^-39 of the scene's largest term for a
+// 2048-particle chunk.
 // A run that starts its tile writes the tile partial tpart[tile]; a run that continues a tile
 // begun in an earlier chunk is the chunk's first run and writes hpart[chunk]. The grid merge
 // adds tpart[tile] and the hpart of the tile's later chunks in chunk order.
@@ -230,6 +231,130 @@ __device__ __forceinline__ long long fixed_round(float x) {
   return __double_as_longlong(double(x) + magic) - __double_as_longlong(magic);
 }
 
+// Fixed-point bounds of the next P2G. Every kernel that writes particle records for a P2G (the
+// fused G2P, the stage-API advance, the init force pass, the grid-mass pass) also publishes the
+// per-scene maxima of the P2G terms of what it wrote: an atomic max of the non-negative float
+// bits into PB_SLOTS slots per scene and step parity, so the P2G needs no pass over the records
+// of its own. The P2G of step s reads parity s & 1 and clears parity (s+1) & 1, which the
+// writers of step s then fill. Out-of-grid particles are left out (the P2G skips them). Each
+// slot has a 128-byte line of its own (words m, p, f), so the atomics of different waves do not
+// queue on one line.
+constexpr int PB_SLOTS = 64;
+constexpr int PB_STRIDE = 32;  // words per slot
+
+__device__ __forceinline__ uint32_t* bound_slots(uint32_t* pb, int sid, uint32_t parity) {
+  return pb + (size_t(sid) * 2 + parity) * (PB_SLOTS * PB_STRIDE);
+}
+
+// wave_full (uniform): all 64 lanes of the calling wave are live; otherwise each lane publishes
+// its own bound. NaN and infinite terms count as the largest float.
+__device__ __forceinline__ void publish_bound(uint32_t* __restrict__ pb, uint32_t parity, int sid, P2GBound b,
+                                              bool wave_full) {
+  float m = b.m <= __FLT_MAX__ ? b.m : __FLT_MAX__;
+  float p = b.p <= __FLT_MAX__ ? b.p : __FLT_MAX__;
+  float f = b.f <= __FLT_MAX__ ? b.f : __FLT_MAX__;
+  const uint32_t slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % PB_SLOTS;
+  if (wave_full) {
+    const int sid0 = __builtin_amdgcn_readfirstlane(sid);
+    if (__ballot(sid != sid0) == 0ull) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        m = fmaxf(m, __shfl_xor(m, off, 64));
+        p = fmaxf(p, __shfl_xor(p, off, 64));
+        f = fmaxf(f, __shfl_xor(f, off, 64));
+      }
+      if ((threadIdx.x & 63) == 0) {
+        uint32_t* sl = bound_slots(pb, sid0, parity) + slot * PB_STRIDE;
+        if (m > 0.f) atomicMax(sl, __float_as_uint(m));
+        if (p > 0.f) atomicMax(sl + 1, __float_as_uint(p));
+        if (f > 0.f) atomicMax(sl + 2, __float_as_uint(f));
+      }
+      return;
+    }
+  }
+  uint32_t* sl = bound_slots(pb, sid, parity) + slot * PB_STRIDE;
+  if (m > 0.f) atomicMax(sl, __float_as_uint(m));
+  if (p > 0.f) atomicMax(sl + 1, __float_as_uint(p));
+  if (f > 0.f) atomicMax(sl + 2, __float_as_uint(f));
+}
+
+// The same, reduced over the workgroup first (4x fewer global atomics for 256-thread blocks):
+// s_pb (4 words, zeroed before a barrier at kernel start, before any thread returns) collects
+// the block maxima and counts the waves done; the last live wave publishes them. Threads
+// i >= n have returned (whole waves, the live lanes of a wave being its low lanes). Blocks
+// spanning several scenes (one_scene false, from block_one_scene at kernel start, so the load
+// is not at the end of every wave) publish wave by wave.
+__device__ __forceinline__ bool block_one_scene(const uint16_t* __restrict__ pscene, uint32_t n) {
+  const uint32_t b0 = blockIdx.x * blockDim.x;
+  const uint32_t last = min(n, b0 + blockDim.x) - 1u;
+  return b0 >= n || pscene[b0] == pscene[last];
+}
+
+__device__ __forceinline__ void publish_bound_block(uint32_t* __restrict__ pb, uint32_t parity, int sid, P2GBound b,
+                                                    uint32_t* s_pb, bool one_scene, uint32_t i, uint32_t n) {
+  const uint32_t b0 = blockIdx.x * blockDim.x;
+  const uint32_t live = min(n, b0 + blockDim.x) - b0;
+  if (!one_scene) {
+    publish_bound(pb, parity, sid, b, (i | 63u) < n);
+    return;
+  }
+  float m = b.m <= __FLT_MAX__ ? b.m : __FLT_MAX__;
+  float p = b.p <= __FLT_MAX__ ? b.p : __FLT_MAX__;
+  float f = b.f <= __FLT_MAX__ ? b.f : __FLT_MAX__;
+  const int lane = threadIdx.x & 63;
+  if ((i | 63u) < n) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      m = fmaxf(m, __shfl_xor(m, off, 64));
+      p = fmaxf(p, __shfl_xor(p, off, 64));
+      f = fmaxf(f, __shfl_xor(f, off, 64));
+    }
+    if (lane == 0) {
+      atomicMax(&s_pb[0], __float_as_uint(m));
+      atomicMax(&s_pb[1], __float_as_uint(p));
+      atomicMax(&s_pb[2], __float_as_uint(f));
+    }
+  } else {  // the partial last wave of the launch
+    atomicMax(&s_pb[0], __float_as_uint(m));
+    atomicMax(&s_pb[1], __float_as_uint(p));
+    atomicMax(&s_pb[2], __float_as_uint(f));
+  }
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's maxima are in LDS
+  if (lane == 0 && atomicAdd(&s_pb[3], 1u) == (live + 63u) / 64u - 1u) {
+    const uint32_t slot = blockIdx.x % PB_SLOTS;
+    uint32_t* sl = bound_slots(pb, sid, parity) + slot * PB_STRIDE;
+    const uint32_t bm = s_pb[0], bp = s_pb[1], bf = s_pb[2];
+    if (bm) atomicMax(sl, bm);
+    if (bp) atomicMax(sl + 1, bp);
+    if (bf) atomicMax(sl + 2, bf);
+  }
+}
+
+// Bound of a particle record (x, m | A row 0, vx | A row 1, vy | A row 2, vz) whose cell key is
+// key; zero when the particle is out of the grid.
+__device__ __forceinline__ P2GBound record_bound(const SceneDev& S, uint32_t key, const float4& r0, const float4& r1,
+                                                 const float4& r2, const float4& r3, int skip_heavy) {
+  if (key / TILE_CELLS == S.dead_tile) return P2GBound{0.f, 0.f, 0.f};
+  return particle_bound(r0, r1, r2, r3, skip_heavy);
+}
+
+// Fallback publisher for records no writer has published (host uploads, restored snapshots):
+// parity (step + ahead) & 1.
+__global__ void __launch_bounds__(256) k_particle_bounds(const SceneDev* __restrict__ scenes,
+                                                         const uint16_t* __restrict__ pscene,
+                                                         const float4* __restrict__ rec, const StepCtl* __restrict__ ctl,
+                                                         uint32_t* __restrict__ pb, uint32_t ahead, int skip_heavy,
+                                                         uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sid = pscene[i];
+  const SceneDev& S = scenes[sid];
+  const float4 r0 = rec[4 * size_t(i)], r1 = rec[4 * size_t(i) + 1], r2 = rec[4 * size_t(i) + 2],
+               r3 = rec[4 * size_t(i) + 3];
+  const P2GBound b = record_bound(S, cell_key(S, r0.x, r0.y, r0.z), r0, r1, r2, r3, skip_heavy);
+  publish_bound(pb, (ctl->step + ahead) & 1u, sid, b, (i | 63u) < n);
+}
+
 constexpr int P2G_CHUNK = 32;        // particles staged per wave at a time
 constexpr int P2G_STAGE = 44;        // floats per staged particle
 constexpr uint8_t P2G_DEAD = 0xff;   // local-cell mark of out-of-grid particles
@@ -242,7 +367,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     uint32_t* __restrict__ tile_lo, uint32_t* __restrict__ tile_hi, uint32_t* __restrict__ tile_stamp,
     const StepCtl* __restrict__ ctl, uint32_t* __restrict__ list_ctr, uint32_t* __restrict__ tile_list,
     uint32_t list_cap, const float4* __restrict__ rec, double* __restrict__ tpart,
-    double* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene, int skip_heavy) {
+    double* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene, uint32_t* __restrict__ pbound,
+    int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
   __shared__ unsigned long long s_acc[NQ * HALO_NODES];
@@ -254,7 +380,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   __shared__ uint8_t s_cell[P2G_SEG_MAX];        // their local cells (P2G_DEAD: out of grid)
   __shared__ uint16_t s_run[P2G_SEG_MAX + 1];    // run starts (chunk positions), then the end
   __shared__ uint32_t s_flag[P2G_SEG_MAX / 32];  // run-start bits
-  __shared__ float s_wmax[P2G_WAVES][3];
+  __shared__ float s_bound[3];
   __shared__ uint32_t s_nrun;
   __shared__ uint32_t s_edge[2];
 
@@ -270,7 +396,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   float* stage = s_stage[wave];
   const uint32_t chunk = blockIdx.x;
   const uint32_t step = ctl->step;
-  const SceneDev& S = scenes[chunk_scene[chunk]];
+  const int sid = chunk_scene[chunk];
+  const SceneDev& S = scenes[sid];
   const uint32_t lo = uint32_t(S.p_off) + (chunk - S.chunk_off) * S.seg;
   const uint32_t end = uint32_t(S.p_off + S.p_cnt);
   const uint32_t hi = min(end, lo + S.seg);
@@ -280,9 +407,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     s_edge[1] = hi < end ? keys[hi] / TILE_CELLS : 0xffffffffu;
   }
 
-  // ---- prologue: slots, cells, run starts and the chunk's fixed-point bound
+  // ---- prologue: slots, cells, run starts; the scene's fixed-point bound
   {
-    float bm = 0.f, bp = 0.f, bf = 0.f;
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
       const uint32_t o = uint32_t(t + NT * j);
@@ -291,15 +417,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
         const uint32_t key = keys[lo + o];
         const uint32_t tl = key / TILE_CELLS;
         flag = o == 0 || tl != keys[lo + o - 1] / TILE_CELLS;
-        const uint32_t p = vals[lo + o];
-        const bool dead = tl == S.dead_tile;
-        s_val[o] = p;
-        s_cell[o] = dead ? P2G_DEAD : uint8_t(key % TILE_CELLS);
-        if (!dead) {
-          const P2GBound b = particle_bound(rec[4 * size_t(p)], rec[4 * size_t(p) + 1], rec[4 * size_t(p) + 2],
-                                            rec[4 * size_t(p) + 3], skip_heavy);
-          bm = fmaxf(bm, b.m); bp = fmaxf(bp, b.p); bf = fmaxf(bf, b.f);
-        }
+        s_val[o] = vals[lo + o];
+        s_cell[o] = tl == S.dead_tile ? P2G_DEAD : uint8_t(key % TILE_CELLS);
       }
       const unsigned long long bits = __ballot(flag);
       if (lane == 0) {
@@ -308,17 +427,25 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
         s_flag[w + 1] = uint32_t(bits >> 32);
       }
     }
+    const uint32_t par = step & 1u;
+    if (wave == P2G_WAVES - 1) {  // max over the scene's bound slots of this step
+      const uint32_t* sl = bound_slots(pbound, sid, par) + lane * PB_STRIDE;
+      float bm = __uint_as_float(sl[0]), bp = __uint_as_float(sl[1]), bf = __uint_as_float(sl[2]);
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      bm = fmaxf(bm, __shfl_xor(bm, off, 64));
-      bp = fmaxf(bp, __shfl_xor(bp, off, 64));
-      bf = fmaxf(bf, __shfl_xor(bf, off, 64));
+      for (int off = 32; off >= 1; off >>= 1) {
+        bm = fmaxf(bm, __shfl_xor(bm, off, 64));
+        bp = fmaxf(bp, __shfl_xor(bp, off, 64));
+        bf = fmaxf(bf, __shfl_xor(bf, off, 64));
+      }
+      if (lane == 0) {
+        s_bound[0] = bm;
+        s_bound[1] = bp;
+        s_bound[2] = bf;
+      }
     }
-    if (lane == 0) {
-      s_wmax[wave][0] = bm;
-      s_wmax[wave][1] = bp;
-      s_wmax[wave][2] = bf;
-    }
+    // the scene's first chunk clears the slots the writers of this step fill for the next one
+    if (chunk == S.chunk_off && t < PB_SLOTS * 4)
+      bound_slots(pbound, sid, par ^ 1u)[(t >> 2) * PB_STRIDE + (t & 3)] = 0u;
     for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0ull;
   }
   __syncthreads();
@@ -342,13 +469,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       s_run[incl] = uint16_t(len);
     }
   }
-  float bm = 0.f, bp = 0.f, bf = 0.f;
-#pragma unroll
-  for (int w = 0; w < P2G_WAVES; ++w) {
-    bm = fmaxf(bm, s_wmax[w][0]);
-    bp = fmaxf(bp, s_wmax[w][1]);
-    bf = fmaxf(bf, s_wmax[w][2]);
-  }
+  const float bm = s_bound[0], bp = s_bound[1], bf = s_bound[2];
   const int em = fixed_exponent(bm, len), ep = fixed_exponent(bp, len), ef = fixed_exponent(bf, len);
   const float sm = ldexpf(1.0f, em), sp = ldexpf(1.0f, ep), sf = ldexpf(1.0f, ef);
   const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
@@ -1043,8 +1164,14 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
     float* __restrict__ fe_out, float* __restrict__ fp_out, float* __restrict__ rho_out,
     uint32_t* __restrict__ pid_out, uint32_t* __restrict__ keys_next, uint32_t* __restrict__ vals_next,
     const float2* __restrict__ g_node, const Collider* __restrict__ cols,
-    float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count) {
+    float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count, const StepCtl* __restrict__ ctl,
+    uint32_t* __restrict__ pbound) {
   __shared__ float2 s_box[4][3 * G2P_BOX];  // one node box per wave (blocks of up to 256 threads)
+  __shared__ uint32_t s_pb[4];               // block maxima of the P2G bound, waves done
+  if (threadIdx.x < 4) s_pb[threadIdx.x] = 0u;
+  const bool one_scene = block_one_scene(pscene, n);
+  const uint32_t next_parity = (ctl->step + 1u) & 1u;
+  __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int sid = pscene[i];
@@ -1095,16 +1222,20 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
   const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
 
   // rec rows: A row r = (A[r], A[3+r], A[6+r]) from the column-major A
-  rec_out[4 * size_t(i)] = make_float4(x, y, z, r0.w);
-  rec_out[4 * size_t(i) + 1] = make_float4(A[0], A[3], A[6], vx);
-  rec_out[4 * size_t(i) + 2] = make_float4(A[1], A[4], A[7], vy);
-  rec_out[4 * size_t(i) + 3] = make_float4(A[2], A[5], A[8], vz);
+  const float4 o0 = make_float4(x, y, z, r0.w), o1 = make_float4(A[0], A[3], A[6], vx),
+               o2 = make_float4(A[1], A[4], A[7], vy), o3 = make_float4(A[2], A[5], A[8], vz);
+  rec_out[4 * size_t(i)] = o0;
+  rec_out[4 * size_t(i) + 1] = o1;
+  rec_out[4 * size_t(i) + 2] = o2;
+  rec_out[4 * size_t(i) + 3] = o3;
 #pragma unroll
   for (int e = 0; e < 9; ++e) { fe_out[size_t(e) * n + i] = FEo[e]; fp_out[size_t(e) * n + i] = FPo[e]; }
   rho_out[i] = rho0;
   pid_out[i] = id;
-  keys_next[i] = cell_key(S, x, y, z);
+  const uint32_t key = cell_key(S, x, y, z);
+  keys_next[i] = key;
   vals_next[i] = i;
+  publish_bound_block(pbound, next_parity, sid, record_bound(S, key, o0, o1, o2, o3, 1), s_pb, one_scene, i, n);
 }
 
 // Split stages for the per-stage API: G2P alone writes (vpn, dvp) per particle slot;
@@ -1132,10 +1263,12 @@ __global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict
                                                       const float* __restrict__ rho, const float* __restrict__ aux,
                                                       const Collider* __restrict__ cols, float dt_override,
                                                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                      uint32_t n) {
+                                                      uint32_t n, const StepCtl* __restrict__ ctl,
+                                                      uint32_t* __restrict__ pbound) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const SceneDev& S = scenes[pscene[i]];
+  const int sid = pscene[i];
+  const SceneDev& S = scenes[sid];
   const float dt = dt_override > 0.0f ? dt_override : S.dt;
   const float4 r0 = rec[4 * size_t(i)];
   const float4 r1 = rec[4 * size_t(i) + 1];
@@ -1155,14 +1288,18 @@ __global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict
   const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
   float A[9];
   force_matrix_wv<float>(FEo, FPo, W, V, r0.w, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
-  rec[4 * size_t(i)] = make_float4(x, y, z, r0.w);
-  rec[4 * size_t(i) + 1] = make_float4(A[0], A[3], A[6], vx);
-  rec[4 * size_t(i) + 2] = make_float4(A[1], A[4], A[7], vy);
-  rec[4 * size_t(i) + 3] = make_float4(A[2], A[5], A[8], vz);
+  const float4 o0 = make_float4(x, y, z, r0.w), o1 = make_float4(A[0], A[3], A[6], vx),
+               o2 = make_float4(A[1], A[4], A[7], vy), o3 = make_float4(A[2], A[5], A[8], vz);
+  rec[4 * size_t(i)] = o0;
+  rec[4 * size_t(i) + 1] = o1;
+  rec[4 * size_t(i) + 2] = o2;
+  rec[4 * size_t(i) + 3] = o3;
 #pragma unroll
   for (int e = 0; e < 9; ++e) { fe[size_t(e) * n + i] = FEo[e]; fp[size_t(e) * n + i] = FPo[e]; }
-  keys[i] = cell_key(S, x, y, z);
+  const uint32_t key = cell_key(S, x, y, z);
+  keys[i] = key;
   vals[i] = i;
+  publish_bound(pbound, (ctl->step + 1u) & 1u, sid, record_bound(S, key, o0, o1, o2, o3, 1), (i | 63u) < n);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1207,15 +1344,17 @@ __global__ void __launch_bounds__(256) k_particle_volume(const SceneDev* __restr
 __global__ void __launch_bounds__(256) k_force_prepass(const SceneDev* __restrict__ scenes,
                                                        const uint16_t* __restrict__ pscene, float4* __restrict__ rec,
                                                        const float* __restrict__ fe, const float* __restrict__ fp,
-                                                       const float* __restrict__ rho, uint32_t n) {
+                                                       const float* __restrict__ rho, uint32_t n,
+                                                       const StepCtl* __restrict__ ctl, uint32_t* __restrict__ pbound) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const SceneDev& S = scenes[pscene[i]];
+  const int sid = pscene[i];
+  const SceneDev& S = scenes[sid];
   float FE[9], FP[9], A[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) { FE[e] = fe[size_t(e) * n + i]; FP[e] = fp[size_t(e) * n + i]; }
-  const float m = rec[4 * size_t(i)].w;
-  force_matrix<float>(FE, FP, m, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
+  const float4 r0 = rec[4 * size_t(i)];
+  force_matrix<float>(FE, FP, r0.w, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
   float4 r1 = rec[4 * size_t(i) + 1], r2 = rec[4 * size_t(i) + 2], r3 = rec[4 * size_t(i) + 3];
   r1.x = A[0]; r1.y = A[3]; r1.z = A[6];
   r2.x = A[1]; r2.y = A[4]; r2.z = A[7];
@@ -1223,18 +1362,30 @@ __global__ void __launch_bounds__(256) k_force_prepass(const SceneDev* __restric
   rec[4 * size_t(i) + 1] = r1;
   rec[4 * size_t(i) + 2] = r2;
   rec[4 * size_t(i) + 3] = r3;
+  publish_bound(pbound, (ctl->step + 1u) & 1u, sid,
+                record_bound(S, cell_key(S, r0.x, r0.y, r0.z), r0, r1, r2, r3, 1), (i | 63u) < n);
 }
 
 // Zero the force matrices (init: the grid-mass pass needs m only).
-__global__ void __launch_bounds__(256) k_zero_force(float4* __restrict__ rec, uint32_t n) {
+__global__ void __launch_bounds__(256) k_zero_force(const SceneDev* __restrict__ scenes,
+                                                    const uint16_t* __restrict__ pscene, float4* __restrict__ rec,
+                                                    uint32_t n, const StepCtl* __restrict__ ctl,
+                                                    uint32_t* __restrict__ pbound) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int sid = pscene[i];
+  const SceneDev& S = scenes[sid];
+  const float4 r0 = rec[4 * size_t(i)];
+  float4 r[3];
 #pragma unroll
-  for (int r = 1; r < 4; ++r) {
-    float4 v = rec[4 * size_t(i) + r];
-    v.x = 0.f; v.y = 0.f; v.z = 0.f;
-    rec[4 * size_t(i) + r] = v;
+  for (int k = 0; k < 3; ++k) {
+    r[k] = rec[4 * size_t(i) + 1 + k];
+    r[k].x = 0.f; r[k].y = 0.f; r[k].z = 0.f;
+    rec[4 * size_t(i) + 1 + k] = r[k];
   }
+  // the grid-mass P2G keeps particles of any mass (gMassCompute.glsl has no |m| > 100 skip)
+  publish_bound(pbound, (ctl->step + 1u) & 1u, sid,
+                record_bound(S, cell_key(S, r0.x, r0.y, r0.z), r0, r[0], r[1], r[2], 0), (i | 63u) < n);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -93,6 +93,40 @@ def test_p2g_out_of_grid(oracle_mod):
     assert rel_err(g["mv"], o.gmv) < 2e-6
 
 
+def test_p2g_bound_follows_host_writes(oracle_mod):
+    """The P2G's fixed-point scale comes from bounds published by the kernels that wrote the records.
+    After fused steps, a host write that makes every momentum ~1e5 times larger must not reuse the
+    published bound (the sums would leave the fixed-point range): the P2G must match the oracle,
+    and graph replays from the written state must equal eager steps bit for bit."""
+    sc = deformed(scenes.tiny_block(n_side=10, dims=16))
+    dt = sc.dt
+    with SnowContext(sc) as a, SnowContext(sc) as b:
+        for ctx in (a, b):
+            ctx.init()
+            for _ in range(3):
+                ctx.step(dt)
+        st = a.read_particles()
+        v_big = (st["v"] * 1000.0 + np.array([200.0, -50.0, 30.0])).astype(np.float32)
+        for ctx in (a, b):
+            ctx.write_particles(v=v_big)
+        a.stage("p2g")
+        g = a.read_grid(fields=("m", "mv"))
+        for _ in range(2):
+            a.step(dt)
+        b.run_graph(2)
+        pa, pb = a.read_particles(), b.read_particles()
+    sc2 = scenes.tiny_block(n_side=10, dims=16)
+    sc2.particles.x, sc2.particles.v = st["x"].copy(), v_big.copy()
+    sc2.particles.fe, sc2.particles.fp = st["fe"].copy(), st["fp"].copy()
+    o = oracle_mod.Oracle(sc2)
+    o.rho0[:] = st["rho0"]
+    o.p2g()
+    assert rel_err(g["m"], o.gm) < 2e-6
+    assert rel_err(g["mv"], o.gmv) < 2e-6
+    for k in ("x", "v", "fe", "fp"):
+        assert np.array_equal(pa[k], pb[k]), k
+
+
 def test_grid_update_teacher_forced(oracle_mod, c1_scene):
     sc = with_colliders(deformed(scenes.snowball_drop()))
     o = oracle_mod.Oracle(sc)
