@@ -84,9 +84,9 @@ struct snow_ctx {
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   uint32_t *d_tile_lo = nullptr, *d_tile_hi = nullptr;
-  double* d_tpart = nullptr;
-  double* d_hpart = nullptr;
-  double* d_zero_part = nullptr;  // NQ * HALO_NODES zeros
+  float* d_tpart = nullptr;
+  float* d_hpart = nullptr;
+  float* d_zero_part = nullptr;  // NQ * HALO_NODES zeros
   uint32_t* d_list_ctr = nullptr;  // TILE_LISTS append counters
   uint32_t* d_tile_list = nullptr; // TILE_LISTS sublists of total_ctiles entries
   StepCtl* d_ctl = nullptr;
@@ -604,7 +604,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_list_ctr, size_t(TILE_LISTS) * TILE_LIST_STRIDE));
   CREATECHK(hipMemset(c->d_list_ctr, 0, sizeof(uint32_t) * TILE_LISTS * TILE_LIST_STRIDE));
   CREATECHK(dalloc(&c->d_tile_list, size_t(TILE_LISTS) * c->total_ctiles));
-  CREATECHK(hipMemset(c->d_zero_part, 0, sizeof(double) * NQ * HALO_NODES));
+  CREATECHK(hipMemset(c->d_zero_part, 0, sizeof(float) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_ctl, 1));
   CREATECHK(dalloc(&c->d_pbound, size_t(c->nscenes) * 2 * PB_SLOTS * PB_STRIDE));
   CREATECHK(dalloc(&c->d_tile_stamp, c->total_ctiles));

@@ -12,7 +12,8 @@
 //    g_mmv,g_f: float4 (p, m), (f, 0)    - gv / gf of the reference (stage API, debug)
 //  cell tiles: 4x4x4 cells per tile over the shifted cell range c = base + 2, base in [-2, D]
 //    (a cubic stencil of base cell b touches nodes b-1..b+2); a tile's 7^3-node halo starts
-//    at node 4k-3; tpart[tile][q][7^3] holds the tile's partial sums of the 7 P2G quantities.
+//    at node 4k-3; tpart[tile][q][7^3] holds the tile's partial sums of the 7 P2G quantities
+//    (exact fixed-point sums, rounded once to float).
 #pragma once
 #include "snow_device.hpp"
 
@@ -366,8 +367,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     uint32_t* __restrict__ tile_lo, uint32_t* __restrict__ tile_hi, uint32_t* __restrict__ tile_stamp,
     const StepCtl* __restrict__ ctl, uint32_t* __restrict__ list_ctr, uint32_t* __restrict__ tile_list,
-    uint32_t list_cap, const float4* __restrict__ rec, double* __restrict__ tpart,
-    double* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene, uint32_t* __restrict__ pbound,
+    uint32_t list_cap, const float4* __restrict__ rec, float* __restrict__ tpart,
+    float* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene, uint32_t* __restrict__ pbound,
     int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
@@ -633,7 +634,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     }
     if (cur >= 0) commit();
     __syncthreads();
-    double* out = starts_tile ? tpart + size_t(tile) * (NQ * HALO_NODES) : hpart + size_t(chunk) * (NQ * HALO_NODES);
+    float* out = starts_tile ? tpart + size_t(tile) * (NQ * HALO_NODES) : hpart + size_t(chunk) * (NQ * HALO_NODES);
     if (t == 0) {
       if (starts_tile) {
         tile_lo[tile] = lo + r_lo;
@@ -649,7 +650,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       const int e = q == 0 ? em : (q < 4 ? ep : ef);
       const unsigned long long v = s_acc[i];
       s_acc[i] = 0ull;
-      out[i] = ldexp(double((long long)v), -e);  // relative rounding <= 2^-53
+      out[i] = float(ldexp(double((long long)v), -e));  // exact in double, one rounding to float
     }
     __syncthreads();
   }
@@ -658,14 +659,14 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
 // ---------------------------------------------------------------------------------------
 // A tile whose particles span several P2G chunks has its first run in tpart[tile] and one
 // continuation partial per later chunk. The block of the tile's last chunk adds the
-// continuations to tpart[tile] in chunk order (double sums of exact fixed-point partials).
+// continuations to tpart[tile] in chunk order (double sums of the float partials).
 __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restrict__ scenes,
                                                        const uint16_t* __restrict__ chunk_scene,
                                                        const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ tile_lo,
                                                        const uint32_t* __restrict__ tile_hi,
-                                                       double* __restrict__ tpart,
-                                                       const double* __restrict__ hpart) {
+                                                       float* __restrict__ tpart,
+                                                       const float* __restrict__ hpart) {
   const uint32_t chunk = blockIdx.x;
   const SceneDev& S = scenes[chunk_scene[chunk]];
   const uint32_t pbase = uint32_t(S.p_off);
@@ -676,19 +677,19 @@ __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restri
   const uint32_t tlo = tile_lo[tile], thi = tile_hi[tile];
   if (tlo == lo || thi > hi) return;  // no continuation here, or a later chunk folds
   const uint32_t c_first = S.chunk_off + (tlo - pbase) / S.seg + 1u;
-  double* dst = tpart + size_t(tile) * (NQ * HALO_NODES);
+  float* dst = tpart + size_t(tile) * (NQ * HALO_NODES);
   for (int i = threadIdx.x; i < NQ * HALO_NODES; i += blockDim.x) {
-    double acc = dst[i];
+    double acc = double(dst[i]);
     // eight loads in flight at a time; the sum stays in chunk order (adding 0.0 is exact)
     for (uint32_t c = c_first; c <= chunk; c += 8) {
       double v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        v[u] = c + u <= chunk ? hpart[size_t(c + u) * (NQ * HALO_NODES) + i] : 0.0;
+        v[u] = c + u <= chunk ? double(hpart[size_t(c + u) * (NQ * HALO_NODES) + i]) : 0.0;
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc += v[u];
     }
-    dst[i] = acc;
+    dst[i] = float(acc);
   }
 }
 
@@ -714,8 +715,8 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
                                                        const uint32_t* __restrict__ tile_list, uint32_t list_cap,
                                                        const StepCtl* __restrict__ ctl,
                                                        const uint32_t* __restrict__ tile_stamp,
-                                                       const double* __restrict__ tpart,
-                                                       const double* __restrict__ zero_part,
+                                                       const float* __restrict__ tpart,
+                                                       const float* __restrict__ zero_part,
                                                        float4* __restrict__ g_mmv, float4* __restrict__ g_f,
                                                        float2* __restrict__ g_node, float* __restrict__ g_mass,
                                                        const Collider* __restrict__ cols, float dt_override,
@@ -759,10 +760,10 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
     const int nx = TILE * Kx + rx, ny = TILE * Ky + ry, nz = TILE * Kz + rz;
     const bool inside = nx < S.D[0] && ny < S.D[1] && nz < S.D[2];
     if (!inside) continue;
-    // the tile partials are exact fixed-point sums held in double; their sum in double, in a
-    // fixed order, has a relative error of a few 2^-53 before the one rounding to float.
+    // the tile partials are exact fixed-point sums rounded once to float; they are summed in
+    // double, in a fixed order, before the one rounding of the node value to float.
     // Candidates that do not contribute read a zero block, so all loads issue together.
-    const double* src[8];
+    const float* src[8];
 #pragma unroll
     for (int bb = 0; bb < 8; ++bb) {
       const int bx = bb & 1, by = (bb >> 1) & 1, bz = bb >> 2;
@@ -774,9 +775,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      acc[q] = src[0][q * HALO_NODES];
+      acc[q] = double(src[0][q * HALO_NODES]);
 #pragma unroll
-      for (int bb = 1; bb < 8; ++bb) acc[q] += src[bb][q * HALO_NODES];
+      for (int bb = 1; bb < 8; ++bb) acc[q] += double(src[bb][q * HALO_NODES]);
     }
     const float m = float(acc[0]);
     const float px = float(acc[1]), py = float(acc[2]), pz = float(acc[3]);
@@ -803,7 +804,10 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
       }
       store_node(g_node, g_mass, node, vx, vy, vz, bx, by, bz, m);
     }
-    if (active_count && m > 0.0f) atomicAdd(active_count + sid, 1ull);
+    // active nodes of the scene: one atomic per wave (all lanes of the wave share the scene)
+    const unsigned long long act = __ballot(m > 0.0f);
+    if (active_count && lane == __ffsll((long long)__ballot(true)) - 1 && act)
+      atomicAdd(active_count + sid, (unsigned long long)__popcll(act));
   }
 }
 
