@@ -189,8 +189,9 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
 // evaluates the particle's 12 weights and 12 gradients once, from records loaded one batch ahead.
 // The node sums of a cell are products of small matrices over the cell's particles, done on the
 // matrix cores (below). When the cell changes they are added to the run's LDS accumulator as
-// 64-bit fixed point: integer addition is associative, so the run sums are exact whatever the
-// order of the commits, and the result is bit-reproducible. The fixed-point scale is a power of
+// fixed point: rounded to integers held in doubles, whose sums stay below 2^53 and are therefore
+// exact, so the run sums do not depend on the order of the commits and the result is
+// bit-reproducible. The fixed-point scale is a power of
 // two per chunk and quantity, 2^FIX_BITS / (count * max |per-particle term| of the scene), so no
 // sum can reach 2^FIX_BITS and the resolution is ~2^-39 of the scene's largest term for a
 // 2048-particle chunk.
@@ -213,7 +214,7 @@ __device__ __forceinline__ P2GBound particle_bound(const float4& r0, const float
   return b;
 }
 
-constexpr int FIX_BITS = 50;  // |fixed-point sums| < 2^FIX_BITS
+constexpr int FIX_BITS = 50;  // |fixed-point sums| < 2^FIX_BITS < 2^53: exact in double
 
 // exponent e with count*maxval*2^e <= 2^FIX_BITS (clamped to the float range)
 __device__ __forceinline__ int fixed_exponent(float maxval, uint32_t count) {
@@ -223,13 +224,6 @@ __device__ __forceinline__ int fixed_exponent(float maxval, uint32_t count) {
   frexpf(bound, &ex);  // bound < 2^ex
   const int e = FIX_BITS - ex;
   return e > 126 ? 126 : (e < -126 ? -126 : e);
-}
-
-// Round-to-nearest-even of x to an integer, for |x| < 2^51: x converts to double exactly, and
-// adding 1.5*2^52 puts the rounded value into the low mantissa bits of a fixed binade.
-__device__ __forceinline__ long long fixed_round(float x) {
-  const double magic = 6755399441055744.0;  // 1.5 * 2^52
-  return __double_as_longlong(double(x) + magic) - __double_as_longlong(magic);
 }
 
 // Fixed-point bounds of the next P2G. Every kernel that writes particle records for a P2G (the
@@ -372,7 +366,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
-  __shared__ unsigned long long s_acc[NQ * HALO_NODES];
+  __shared__ double s_acc[NQ * HALO_NODES];  // integer-valued fixed-point sums
   // per-wave staging of P2G_CHUNK particles: for each axis the interleaved (w_o, q_o), o = 0..3
   // (24 floats), then (m, m*v) (4), then the rows of A as (A_c0, A_c1, A_c2, cell) for c = 0..2
   // and a zero row (0, 0, 0, cell) (16); cell = local cell of the particle, as bits
@@ -447,7 +441,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     // the scene's first chunk clears the slots the writers of this step fill for the next one
     if (chunk == S.chunk_off && t < PB_SLOTS * 4)
       bound_slots(pbound, sid, par ^ 1u)[(t >> 2) * PB_STRIDE + (t & 3)] = 0u;
-    for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0ull;
+    for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0.0;
   }
   __syncthreads();
   if (wave == 0) {  // ordered run starts from the bit mask, one 32-position word per lane
@@ -531,14 +525,14 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       const int cx = cur & 3, cy = (cur >> 2) & 3, cz = cur >> 4;
       // D row block kp is quantity kp (D1) and force row kp (D2); r = o_x; column (rx, rq)
       const int jn = cx + HALO * (cy + rx) + HALO * HALO * (cz + rq);
-      unsigned long long* a1 = s_acc + kp * HALO_NODES + jn;
+      double* a1 = s_acc + kp * HALO_NODES + jn;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) atomicAdd(a1 + i, (unsigned long long)fixed_round(acc1[i] * s1));
+      for (int i = 0; i < 4; ++i) atomicAdd(a1 + i, double(rintf(acc1[i] * s1)));
       if (kp < 3) {
         // the sums hold +A grad(w); the node force is -A grad(w)
-        unsigned long long* a2 = s_acc + (4 + kp) * HALO_NODES + jn;
+        double* a2 = s_acc + (4 + kp) * HALO_NODES + jn;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) atomicAdd(a2 + i, (unsigned long long)fixed_round(-acc2[i] * sf));
+        for (int i = 0; i < 4; ++i) atomicAdd(a2 + i, double(rintf(-acc2[i] * sf)));
       }
       acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
       acc2 = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -648,9 +642,9 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     for (int i = t; i < NQ * HALO_NODES; i += NT) {
       const int q = i / HALO_NODES;
       const int e = q == 0 ? em : (q < 4 ? ep : ef);
-      const unsigned long long v = s_acc[i];
-      s_acc[i] = 0ull;
-      out[i] = float(ldexp(double((long long)v), -e));  // exact in double, one rounding to float
+      const double v = s_acc[i];
+      s_acc[i] = 0.0;
+      out[i] = float(ldexp(v, -e));  // one rounding to float
     }
     __syncthreads();
   }
