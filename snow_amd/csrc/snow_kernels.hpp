@@ -183,16 +183,17 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
 // same alone or in a batch.
 // The workgroup first reads the chunk's keys and slots once, all loads in flight together: it
 // keeps the slots and local cells in LDS and marks where the cell tile changes (the chunk falls
-// apart into runs of one tile, contiguous and cell by cell); the fixed-point bound is the scene's,
-// published by the writers of the records (below). Then, run by run, the 8 waves take equal contiguous
-// shares and stage P2G_CHUNK particles at a time into their LDS slot: one lane per particle
-// evaluates the particle's 12 weights and 12 gradients once, from records loaded one batch ahead.
-// The node sums of a cell are products of small matrices over the cell's particles, done on the
-// matrix cores (below). When the cell changes they are added to the run's LDS accumulator as
-// fixed point: rounded to integers held in doubles, whose sums stay below 2^53 and are therefore
-// exact, so the run sums do not depend on the order of the commits and the result is
-// bit-reproducible. The fixed-point scale is a power of
-// two per chunk and quantity, 2^FIX_BITS / (count * max |per-particle term| of the scene), so no
+// apart into runs of one tile, contiguous and cell by cell); the fixed-point bound is the
+// scene's, published by the writers of the records (below). Then, run by run, the 8 waves take
+// equal contiguous shares and stage P2G_CHUNK particles at a time into their LDS slot: one lane
+// per particle evaluates the particle's 12 weights and 12 gradients once, from records loaded one
+// batch ahead. Then lane l of the wave is the stencil offset o = (l&3, (l>>2)&3, l>>4) (getIJK
+// order, neighborIndex.include.glsl:6-12): for each staged particle the 64 lanes compute the 64
+// node contributions from broadcast LDS reads and sum them in registers. When the cell changes,
+// the register sums are added to the run's LDS accumulator as fixed point: rounded to integers
+// held in doubles, whose sums stay below 2^53 and are therefore exact, so the run sums do not
+// depend on the order of the commits and the result is bit-reproducible. The fixed-point scale
+// is a power of two per chunk and quantity, 2^FIX_BITS / (count * max |per-particle term| of the scene), so no
 // sum can reach 2^FIX_BITS and the resolution is ~2^-39 of the scene's largest term for a
 // 2048-particle chunk.
 // A run that starts its tile writes the tile partial tpart[tile]; a run that continues a tile
@@ -351,10 +352,8 @@ __global__ void __launch_bounds__(256) k_particle_bounds(const SceneDev* __restr
 }
 
 constexpr int P2G_CHUNK = 32;        // particles staged per wave at a time
-constexpr int P2G_STAGE = 44;        // floats per staged particle
+constexpr int P2G_STAGE = 40;        // floats per staged particle
 constexpr uint8_t P2G_DEAD = 0xff;   // local-cell mark of out-of-grid particles
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
@@ -368,8 +367,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
   __shared__ double s_acc[NQ * HALO_NODES];  // integer-valued fixed-point sums
   // per-wave staging of P2G_CHUNK particles: for each axis the interleaved (w_o, q_o), o = 0..3
-  // (24 floats), then (m, m*v) (4), then the rows of A as (A_c0, A_c1, A_c2, cell) for c = 0..2
-  // and a zero row (0, 0, 0, cell) (16); cell = local cell of the particle, as bits
+  // (24 floats), then m, m*v (4), then A row-major (9), 3 pad
   __shared__ __attribute__((aligned(16))) float s_stage[P2G_WAVES][P2G_CHUNK * P2G_STAGE];
   __shared__ uint32_t s_val[P2G_SEG_MAX];        // slots of the chunk's particles
   __shared__ uint8_t s_cell[P2G_SEG_MAX];        // their local cells (P2G_DEAD: out of grid)
@@ -382,12 +380,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int lane = t & 63;
-  // Matrix-core roles of the lane (v_mfma_f32_16x16x4f32: lane l supplies A[l%16][l/16] and
-  // B[l/16][l%16] and receives D[4(l/16) + r][l%16], r = 0..3). Rows are (quantity, o_x) =
-  // 4*quantity + o_x, columns (o_y, o_z) = o_y + 4*o_z, and K runs over 4 particles.
-  const int kp = lane >> 4;         // particle of the group whose operands the lane supplies
-  const int rq = (lane >> 2) & 3;   // A operand row block (quantity); B operand column o_z
-  const int rx = lane & 3;          // A operand row o_x; B operand column o_y
+  // lane l is the stencil offset o = (l&3, (l>>2)&3, l>>4) (getIJK order, neighborIndex.include.glsl:6-12)
+  const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
   float* stage = s_stage[wave];
   const uint32_t chunk = blockIdx.x;
   const uint32_t step = ctl->step;
@@ -509,49 +503,37 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const bool starts_tile = r > 0 || s_edge[0] != tile;
     const bool ends_tile = r + 1 < nrun || s_edge[1] != tile;
 
-    // Per cell the 64 node sums are small matrix products over the cell's particles p:
-    //   mass/momentum  D1[(q, o_x), (o_y, o_z)] = sum_p (mq_p wx_p[o_x]) (wy_p[o_y] wz_p[o_z]),
-    //                  mq = (m, m v);
-    //   A grad(w)      D2[(c, o_x), (o_y, o_z)] = sum_p sum_t A_p[c][t] gx_t[o_x] gyz_t[o_y, o_z],
-    //                  (gx, gyz)_0 = (qx, wy wz), _1 = (wx, qy wz), _2 = (wx, wy qz),
-    // which is A_p (qx wy wz, wx qy wz, wx wy qz)^T, A grad(w) of p2gTransfer.glsl:109-141.
-    // Each group of 4 staged particles is one K=4 step of D1 and three (t = 0,1,2) of D2, on
-    // the matrix cores. A group whose particles span several cells is applied once per cell,
-    // with the B operands of the other cells' particles set to zero.
-    int cur = -1;  // local cell of the accumulators
-    f32x4 acc1 = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
-    const float s1 = kp == 0 ? sm : sp;
+    // Lane l sums the contributions of the staged particles to its stencil node o in registers
+    // (mass, momentum and A grad(w), p2gTransfer.glsl:84-146) while the cell stays the same;
+    // all particles of a cell share the 64 stencil nodes.
+    int cur = -1;  // local cell of the register sums
+    float am = 0.f, apx = 0.f, apy = 0.f, apz = 0.f, afx = 0.f, afy = 0.f, afz = 0.f;
     auto commit = [&]() {
       const int cx = cur & 3, cy = (cur >> 2) & 3, cz = cur >> 4;
-      // D row block kp is quantity kp (D1) and force row kp (D2); r = o_x; column (rx, rq)
-      const int jn = cx + HALO * (cy + rx) + HALO * HALO * (cz + rq);
-      double* a1 = s_acc + kp * HALO_NODES + jn;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) atomicAdd(a1 + i, double(rintf(acc1[i] * s1)));
-      if (kp < 3) {
-        // the sums hold +A grad(w); the node force is -A grad(w)
-        double* a2 = s_acc + (4 + kp) * HALO_NODES + jn;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) atomicAdd(a2 + i, double(rintf(-acc2[i] * sf)));
-      }
-      acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
-      acc2 = f32x4{0.f, 0.f, 0.f, 0.f};
+      double* a = s_acc + (cx + ox) + HALO * (cy + oy) + HALO * HALO * (cz + oz);
+      atomicAdd(a + 0 * HALO_NODES, double(rintf(am * sm)));
+      atomicAdd(a + 1 * HALO_NODES, double(rintf(apx * sp)));
+      atomicAdd(a + 2 * HALO_NODES, double(rintf(apy * sp)));
+      atomicAdd(a + 3 * HALO_NODES, double(rintf(apz * sp)));
+      // the sums hold +A grad(w); the node force is -A grad(w)
+      atomicAdd(a + 4 * HALO_NODES, double(rintf(-afx * sf)));
+      atomicAdd(a + 5 * HALO_NODES, double(rintf(-afy * sf)));
+      atomicAdd(a + 6 * HALO_NODES, double(rintf(-afz * sf)));
+      am = apx = apy = apz = afx = afy = afz = 0.f;
     };
     while (nr == r) {  // the wave's batches of this run
       const uint32_t cnt = min(uint32_t(P2G_CHUNK), ne - nc);
-      // stage: lane k < cnt evaluates the weights of its prefetched particle; the lanes
-      // cnt <= k < P2G_CHUNK write zero records, which contribute nothing to a group
+      // stage: lane k < cnt evaluates the weights of its prefetched particle once
       const uint32_t cellv = uint32_t(lane) < cnt ? qcell : 0xffffffffu;
-      if (lane < P2G_CHUNK) {
+      if (uint32_t(lane) < cnt) {
         float4* d4 = reinterpret_cast<float4*>(stage + lane * P2G_STAGE);
-        if (uint32_t(lane) < cnt) {
+        {
           // p2gTransfer.glsl:64-66 skips |m| > 100 (gMassCompute does not)
           const float use = (skip_heavy && fabsf(q0.w) > 100.0f) ? 0.0f : 1.0f;
           const float m = q0.w * use;
           // every particle of a cell has base = floor(u) = that cell; f = u - base
           const float ux = fmaf(q0.x, inv_h, uo0), uy = fmaf(q0.y, inv_h, uo1), uz = fmaf(q0.z, inv_h, uo2);
           const float fx = ux - floorf(ux), fy = uy - floorf(uy), fz = uz - floorf(uz);
-          const float cb = __uint_as_float(cellv);
           d4[0] = make_float4(spline_w(0, fx), spline_q(0, fx), spline_w(1, fx), spline_q(1, fx));
           d4[1] = make_float4(spline_w(2, fx), spline_q(2, fx), spline_w(3, fx), spline_q(3, fx));
           d4[2] = make_float4(spline_w(0, fy), spline_q(0, fy), spline_w(1, fy), spline_q(1, fy));
@@ -559,17 +541,9 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
           d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
           d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
           d4[6] = make_float4(m, q1.w * m, q2.w * m, q3.w * m);
-          d4[7] = make_float4(q1.x * use, q1.y * use, q1.z * use, cb);
-          d4[8] = make_float4(q2.x * use, q2.y * use, q2.z * use, cb);
-          d4[9] = make_float4(q3.x * use, q3.y * use, q3.z * use, cb);
-          d4[10] = make_float4(0.f, 0.f, 0.f, cb);
-        } else {
-          const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-          const float4 zc = make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
-#pragma unroll
-          for (int i = 0; i < 7; ++i) d4[i] = z;
-#pragma unroll
-          for (int i = 7; i < 11; ++i) d4[i] = zc;
+          d4[7] = make_float4(q1.x * use, q1.y * use, q1.z * use, q2.x * use);
+          d4[8] = make_float4(q2.y * use, q2.z * use, q3.x * use, q3.y * use);
+          d4[9] = make_float4(q3.z * use, 0.f, 0.f, 0.f);
         }
       }
       // advance the cursor and load the next batch's records while this one is summed
@@ -578,51 +552,31 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       fetch();
       // the wave's own LDS writes complete before its reads (in-order LDS per wave)
       __builtin_amdgcn_wave_barrier();
-      const uint32_t ngroups = (cnt + 3u) >> 2;
-      for (uint32_t g = 0; g < ngroups; ++g) {
-        const uint32_t k0 = 4u * g, kl = min(k0 + 3u, cnt - 1u);
-        const int c_first = __builtin_amdgcn_readlane(int(cellv), int(k0));
-        const int c_last = __builtin_amdgcn_readlane(int(cellv), int(kl));
-        const float* d = stage + (k0 + uint32_t(kp)) * P2G_STAGE;
-        const float2 xq = reinterpret_cast<const float2*>(d)[rx];
-        const float2 yq = reinterpret_cast<const float2*>(d + 8)[rx];
-        const float2 zq = reinterpret_cast<const float2*>(d + 16)[rq];
-        const float mq = d[24 + rq];
-        const float4 ar = reinterpret_cast<const float4*>(d + 28)[rq];
-        const float a1 = mq * xq.x;
-        const float a20 = ar.x * xq.y, a21 = ar.y * xq.x, a22 = ar.z * xq.x;
-        const float b0 = yq.x * zq.x, b1 = yq.y * zq.x, b2 = yq.x * zq.y;
-        if (c_first != cur) {
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const int cell = __builtin_amdgcn_readlane(int(cellv), int(k));
+        if (cell != cur) {
           if (cur >= 0) commit();
-          cur = c_first;
+          cur = cell;
         }
-        if (c_last == c_first) {
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc1, 0, 0, 0);
-          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a20, b0, acc2, 0, 0, 0);
-          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a21, b1, acc2, 0, 0, 0);
-          acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a22, b2, acc2, 0, 0, 0);
-        } else {
-          // the group's cells are non-decreasing from c_first to c_last: one pass per cell
-          const int pc = int(__float_as_uint(ar.w));
-          for (;;) {
-            const bool in = pc == cur;
-            const float m0 = in ? b0 : 0.f, m1 = in ? b1 : 0.f, m2 = in ? b2 : 0.f;
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, m0, acc1, 0, 0, 0);
-            acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a20, m0, acc2, 0, 0, 0);
-            acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a21, m1, acc2, 0, 0, 0);
-            acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a22, m2, acc2, 0, 0, 0);
-            if (cur == c_last) break;
-            commit();
-            // next cell: that of the first particle of the group beyond the current one
-            int nxt = c_last;
-#pragma unroll
-            for (int i = 2; i >= 1; --i) {
-              const int ci = __builtin_amdgcn_readlane(int(cellv), int(min(k0 + uint32_t(i), kl)));
-              nxt = ci > cur ? ci : nxt;
-            }
-            cur = nxt;
-          }
-        }
+        const float* d = stage + k * P2G_STAGE;
+        const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
+        const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
+        const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
+        const float4 u0 = reinterpret_cast<const float4*>(d)[6];
+        const float4 u1 = reinterpret_cast<const float4*>(d)[7];
+        const float4 u2 = reinterpret_cast<const float4*>(d)[8];
+        const float a22 = d[36];
+        // w = wx wy wz and grad(w) = (qx wy wz, wx qy wz, wx wy qz) (cubic.include.glsl:36-40)
+        const float wyz = wqy.x * wqz.x;
+        const float w = wqx.x * wyz;
+        const float g0 = wqx.y * wyz, g1 = wqy.y * (wqx.x * wqz.x), g2 = wqz.y * (wqx.x * wqy.x);
+        am = fmaf(u0.x, w, am);
+        apx = fmaf(u0.y, w, apx);
+        apy = fmaf(u0.z, w, apy);
+        apz = fmaf(u0.w, w, apz);
+        afx = fmaf(u1.x, g0, fmaf(u1.y, g1, fmaf(u1.z, g2, afx)));
+        afy = fmaf(u1.w, g0, fmaf(u2.x, g1, fmaf(u2.y, g2, afy)));
+        afz = fmaf(u2.z, g0, fmaf(u2.w, g1, fmaf(a22, g2, afz)));
       }
       __builtin_amdgcn_wave_barrier();
     }
