@@ -17,6 +17,7 @@
 #define SNOW_TIME_UPDATE_HPP
 
 #include <cstdint>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -70,6 +71,21 @@ class SnowExplicitTimeUpdate : public TimeUpdate {
     if (fp) fp->resize(9 * n);
     check(snow_read_particles(ctx_, scene, x ? x->data() : nullptr, v ? v->data() : nullptr,
                               fe ? fe->data() : nullptr, fp ? fp->data() : nullptr, nullptr));
+  }
+
+  // One frame of one scene as a binary little-endian PLY point set with the properties x, y, z and
+  // mass: the vec4 (x, m) buffer the reference renders as GL points (particlesystem.cpp:4-13,
+  // pPoints.vert:2-12), in creation order. The host must be little-endian.
+  void write_ply(int32_t scene, const std::string& path) {
+    const size_t n = size_t(num_particles(scene));
+    std::vector<float> pxm(4 * n);
+    check(snow_export_reference_particles(ctx_, scene, pxm.data(), nullptr, nullptr, nullptr));
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("snow: cannot write " + path);
+    f << "ply\nformat binary_little_endian 1.0\nelement vertex " << n
+      << "\nproperty float x\nproperty float y\nproperty float z\nproperty float mass\nend_header\n";
+    f.write(reinterpret_cast<const char*>(pxm.data()), std::streamsize(pxm.size() * sizeof(float)));
+    if (!f) throw std::runtime_error("snow: write failed: " + path);
   }
 
   snow_ctx* handle() { return ctx_; }

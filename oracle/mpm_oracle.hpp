@@ -491,13 +491,20 @@ template <class R> void rigid_advance(Scene<R>& s, R dt) {
   for (int i = 0; i < s.nc; ++i) s.col[i].pos = s.col[i].pos + s.col[i].vel * dt;
 }
 
-// gForceCompute.glsl:57-103 (bounds-checked). Node position = origin + h*(i,j,k) (grid.cpp:205-215).
+// Node coordinate as the reference stores it in the gxm buffer: computed on the host in float,
+// x_off + h * (float)i with two roundings (grid.cpp:98-102), and read by the shader as data. (In
+// exact arithmetic a node can lie exactly on a collider plane; the float value decides.)
+inline float node_coord(float x_off, float h, int64_t i) { return x_off + h * float(i); }
+
+// gForceCompute.glsl:57-103 (bounds-checked).
 template <class R> void grid_update(Scene<R>& s, R dt) {
   const int Dx = s.D[0], Dy = s.D[1];
   parallel_for(s.nthreads, s.nodes(), [&](int, int64_t b, int64_t e) {
     for (int64_t gI = b; gI < e; ++gI) {
       int64_t i = gI % Dx, j = (gI / Dx) % Dy, k = gI / (int64_t(Dx) * Dy);
-      Vec3<R> xi = v3(s.origin[0] + s.h * R(i), s.origin[1] + s.h * R(j), s.origin[2] + s.h * R(k));
+      const float hf = float(s.h);
+      Vec3<R> xi = v3(R(node_coord(float(s.origin[0]), hf, i)), R(node_coord(float(s.origin[1]), hf, j)),
+                      R(node_coord(float(s.origin[2]), hf, k)));
       R mi = s.gm[gI];
       Vec3<R> vi = v3(s.gmv[3*gI], s.gmv[3*gI+1], s.gmv[3*gI+2]);
       Vec3<R> fi = v3(s.gf[3*gI], s.gf[3*gI+1], s.gf[3*gI+2]);

@@ -24,7 +24,7 @@ from ._capi import (FLAG_COMPENSATED_G2P, FLAG_KEEP_GRID, FLAG_MIXED, Collider a
 __all__ = [
     "Material", "Grid", "Collider", "CollisionObjects", "ParticleSystem", "PhysicalScene", "ExplicitTimeUpdate",
     "MPMPhysicEngine", "SnowContext", "SnowError", "FLAG_COMPENSATED_G2P", "FLAG_MIXED", "FLAG_KEEP_GRID",
-    "HALF_PLANE", "SPHERE", "BOX", "default_boundary_colliders",
+    "HALF_PLANE", "SPHERE", "BOX", "default_boundary_colliders", "read_ply",
 ]
 
 HALF_PLANE, SPHERE, BOX = 0, 1, 2
@@ -202,6 +202,58 @@ class PhysicalScene:
             for a in (ps.x, ps.v, ps.mass, ps.fe, ps.fp):
                 f.write(np.ascontiguousarray(a, np.float32).tobytes())
 
+    @classmethod
+    def load(cls, filename: str) -> "PhysicalScene":
+        """Read a SNOWSCN1 scene file (the format of save() and of the C++ driver's --dump-scene)."""
+        with open(filename, "rb") as f:
+            data = f.read()
+        if data[:8] != b"SNOWSCN1":
+            raise ValueError(f"{filename}: not a SNOWSCN1 scene file")
+        off = 8
+
+        def take(dtype, count):
+            nonlocal off
+            a = np.frombuffer(data, dtype, count, off)
+            off += a.nbytes
+            return a
+
+        n = int(take(np.int64, 1)[0])
+        dim = take(np.int32, 3)
+        origin, h = take(np.float32, 3), float(take(np.float32, 1)[0])
+        mt = take(np.float32, 10)
+        ncol = int(take(np.int32, 1)[0])
+        cols = take(np.float32, 12 * ncol).reshape(ncol, 12)
+        dt = float(take(np.float32, 1)[0])
+        x, v, mass = take(np.float32, 3 * n), take(np.float32, 3 * n), take(np.float32, n)
+        fe, fp = take(np.float32, 9 * n), take(np.float32, 9 * n)
+        grid = Grid(int(dim[0]), int(dim[1]), int(dim[2]), h, float(origin[0]), float(origin[1]), float(origin[2]))
+        material = Material(float(mt[0]), float(mt[1]), float(mt[2]), float(mt[3]), float(mt[4]), float(mt[5]),
+                            (float(mt[6]), float(mt[7]), float(mt[8])), float(mt[9]))
+        colliders = [Collider(tuple(map(float, c[0:3])), int(c[3]), tuple(map(float, c[4:7])), float(c[7]),
+                              tuple(map(float, c[8:11])), float(c[11])) for c in cols]
+        ps = ParticleSystem(x.reshape(n, 3), v.reshape(n, 3), mass, fe.reshape(n, 9), fp.reshape(n, 9))
+        return cls(ps, grid, CollisionObjects(colliders), material, dt)
+
+
+def read_ply(filename: str) -> dict:
+    """Read a binary little-endian PLY point set of float properties (the frames written by
+    SnowContext.export_ply and the C++ driver's --ply): {property name: array [N]}."""
+    with open(filename, "rb") as f:
+        data = f.read()
+    end = data.index(b"end_header\n") + len(b"end_header\n")
+    header = data[:end].decode("ascii").split("\n")
+    if header[0] != "ply" or header[1] != "format binary_little_endian 1.0":
+        raise ValueError(f"{filename}: not a binary little-endian PLY file")
+    n, props = 0, []
+    for line in header[2:]:
+        w = line.split()
+        if w[:2] == ["element", "vertex"]:
+            n = int(w[2])
+        elif w[:2] == ["property", "float"]:
+            props.append(w[2])
+    a = np.frombuffer(data, np.float32, n * len(props), end).reshape(n, len(props))
+    return {p: a[:, i].copy() for i, p in enumerate(props)}
+
 
 def load_run_output(filename: str) -> dict:
     """Read the SNOWOUT1 file written by the C++ driver: x, v [N,3], fe, fp [N,9]."""
@@ -297,6 +349,17 @@ class SnowContext:
         self._chk(fn(self.h) if dt is None else fn(self.h, dt))
 
     # -- state transfer
+    def export_ply(self, filename: str, scene: int = 0) -> None:
+        """Write one frame of a scene as a binary little-endian PLY point set (x, y, z, mass): the
+        vec4 (x, m) buffer the reference renders as GL points (particlesystem.cpp:4-13), in creation
+        order."""
+        pxm, _, _, _ = self.export_reference(scene)
+        pxm = np.ascontiguousarray(pxm, "<f4").reshape(-1, 4)
+        with open(filename, "wb") as f:
+            f.write(("ply\nformat binary_little_endian 1.0\nelement vertex %d\nproperty float x\nproperty float y\n"
+                     "property float z\nproperty float mass\nend_header\n" % len(pxm)).encode("ascii"))
+            f.write(pxm.tobytes())
+
     def read_particles(self, scene: int = 0) -> dict:
         n = self.lib.snow_num_particles(self.h, scene)
         out = {"x": np.empty((n, 3), np.float32), "v": np.empty((n, 3), np.float32),

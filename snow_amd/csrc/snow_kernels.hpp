@@ -55,6 +55,13 @@ struct SceneDev {
 // Grid coordinate of a position: u = (x - origin)/h evaluated as x*(1/h) - origin/h (one FMA).
 __device__ __forceinline__ float grid_u(const SceneDev& S, float x, int k) { return fmaf(x, S.inv_h, S.uo[k]); }
 
+// Node coordinate as the reference stores it in the gxm buffer: origin + h * (float)i computed on
+// the host in float with two roundings (grid.cpp:98-102); kept unfused, because a node can lie
+// exactly on a collider plane and the rounded value decides whether it collides.
+__device__ __forceinline__ float node_coord(const SceneDev& S, int k, int i) {
+  return __fadd_rn(S.origin[k], __fmul_rn(S.h, float(i)));
+}
+
 // Cubic B-spline weights of the four stencil nodes of one axis, for base = floor(u) and
 // f = u - base in [0,1). Node o (o = 0..3) sits at base + o - 1, so its distance d = o - 1 - f
 // always falls in the same polynomial piece of cubic.include.glsl:7-16:
@@ -742,9 +749,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
         vx = px / m + dt * (fx / m + S.g[0]);
         vy = py / m + dt * (fy / m + S.g[1]);
         vz = pz / m + dt * (fz / m + S.g[2]);
-        const float xi = S.origin[0] + S.h * float(nx);
-        const float yi = S.origin[1] + S.h * float(ny);
-        const float zi = S.origin[2] + S.h * float(nz);
+        const float xi = node_coord(S, 0, nx), yi = node_coord(S, 1, ny), zi = node_coord(S, 2, nz);
         collide(xi, yi, zi, vx, vy, vz, cols + S.col_off, S.ncol);
         bx = vx - S.alpha * (px / m);
         by = vy - S.alpha * (py / m);
@@ -782,8 +787,7 @@ __global__ void __launch_bounds__(256) k_grid_update_dense(const SceneDev* __res
     vx = mm.x / m + dt * (ff.x / m + S.g[0]);
     vy = mm.y / m + dt * (ff.y / m + S.g[1]);
     vz = mm.z / m + dt * (ff.z / m + S.g[2]);
-    collide(S.origin[0] + S.h * float(nx), S.origin[1] + S.h * float(ny), S.origin[2] + S.h * float(nz), vx, vy,
-            vz, cols + S.col_off, S.ncol);
+    collide(node_coord(S, 0, nx), node_coord(S, 1, ny), node_coord(S, 2, nz), vx, vy, vz, cols + S.col_off, S.ncol);
     bx = vx - S.alpha * (mm.x / m);
     by = vy - S.alpha * (mm.y / m);
     bz = vz - S.alpha * (mm.z / m);

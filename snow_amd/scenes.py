@@ -11,11 +11,16 @@ Workloads (unit cube domain, node i at i*h):
 * ``two_snowballs``      128^3, h=1/128, spheres r=0.12 at x=0.3/0.7 moving at +-3 m/s, N=2^20, dt=5e-5
 * ``slab_impact``        256^3, h=1/256, slab [0.2,0.8]x[0.5,0.7]x[0.2,0.8], box collider, sphere impactor
 * ``parameter_sweep``    64 scenes, log-uniform N in [4k, 256k], grids {32,48,64,96}^3, random materials
-The reference's own scenes (src/exe/*.cpp) draw positions with rand(); these generators replace them.
+The reference's own scene programs (src/exe/*.cpp) are available unchanged through ``reference``:
+it builds them with the C++ scene builders of include/snow_scenes.hpp (the same particle sets as
+the reference's rand()-based samplers) by way of the driver snow_amd/snow_run.
 """
 from __future__ import annotations
 
 import math
+import os
+import subprocess
+import tempfile
 
 import numpy as np
 
@@ -192,3 +197,24 @@ def large_batch(num_small=512, seed=5000, nmin=1024, nmax=65536, dims_choices=(1
     big.dt = 1.25e-5
     return small + [big]
 
+
+
+REFERENCE_SCENES = ("snowBallDrop", "snowBallSmash", "snowBallsCrash", "snowSingleParticle", "snowSquare",
+                    "snowPlowTrack", "snowPlowField", "snowBunnyScene")
+
+
+def reference(name: str, asset: str | None = None) -> PhysicalScene:
+    """The scene of the reference's program src/exe/<name>.cpp with the grid, boundary half-planes,
+    material and step size that launchSnow adds (explicit.cpp:2-67). Built by the C++ scene builders
+    (include/snow_scenes.hpp) through ``snow_run --scene NAME --dump-scene FILE``. ``asset`` is the
+    particle file of snowBunnyScene ("x,y,z" lines; the reference's model/bunny.voxel is not shipped)."""
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "snow_run")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is not built (python -m snow_amd.build)")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, name + ".scn")
+        cmd = [exe, "--scene", name] + (["--asset", asset] if asset else []) + ["--dump-scene", path]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise ValueError(res.stderr.strip() or f"snow_run failed ({res.returncode})")
+        return PhysicalScene.load(path)

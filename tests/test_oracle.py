@@ -301,6 +301,32 @@ def test_grid_update_free_fall(oracle_mod):
     np.testing.assert_allclose(o.gvstar, vel + float(dt) * g, rtol=1e-12, atol=1e-12)
 
 
+def test_grid_update_node_positions_are_reference_floats(oracle_mod):
+    """Node coordinates are the host-computed floats of grid.cpp:98-102 (x_off + h * (float)j, two
+    roundings). With the reference grid (origin y 0.5125, h 0.05) the nodes j = 4 lie exactly on the
+    ground half-plane at float(0.5125 + 4 * 0.05) in float arithmetic, though 1.5e-8 above it in
+    double, so they collide (stick: no tangential velocity) and j = 5 does not."""
+    f = np.float32
+    plane_y = f(0.5125 + 4 * 0.05)
+    assert f(0.5125) + f(0.05) * f(4) == plane_y
+    assert float(f(0.5125)) + float(f(0.05)) * 4 > float(plane_y)
+    sc = scenes.tiny_block(n_side=2, dims=12)
+    sc.grid = Grid(12, 12, 12, 0.05, 0.5125, 0.5125, 1.5125)
+    sc.colliders = CollisionObjects([Collider((0.0, float(plane_y), 0.0), HALF_PLANE, normal=(0.0, 1.0, 0.0),
+                                              friction=0.2)])
+    o = oracle_mod.Oracle(sc)
+    o.gm[:] = 1.0
+    o.gmv[:] = 0.0
+    o.gmv[:, 1] = -1.0
+    o.gf[:] = 0.0
+    dt = f(1e-3)
+    o.grid_update(dt)
+    vy = o.gvstar[:, 1].reshape(12, 12, 12)  # [k][j][i]
+    assert np.all(o.gvstar.reshape(12, 12, 12, 3)[:, 4, :, :] == 0.0)
+    assert np.all(vy[:, :4, :] == 0.0)
+    np.testing.assert_allclose(vy[:, 5:, :], -1.0 + float(dt) * float(f(-9.81)), rtol=1e-12)
+
+
 def test_grid_update_zero_mass_nodes(oracle_mod):
     sc = small_scene()
     o = oracle_mod.Oracle(sc)
