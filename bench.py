@@ -230,6 +230,27 @@ def large_batch(steps: int) -> dict:
     return out
 
 
+def reference_scenes(steps: int) -> dict:
+    """The reference's own scene programs (src/exe/*.cpp, include/snow_scenes.hpp) on the reference
+    grid (201^3, h = 0.05, dt = 1e-3), each stepped alone: particle-steps/s and ms per step. The bunny
+    scene is left out (its particle file is not in the reference repository)."""
+    from snow_amd import SnowContext, scenes
+    out = {"unit": "particle-steps/s", "steps": steps, "grid": [201, 201, 201]}
+    for name in scenes.REFERENCE_SCENES:
+        if name == "snowBunnyScene":
+            continue
+        sc = scenes.reference(name)
+        n = len(sc.particles)
+        with SnowContext(sc) as ctx:
+            ctx.init()
+            for _ in range(5):
+                ctx.step(sc.dt)
+            el = timed_steps(ctx, steps, dt=sc.dt)
+        out[name] = {"particles": n, "value": n * steps / el, "ms_per_step": el / steps * 1e3}
+        print(f"[bench] reference scene {name} done", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(mib: int) -> None:
     """One device-to-device hipMemcpy of `mib` MiB (HIP's copy kernel): the reference point for the
     FETCH_SIZE / WRITE_SIZE counters of the traffic measurement."""
@@ -264,6 +285,8 @@ def main():
     ap.add_argument("--stress-steps", type=int, default=10)
     ap.add_argument("--large", action="store_true", help="also run the 512+1-scene batch (2^25 + ~8M particles)")
     ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--reference-scenes", action="store_true",
+                    help="also time the reference's own scene programs (src/exe/*.cpp) on the 201^3 grid")
     ap.add_argument("--only-sweep", action="store_true", help="run only the batched 64-scene sweep")
     ap.add_argument("--error-steps", type=int, default=10)
     ap.add_argument("--sweep-steps", type=int, default=100)
@@ -374,6 +397,8 @@ def main():
         print("[bench] stress slab done", file=sys.stderr, flush=True)
     if args.large:
         out["forms"]["large_batch"] = large_batch(args.large_steps)
+    if args.reference_scenes:
+        out["forms"]["reference_scenes"] = reference_scenes(100)
         print("[bench] large batch done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)

@@ -134,14 +134,16 @@ int snow_step_batch(snow_ctx* ctx);
  * buffer parity, captured on first use). No reference counterpart: the reference dispatches
  * each compute shader from the host every step (explicittimeupdate.cpp:36-63). */
 int snow_run_graph(snow_ctx* ctx, int32_t steps);
-/* Time the G2P workgroup size (the launch shape that does not change the results), `steps`
- * steps per candidate on a snapshot of the current state, keep the fastest and restore the state.
- * *p2g_chunk receives the P2G chunk size (particles) of scene 0 and *grid_blocks the number of
- * P2G / grid-merge workgroups; both follow from the particle counts at creation (not tuned: the
- * P2G rounding depends on the chunking). Any of the out pointers may be NULL. No reference
- * counterpart (the reference's work-group sizes are fixed at 1024, defines.hpp:28). */
+/* Time the launch shapes that do not change the results, `steps` steps per candidate on a
+ * snapshot of the current state, keep the fastest and restore the state: the G2P workgroup size
+ * (64, 128, 256; *g2p_block) and the sort-bin width, the largest radix digit of the stable sort
+ * (8..11 bits; *sort_bits). *p2g_chunk receives the P2G chunk size (particles) of scene 0 and
+ * *grid_blocks the number of P2G / grid-merge workgroups; both follow from the particle counts at
+ * creation (not tuned: the P2G rounding depends on the chunking). Any of the out pointers may be
+ * NULL. No reference counterpart (the reference's work-group sizes are fixed at 1024,
+ * defines.hpp:28). */
 int snow_autotune(snow_ctx* ctx, int32_t steps, int32_t* g2p_block, int32_t* p2g_chunk,
-                  int32_t* grid_blocks);
+                  int32_t* grid_blocks, int32_t* sort_bits);
 
 /* ---- per-stage entry points (the Technique::dispatch of each compute shader) ---- */
 int snow_stage_grid_mass(snow_ctx* ctx);        /* gMassCompute.glsl:33-84 (init)      */

@@ -333,9 +333,9 @@ class SnowContext:
         self._chk(self.lib.snow_run_graph(self.h, steps))
 
     def autotune(self, steps: int = 5) -> dict:
-        g, p, q = C.c_int32(), C.c_int32(), C.c_int32()
-        self._chk(self.lib.snow_autotune(self.h, steps, C.byref(g), C.byref(p), C.byref(q)))
-        return {"g2p_block": g.value, "p2g_chunk": p.value, "grid_blocks": q.value}
+        g, p, q, b = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.snow_autotune(self.h, steps, C.byref(g), C.byref(p), C.byref(q), C.byref(b)))
+        return {"g2p_block": g.value, "p2g_chunk": p.value, "grid_blocks": q.value, "sort_bits": b.value}
 
     def sync(self):
         self._chk(self.lib.snow_sync(self.h))

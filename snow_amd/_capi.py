@@ -72,7 +72,7 @@ SIGNATURES = [
     ("snow_step", C.c_int, [C.c_void_p, C.c_double]),
     ("snow_step_batch", C.c_int, [C.c_void_p]),
     ("snow_run_graph", C.c_int, [C.c_void_p, C.c_int32]),
-    ("snow_autotune", C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, i32p]),
+    ("snow_autotune", C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, i32p, i32p]),
     ("snow_stage_grid_mass", C.c_int, [C.c_void_p]),
     ("snow_stage_particle_volume", C.c_int, [C.c_void_p]),
     ("snow_stage_rigid_advance", C.c_int, [C.c_void_p, C.c_double]),

@@ -19,7 +19,7 @@
 
 namespace snow {
 hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                            const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit,
+                            const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit, int digit_bits,
                             hipStream_t stream);
 }
 
@@ -71,6 +71,7 @@ struct snow_ctx {
   int64_t total_nodes = 0;
   uint32_t total_ctiles = 0, total_ntiles = 0;
   int key_bits = 1;
+  int sort_bits = 11;  // largest radix digit width (sort-bin width), tuned by snow_autotune
 
   SceneDev* d_scenes = nullptr;
   uint16_t *d_pscene = nullptr, *d_tile_scene = nullptr, *d_ntile_scene = nullptr;
@@ -323,7 +324,7 @@ int run_sort(snow_ctx* c, float dt_rigid) {
   if (c->N) {
     size_t tb = c->sort_tmp_bytes;
     HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->N,
-                               c->key_bits, c->stream));
+                               c->key_bits, c->sort_bits, c->stream));
   }
   return tm.done();
 }
@@ -591,7 +592,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   {
     size_t tb = 0;
     CREATECHK(radix_sort_pairs(nullptr, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1],
-                               std::max<uint32_t>(N, 1), c->key_bits, c->stream));
+                               std::max<uint32_t>(N, 1), c->key_bits, 11, c->stream));
     c->sort_tmp_bytes = tb;
     CREATECHK(hipMalloc(&c->d_sort_tmp, std::max<size_t>(tb, 16)));
   }
@@ -822,7 +823,8 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
   return 0;
 }
 
-int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_chunk, int32_t* grid_blocks) {
+int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_chunk, int32_t* grid_blocks,
+                  int32_t* sort_bits) {
   if (!c || steps <= 0) return SNOW_E_ARG;
   if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before autotuning");
   for (const auto& S : c->scenes)
@@ -887,6 +889,17 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
     if (t < best) { best = t; best_g = g; }
   }
   c->g2p_block = best_g;
+  // the sort-bin width: the largest radix digit (the stable sort's result does not depend on it)
+  const int sort_cands[] = {8, 9, 10, 11};
+  int best_s = c->sort_bits;
+  best = 1e30;
+  for (int sb : sort_cands) {
+    if (rc) break;
+    c->sort_bits = sb;
+    if ((rc = trial(t))) break;
+    if (t < best) { best = t; best_s = sb; }
+  }
+  c->sort_bits = best_s;
   c->timers = timers;
   const int rr = restore();
   free_all();
@@ -902,6 +915,7 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
   if (g2p_block) *g2p_block = c->g2p_block;
   if (p2g_chunk) *p2g_chunk = c->scenes.empty() ? 0 : int32_t(c->scenes[0].seg);
   if (grid_blocks) *grid_blocks = int32_t(c->total_chunks);
+  if (sort_bits) *sort_bits = c->sort_bits;
   return 0;
 }
 

@@ -210,10 +210,10 @@ SortLayout layout(uint32_t n) {
 }  // namespace
 
 // Stable sort of (keys_in, vals_in) by the low end_bit bits of the key into (keys_out,
-// vals_out). With temp == nullptr, returns the temporary storage size in *temp_bytes.
-// Arrays must be 16-byte aligned.
+// vals_out), in passes of at most digit_bits (4..11) bits. With temp == nullptr, returns the
+// temporary storage size in *temp_bytes. Arrays must be 16-byte aligned.
 hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                            const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit,
+                            const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit, int digit_bits,
                             hipStream_t stream) {
   const SortLayout L = layout(n);
   if (!temp) {
@@ -228,7 +228,8 @@ hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys
   uint32_t* ktmp = reinterpret_cast<uint32_t*>(base + L.ktmp_off);
   uint32_t* vtmp = reinterpret_cast<uint32_t*>(base + L.vtmp_off);
   const int eb = end_bit < 1 ? 1 : (end_bit > 32 ? 32 : end_bit);
-  const int passes = (eb + MAX_BITS - 1) / MAX_BITS;
+  const int db = digit_bits < 4 ? 4 : (digit_bits > MAX_BITS ? MAX_BITS : digit_bits);
+  const int passes = (eb + db - 1) / db;
   const int dbits = (eb + passes - 1) / passes;
   // choose the first destination so that the last pass writes keys_out
   const uint32_t* kin = keys_in;

@@ -362,6 +362,7 @@ def test_autotune_keeps_state_and_results():
         before = b.read_particles(2)
         shape = b.autotune(steps=2)
         assert shape["g2p_block"] in (64, 128, 256)
+        assert shape["sort_bits"] in (8, 9, 10, 11)  # a different radix digit must not change the bits below
         after = b.read_particles(2)
         for k in ("x", "v", "fe", "fp"):
             assert np.array_equal(before[k], after[k]), k
