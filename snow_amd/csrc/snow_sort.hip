@@ -1,18 +1,22 @@
 // snow_sort.hip - stable LSD radix sort of (cell key, slot) pairs for gfx950.
 //
 // The key bits are split into the fewest passes of at most 11 bits (22-bit keys: two passes).
-// Each pass runs three kernels over tiles of 4096 keys:
+// Each pass runs three kernels over tiles of 4096 keys, one 16-wave workgroup per tile (the
+// ranking is a chain of dependent ballots and LDS round trips, so a tile is spread over a CU's
+// worth of waves rather than a few waves with long chains):
 //   k_radix_hist     per-tile digit histogram, written tile-major (hist[tile][d], one coalesced
-//                    row per tile); each thread counts 16 consecutive keys and adds runs of equal
+//                    row per tile); each thread counts 4 consecutive keys and adds runs of equal
 //                    digits with one LDS atomic (nearly sorted input makes long runs)
 //   k_radix_colscan  64 digits per workgroup: exclusive scan of each digit's column over the
 //                    tiles (16 groups of tiles per digit, rows read as coalesced 256-byte
-//                    segments), and the digit totals
-//   k_radix_scatter  each wave owns 1024 consecutive keys of the tile and ranks them in input
+//                    segments, up to 16 tiles per group kept in registers between the sum and
+//                    the scan), and the digit totals
+//   k_radix_scatter  each wave owns 256 consecutive keys of the tile and ranks them in input
 //                    order (keys with equal digits in a round of 64 are matched with ballots,
-//                    running per-wave digit counters in LDS carry the rank across rounds); the
-//                    wave bases follow from the per-wave counts, the tile's row offsets and the
-//                    exclusive scan of the digit totals, which every tile recomputes.
+//                    running per-wave digit counters in LDS, 16 x 2048 words, carry the rank
+//                    across rounds); the wave bases follow from the per-wave counts, the tile's
+//                    row offsets and the exclusive scan of the digit totals, which every tile
+//                    recomputes.
 // Stability makes the result a deterministic function of the input order. No hipMemset and no
 // host synchronisation: the pass sequence can be captured in a HIP graph.
 #include <hip/hip_runtime.h>
@@ -23,9 +27,9 @@ namespace snow {
 namespace {
 constexpr int MAX_BITS = 11;
 constexpr int MAX_BINS = 1 << MAX_BITS;
-constexpr int SORT_THREADS = 256;
+constexpr int SORT_THREADS = 1024;                    // 16 waves per tile: one CU's worth
 constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr int SORT_ITEMS = 16;
+constexpr int SORT_ITEMS = 4;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // keys per tile
 constexpr int WAVE_KEYS = SORT_TILE / SORT_WAVES;      // consecutive keys per wave
 
@@ -74,6 +78,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_hist(const uint32_t* __r
 // totals[d]. Workgroup g: digits [64g, 64g+64) (lane = digit), COL_PARTS groups of tiles (wave =
 // group).
 constexpr int COL_PARTS = 16;
+constexpr int COL_CACHE = 16;
 __global__ void __launch_bounds__(64 * COL_PARTS) k_radix_colscan(uint32_t* __restrict__ hist, uint32_t ntiles,
                                                                   int bins, uint32_t* __restrict__ totals) {
   __shared__ uint32_t s_part[COL_PARTS][64];
@@ -82,8 +87,17 @@ __global__ void __launch_bounds__(64 * COL_PARTS) k_radix_colscan(uint32_t* __re
   const bool on = d < bins;
   const uint32_t q = (ntiles + COL_PARTS - 1) / COL_PARTS;
   const uint32_t b0 = min(ntiles, uint32_t(part) * q), b1 = min(ntiles, b0 + q);
+  // up to COL_CACHE tiles per group stay in registers between the sum and the scan
+  uint32_t cache[COL_CACHE];
+  const bool cached = b1 - b0 <= uint32_t(COL_CACHE);
   uint32_t sum = 0u;
-  if (on) {
+  if (on && cached) {
+#pragma unroll
+    for (int j = 0; j < COL_CACHE; ++j) {
+      cache[j] = b0 + j < b1 ? hist[size_t(b0 + j) * bins + d] : 0u;
+      sum += cache[j];
+    }
+  } else if (on) {
 #pragma unroll 8
     for (uint32_t b = b0; b < b1; ++b) sum += hist[size_t(b) * bins + d];
   }
@@ -96,7 +110,14 @@ __global__ void __launch_bounds__(64 * COL_PARTS) k_radix_colscan(uint32_t* __re
     run += p < part ? v : 0u;
     total += v;
   }
-  if (on) {
+  if (on && cached) {
+#pragma unroll
+    for (int j = 0; j < COL_CACHE; ++j) {
+      if (b0 + j < b1) hist[size_t(b0 + j) * bins + d] = run;
+      run += cache[j];
+    }
+    if (part == 0) totals[d] = total;
+  } else if (on) {
 #pragma unroll 8
     for (uint32_t b = b0; b < b1; ++b) {
       const size_t idx = size_t(b) * bins + d;

@@ -454,3 +454,57 @@ def test_positions_view_is_the_particle_state(c1_scene):
     order_g = np.lexsort(got.T[::-1])
     order_w = np.lexsort(want.T[::-1])
     assert np.array_equal(got[order_g], want[order_w])
+
+
+def _storage_ids(ctx, x_creation):
+    """Creation index of every particle in the device's storage order, matched by exact position
+    (the zero-copy view holds x, y, z, m in storage order)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    ctx.sync()
+    ptr, stride, n = ctx.positions_view(0)
+    raw = np.empty((n, stride // 4), np.float32)
+    assert hip.hipMemcpy(raw.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(n * stride), 2) == 0
+    xs = np.ascontiguousarray(raw[:, :3]).view(np.uint32)
+    xc = np.ascontiguousarray(x_creation, np.float32).view(np.uint32)
+    ks = np.lexsort(xs.T[::-1])
+    kc = np.lexsort(xc.T[::-1])
+    assert np.array_equal(xs[ks], xc[kc])  # the same set of positions
+    assert not (xc[kc][1:] == xc[kc][:-1]).all(axis=1).any()  # all distinct, so the match is unique
+    ids = np.empty(n, np.int64)
+    ids[ks] = kc
+    return ids
+
+
+@pytest.mark.parametrize("name", ["snowball_drop", "two_snowballs"])
+def test_storage_order_is_stable_cell_sort(name):
+    """A step sorts the particles by the cell key of their positions at the start of the step (cell
+    tiles of 4^3, tile-major, cells inside a tile x-fastest), stably: particles of one cell keep the
+    order they had. Checked on the storage order before and after one step, for every particle
+    whose cell is unambiguous (u = (x - origin)/h further than 1e-4 from an integer)."""
+    sc = getattr(scenes, name)()
+    with SnowContext(sc) as ctx:
+        ctx.init()
+        for _ in range(2):
+            ctx.step(sc.dt)
+        x0 = ctx.read_particles()["x"]
+        before = _storage_ids(ctx, x0)
+        ctx.step(sc.dt)
+        after = _storage_ids(ctx, ctx.read_particles()["x"])
+    g = sc.grid
+    origin = np.array([g.x_off, g.y_off, g.z_off], np.float64)
+    u = (x0.astype(np.float64) - origin) / float(g.h)
+    sure = (np.abs(u - np.rint(u)) > 1e-4).all(axis=1)
+    c = np.floor(u).astype(np.int64) + 2
+    ct = (np.array(g.dims) + 3 + 3) // 4
+    assert (c >= 0).all() and (c < 4 * ct).all()  # every particle inside the stencil range here
+    tile = (c[:, 0] >> 2) + ct[0] * ((c[:, 1] >> 2) + ct[1] * (c[:, 2] >> 2))
+    key = tile * 64 + (c[:, 0] & 3) + 4 * (c[:, 1] & 3) + 16 * (c[:, 2] & 3)
+    rank_before = np.empty(len(before), np.int64)
+    rank_before[before] = np.arange(len(before))
+    seq = after[sure[after]]  # storage order after the step, unambiguous particles only
+    k, r = key[seq], rank_before[seq]
+    print(f"\n[{name}] {len(seq)} of {len(after)} particles checked, {len(np.unique(k))} cells", flush=True)
+    assert (np.diff(k) >= 0).all()
+    same = np.diff(k) == 0
+    assert (np.diff(r)[same] > 0).all()
