@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
 ]
-SOURCES = ["snow_capi.hip", "snow_sort.hip"]
+SOURCES = ["snow_capi.hip"]
 
 
 def _hipcc() -> str:

@@ -16,12 +16,7 @@
 
 #include "../../include/snow.h"
 #include "snow_kernels.hpp"
-
-namespace snow {
-hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                            const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit, int digit_bits,
-                            hipStream_t stream);
-}
+#include "snow_sort.hpp"
 
 using namespace snow;
 
@@ -313,18 +308,21 @@ int run_force_prepass(snow_ctx* c) {
   return 0;
 }
 
-// Step start (stamp, counters, rigid advance when dt_rigid >= 0) + sort + cell bounds.
+// Step start (stamp, counters, rigid advance when dt_rigid >= 0) + sort. The step start rides on
+// the sort's first launch; it runs on its own only when there is nothing to sort.
 int run_sort(snow_ctx* c, float dt_rigid) {
-  k_step_begin<<<1, 256, 0, c->stream>>>(c->d_ctl, c->d_counts, c->nscenes + 1, c->d_scenes, c->nscenes, c->d_cols,
-                                         dt_rigid, c->d_list_ctr);
-  LAUNCHCHK(c);
+  const StepBegin begin{c->d_ctl, c->d_counts, c->nscenes + 1, c->nscenes, c->d_scenes, c->d_cols, c->d_list_ctr,
+                        dt_rigid};
   int rc = run_keys(c);
   if (rc) return rc;
   StageTimer tm(c, ST_SORT);
   if (c->N) {
     size_t tb = c->sort_tmp_bytes;
     HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->N,
-                               c->key_bits, c->sort_bits, c->stream));
+                               c->key_bits, c->sort_bits, c->stream, &begin));
+  } else {
+    k_step_begin<<<1, 256, 0, c->stream>>>(begin);
+    LAUNCHCHK(c);
   }
   return tm.done();
 }

@@ -131,7 +131,7 @@ __device__ __forceinline__ void store_node(float2* __restrict__ g, float* __rest
 // Per-step control block in device memory, so every launch of a step has fixed arguments
 // (graph-capturable) while the set of active tiles changes from step to step.
 struct StepCtl {
-  uint32_t step;      // step stamp, incremented by k_step_begin
+  uint32_t step;      // step stamp, incremented at the start of every step (step_begin_block)
   uint32_t n_active;  // unused
   uint32_t spare;
   uint32_t pad;
@@ -144,30 +144,41 @@ constexpr int TILE_LISTS = 64;
 constexpr int TILE_LIST_STRIDE = 32;
 
 // Start of a step: new stamp, cleared counters, and the collider advance of rigidAdvance.glsl:14-18
-// (cx += dt*cv) when dt_rigid >= 0 (0 selects each scene's own dt).
-__global__ void __launch_bounds__(256) k_step_begin(StepCtl* __restrict__ ctl, unsigned long long* __restrict__ counts,
-                                                    int ncounts, const SceneDev* __restrict__ scenes, int nscenes,
-                                                    Collider* __restrict__ cols, float dt_rigid,
-                                                    uint32_t* __restrict__ list_ctr) {
-  const int t = threadIdx.x;
-  if (t < TILE_LISTS) list_ctr[t * TILE_LIST_STRIDE] = 0u;
+// (cx += dt*cv) when dt_rigid >= 0 (0 selects each scene's own dt). Nothing the sort reads is
+// touched, so the first sort launch carries this work in its workgroup 0 (k_radix_hist); the
+// kernel k_step_begin runs it alone when there is nothing to sort.
+struct StepBegin {
+  StepCtl* ctl;
+  unsigned long long* counts;
+  int ncounts;
+  int nscenes;
+  const SceneDev* scenes;
+  Collider* cols;
+  uint32_t* list_ctr;
+  float dt_rigid;
+};
+
+__device__ __forceinline__ void step_begin_block(const StepBegin& a, int t, int nt) {
+  if (t < TILE_LISTS) a.list_ctr[t * TILE_LIST_STRIDE] = 0u;
   if (t == 0) {
-    ctl->step += 1u;
-    ctl->n_active = 0u;
-    ctl->spare = 0u;
+    a.ctl->step += 1u;
+    a.ctl->n_active = 0u;
+    a.ctl->spare = 0u;
   }
-  for (int i = t; i < ncounts; i += blockDim.x) counts[i] = 0ull;
-  if (dt_rigid >= 0.0f) {
-    for (int s = t; s < nscenes; s += blockDim.x) {
-      const SceneDev& S = scenes[s];
-      const float dt = dt_rigid > 0.0f ? dt_rigid : S.dt;
+  for (int i = t; i < a.ncounts; i += nt) a.counts[i] = 0ull;
+  if (a.dt_rigid >= 0.0f) {
+    for (int s = t; s < a.nscenes; s += nt) {
+      const SceneDev& S = a.scenes[s];
+      const float dt = a.dt_rigid > 0.0f ? a.dt_rigid : S.dt;
       for (int c = 0; c < S.ncol; ++c) {
-        Collider& C = cols[S.col_off + c];
+        Collider& C = a.cols[S.col_off + c];
         C.px += dt * C.vx; C.py += dt * C.vy; C.pz += dt * C.vz;
       }
     }
   }
 }
+
+__global__ void __launch_bounds__(256) k_step_begin(StepBegin a) { step_begin_block(a, threadIdx.x, blockDim.x); }
 
 // Collider advance alone (rigidAdvance.glsl:14-18) for the per-stage API.
 __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restrict__ scenes, int nscenes,

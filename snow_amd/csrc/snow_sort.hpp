@@ -1,4 +1,4 @@
-// snow_sort.hip - stable LSD radix sort of (cell key, slot) pairs for gfx950.
+// snow_sort.hpp - stable LSD radix sort of (cell key, slot) pairs for gfx950.
 //
 // The key bits are split into the fewest passes of at most 11 bits (22-bit keys: two passes).
 // Each pass runs three kernels over tiles of 4096 keys, one 16-wave workgroup per tile (the
@@ -18,9 +18,14 @@
 //                    row offsets and the exclusive scan of the digit totals, which every tile
 //                    recomputes.
 // Stability makes the result a deterministic function of the input order. No hipMemset and no
-// host synchronisation: the pass sequence can be captured in a HIP graph.
+// host synchronisation: the pass sequence can be captured in a HIP graph. The first histogram
+// launch can also carry the step's start (step_begin_block in its workgroup 0), which saves the
+// step a launch.
+#pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "snow_kernels.hpp"
 
 namespace snow {
 
@@ -35,9 +40,11 @@ constexpr int WAVE_KEYS = SORT_TILE / SORT_WAVES;      // consecutive keys per w
 
 __global__ void __launch_bounds__(SORT_THREADS) k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
                                                              int shift, int bits, uint32_t ntiles,
-                                                             uint32_t* __restrict__ hist) {
+                                                             uint32_t* __restrict__ hist, StepBegin begin,
+                                                             int with_begin) {
   __shared__ uint32_t h[MAX_BINS];
   const int t = threadIdx.x;
+  if (with_begin && blockIdx.x == 0) step_begin_block(begin, t, SORT_THREADS);
   const int bins = 1 << bits;
   const uint32_t mask = uint32_t(bins - 1);
   for (int d = t; d < bins; d += SORT_THREADS) h[d] = 0u;
@@ -133,13 +140,20 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, int shift, int bits, uint32_t ntiles,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ totals) {
+  constexpr int PER = MAX_BINS / SORT_THREADS;  // digits per thread for the digit bases
+  static_assert(MAX_BINS % SORT_THREADS == 0, "whole digits per thread");
   __shared__ uint32_t s_cnt[SORT_WAVES][MAX_BINS];
-  __shared__ uint32_t s_part[SORT_THREADS];
+  __shared__ uint32_t s_wsum[SORT_WAVES];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int bins = 1 << bits;
   const uint32_t mask = uint32_t(bins - 1);
-  for (int i = t; i < SORT_WAVES * MAX_BINS; i += SORT_THREADS) (&s_cnt[0][0])[i] = 0u;
+  uint32_t* cnt = s_cnt[wave];
+  {  // each wave clears its own counters (in-order LDS per wave: no workgroup barrier needed)
+    uint4* c4 = reinterpret_cast<uint4*>(cnt);
+#pragma unroll
+    for (int i = lane; i < MAX_BINS / 4; i += 64) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
   const uint32_t base = blockIdx.x * uint32_t(SORT_TILE) + uint32_t(wave) * WAVE_KEYS + uint32_t(lane);
   uint32_t kreg[SORT_ITEMS], vreg[SORT_ITEMS];
 #pragma unroll
@@ -148,11 +162,22 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
     kreg[r] = i < n ? kin[i] : 0u;
     vreg[r] = i < n ? vin[i] : 0u;
   }
-  __syncthreads();
+  // thread t owns digits [PER t, PER t + PER): their totals and this tile's row offsets load while
+  // the wave ranks its keys
+  const int d0 = t * PER;
+  uint32_t tot[PER], row[PER];
+  uint32_t part = 0u;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const bool on = d0 + j < bins;
+    tot[j] = on ? totals[d0 + j] : 0u;
+    row[j] = on ? offsets[size_t(blockIdx.x) * bins + d0 + j] : 0u;
+    part += tot[j];
+  }
+  __builtin_amdgcn_wave_barrier();
   // rank within the wave: running per-digit counters, one round of 64 keys at a time
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t loc[SORT_ITEMS];
-  uint32_t* cnt = s_cnt[wave];
 #pragma unroll
   for (int r = 0; r < SORT_ITEMS; ++r) {
     const bool valid = base + uint32_t(r * 64) < n;
@@ -169,27 +194,24 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
     if (valid && rank == 0u) cnt[d] = old + uint32_t(__popcll(peers));
     loc[r] = old + rank;
   }
-  __syncthreads();
-  // digit bases: exclusive scan of the totals, plus this tile's row offsets, plus earlier waves
-  const int per = (bins + SORT_THREADS - 1) / SORT_THREADS;
-  const int d0 = t * per;
-  uint32_t part = 0u;
-  for (int j = 0; j < per; ++j)
-    if (d0 + j < bins) part += totals[d0 + j];
-  s_part[t] = part;
-  __syncthreads();
-  for (int off = 1; off < SORT_THREADS; off <<= 1) {
-    const uint32_t y = t >= off ? s_part[t - off] : 0u;
-    __syncthreads();
-    s_part[t] += y;
-    __syncthreads();
+  // digit bases: exclusive scan of the totals (within the wave by shuffles, then over the wave
+  // sums), plus this tile's row offsets, plus the counts of earlier waves
+  uint32_t incl = part;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
   }
-  uint32_t run = s_part[t] - part;
-  for (int j = 0; j < per; ++j) {
+  if (lane == 63) s_wsum[wave] = incl;
+  __syncthreads();  // wave sums and every wave's counters complete
+  uint32_t run = incl - part;
+  for (int w = 0; w < wave; ++w) run += s_wsum[w];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
     const int d = d0 + j;
     if (d < bins) {
-      uint32_t acc = run + offsets[size_t(blockIdx.x) * bins + d];
-      run += totals[d];
+      uint32_t acc = run + row[j];
+      run += tot[j];
 #pragma unroll
       for (int w = 0; w < SORT_WAVES; ++w) {
         const uint32_t c = s_cnt[w][d];
@@ -232,10 +254,11 @@ SortLayout layout(uint32_t n) {
 
 // Stable sort of (keys_in, vals_in) by the low end_bit bits of the key into (keys_out,
 // vals_out), in passes of at most digit_bits (4..11) bits. With temp == nullptr, returns the
-// temporary storage size in *temp_bytes. Arrays must be 16-byte aligned.
-hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                            const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit, int digit_bits,
-                            hipStream_t stream) {
+// temporary storage size in *temp_bytes. Arrays must be 16-byte aligned. With begin != nullptr
+// (and n > 0) the first launch also runs the step start *begin.
+inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                                   const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit,
+                                   int digit_bits, hipStream_t stream, const StepBegin* begin = nullptr) {
   const SortLayout L = layout(n);
   if (!temp) {
     *temp_bytes = L.total;
@@ -262,7 +285,9 @@ hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys
     const int shift = p * dbits;
     const int bits = eb - shift < dbits ? eb - shift : dbits;
     const int bins = 1 << bits;
-    k_radix_hist<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, n, shift, bits, L.ntiles, hist);
+    const int with_begin = (p == 0 && begin) ? 1 : 0;
+    k_radix_hist<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, n, shift, bits, L.ntiles, hist,
+                                                        with_begin ? *begin : StepBegin{}, with_begin);
     k_radix_colscan<<<div_up(bins, 64), 64 * COL_PARTS, 0, stream>>>(hist, L.ntiles, bins, totals);
     k_radix_scatter<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, vin, ko, vo, n, shift, bits, L.ntiles, hist,
                                                            totals);
