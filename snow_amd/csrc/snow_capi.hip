@@ -319,7 +319,7 @@ int run_sort(snow_ctx* c, float dt_rigid) {
   if (c->N) {
     size_t tb = c->sort_tmp_bytes;
     HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->N,
-                               c->key_bits, c->sort_bits, c->stream, &begin));
+                               c->key_bits, c->sort_bits, c->stream, c->num_cu, &begin));
   } else {
     k_step_begin<<<1, 256, 0, c->stream>>>(begin);
     LAUNCHCHK(c);
@@ -590,7 +590,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   {
     size_t tb = 0;
     CREATECHK(radix_sort_pairs(nullptr, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1],
-                               std::max<uint32_t>(N, 1), c->key_bits, 11, c->stream));
+                               std::max<uint32_t>(N, 1), c->key_bits, 11, c->stream, c->num_cu));
     c->sort_tmp_bytes = tb;
     CREATECHK(hipMalloc(&c->d_sort_tmp, std::max<size_t>(tb, 16)));
   }

@@ -1,21 +1,21 @@
 // snow_sort.hpp - stable LSD radix sort of (cell key, slot) pairs for gfx950.
 //
 // The key bits are split into the fewest passes of at most 11 bits (22-bit keys: two passes).
-// Each pass runs three kernels over tiles of 4096 keys, one 16-wave workgroup per tile (the
-// ranking is a chain of dependent ballots and LDS round trips, so a tile is spread over a CU's
-// worth of waves rather than a few waves with long chains):
+// Each pass runs three kernels over tiles of 4096 keys, one workgroup per tile. The ranking is a
+// chain of dependent ballots and LDS round trips, so while all tiles fit one workgroup per CU a
+// tile gets 16 waves (short chains); with more tiles it gets 4, so that four tiles share a CU:
 //   k_radix_hist     per-tile digit histogram, written tile-major (hist[tile][d], one coalesced
-//                    row per tile); each thread counts 4 consecutive keys and adds runs of equal
-//                    digits with one LDS atomic (nearly sorted input makes long runs)
-//   k_radix_colscan  64 digits per workgroup: exclusive scan of each digit's column over the
-//                    tiles (16 groups of tiles per digit, rows read as coalesced 256-byte
-//                    segments, up to 16 tiles per group kept in registers between the sum and
+//                    row per tile); each thread counts 4 (or 16) consecutive keys and adds runs of
+//                    equal digits with one LDS atomic (nearly sorted input makes long runs)
+//   k_radix_colscan  64, 32 or 16 digits per workgroup: exclusive scan of each digit's column
+//                    over the tiles (16, 32 or 64 groups of tiles per digit, row segments read
+//                    coalesced, up to 16 tiles per group kept in registers between the sum and
 //                    the scan), and the digit totals
-//   k_radix_scatter  each wave owns 256 consecutive keys of the tile and ranks them in input
-//                    order (keys with equal digits in a round of 64 are matched with ballots,
-//                    running per-wave digit counters in LDS, 16 x 2048 words, carry the rank
-//                    across rounds); the wave bases follow from the per-wave counts, the tile's
-//                    row offsets and the exclusive scan of the digit totals, which every tile
+//   k_radix_scatter  each wave owns 256 (or 1024) consecutive keys of the tile and ranks them in
+//                    input order (keys with equal digits in a round of 64 are matched with
+//                    ballots, running per-wave digit counters in LDS carry the rank across
+//                    rounds); the wave bases follow from the per-wave counts, the tile's row
+//                    offsets and the exclusive scan of the digit totals, which every tile
 //                    recomputes.
 // Stability makes the result a deterministic function of the input order. No hipMemset and no
 // host synchronisation: the pass sequence can be captured in a HIP graph. The first histogram
@@ -32,40 +32,37 @@ namespace snow {
 namespace {
 constexpr int MAX_BITS = 11;
 constexpr int MAX_BINS = 1 << MAX_BITS;
-constexpr int SORT_THREADS = 1024;                    // 16 waves per tile: one CU's worth
-constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr int SORT_ITEMS = 4;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // keys per tile
-constexpr int WAVE_KEYS = SORT_TILE / SORT_WAVES;      // consecutive keys per wave
+constexpr int SORT_TILE = 4096;  // keys per tile
 
-__global__ void __launch_bounds__(SORT_THREADS) k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                             int shift, int bits, uint32_t ntiles,
-                                                             uint32_t* __restrict__ hist, StepBegin begin,
-                                                             int with_begin) {
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS) k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                        int bits, uint32_t* __restrict__ hist, StepBegin begin,
+                                                        int with_begin) {
+  constexpr int ITEMS = SORT_TILE / THREADS;  // consecutive keys per thread
   __shared__ uint32_t h[MAX_BINS];
   const int t = threadIdx.x;
-  if (with_begin && blockIdx.x == 0) step_begin_block(begin, t, SORT_THREADS);
+  if (with_begin && blockIdx.x == 0) step_begin_block(begin, t, THREADS);
   const int bins = 1 << bits;
   const uint32_t mask = uint32_t(bins - 1);
-  for (int d = t; d < bins; d += SORT_THREADS) h[d] = 0u;
+  for (int d = t; d < bins; d += THREADS) h[d] = 0u;
   __syncthreads();
-  const uint32_t i0 = blockIdx.x * uint32_t(SORT_TILE) + uint32_t(t) * SORT_ITEMS;
-  uint32_t k[SORT_ITEMS];
-  if (i0 + SORT_ITEMS <= n) {
+  const uint32_t i0 = blockIdx.x * uint32_t(SORT_TILE) + uint32_t(t) * ITEMS;
+  uint32_t k[ITEMS];
+  if (i0 + ITEMS <= n) {
     const uint4* p = reinterpret_cast<const uint4*>(keys + i0);
 #pragma unroll
-    for (int q = 0; q < SORT_ITEMS / 4; ++q) {
+    for (int q = 0; q < ITEMS / 4; ++q) {
       const uint4 v = p[q];
       k[4 * q] = v.x; k[4 * q + 1] = v.y; k[4 * q + 2] = v.z; k[4 * q + 3] = v.w;
     }
   } else {
 #pragma unroll
-    for (int r = 0; r < SORT_ITEMS; ++r) k[r] = i0 + r < n ? keys[i0 + r] : 0u;
+    for (int r = 0; r < ITEMS; ++r) k[r] = i0 + r < n ? keys[i0 + r] : 0u;
   }
-  const int cnt_valid = i0 >= n ? 0 : int(min(uint32_t(SORT_ITEMS), n - i0));
+  const int cnt_valid = i0 >= n ? 0 : int(min(uint32_t(ITEMS), n - i0));
   uint32_t cur = (k[0] >> shift) & mask, run = 0u;
 #pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     if (r < cnt_valid) {
       const uint32_t d = (k[r] >> shift) & mask;
       if (d != cur) {
@@ -78,23 +75,25 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_hist(const uint32_t* __r
   }
   if (run) atomicAdd(&h[cur], run);
   __syncthreads();
-  for (int d = t; d < bins; d += SORT_THREADS) hist[size_t(blockIdx.x) * bins + d] = h[d];
+  for (int d = t; d < bins; d += THREADS) hist[size_t(blockIdx.x) * bins + d] = h[d];
 }
 
 // Exclusive scan of each digit column of the tile-major histogram, in place; column totals to
-// totals[d]. Workgroup g: digits [64g, 64g+64) (lane = digit), COL_PARTS groups of tiles (wave =
-// group).
-constexpr int COL_PARTS = 16;
+// totals[d]. Workgroup g (1024 threads): digits [DG g, DG g + DG) (thread t: digit t % DG), 1024/DG
+// groups of consecutive tiles (group t / DG). Fewer digits per workgroup for more tiles keeps a
+// group's tiles within the COL_CACHE registers that hold them between the sum and the scan.
+constexpr int COL_THREADS = 1024;
 constexpr int COL_CACHE = 16;
-__global__ void __launch_bounds__(64 * COL_PARTS) k_radix_colscan(uint32_t* __restrict__ hist, uint32_t ntiles,
-                                                                  int bins, uint32_t* __restrict__ totals) {
-  __shared__ uint32_t s_part[COL_PARTS][64];
-  const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
-  const int d = int(blockIdx.x) * 64 + lane;
+template <int DG>
+__global__ void __launch_bounds__(COL_THREADS) k_radix_colscan(uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                               int bins, uint32_t* __restrict__ totals) {
+  constexpr int PARTS = COL_THREADS / DG;
+  __shared__ uint32_t s_part[PARTS][DG];
+  const int lane = threadIdx.x % DG, part = threadIdx.x / DG;
+  const int d = int(blockIdx.x) * DG + lane;
   const bool on = d < bins;
-  const uint32_t q = (ntiles + COL_PARTS - 1) / COL_PARTS;
+  const uint32_t q = (ntiles + PARTS - 1) / PARTS;
   const uint32_t b0 = min(ntiles, uint32_t(part) * q), b1 = min(ntiles, b0 + q);
-  // up to COL_CACHE tiles per group stay in registers between the sum and the scan
   uint32_t cache[COL_CACHE];
   const bool cached = b1 - b0 <= uint32_t(COL_CACHE);
   uint32_t sum = 0u;
@@ -111,8 +110,8 @@ __global__ void __launch_bounds__(64 * COL_PARTS) k_radix_colscan(uint32_t* __re
   s_part[part][lane] = sum;
   __syncthreads();
   uint32_t run = 0u, total = 0u;
-#pragma unroll
-  for (int p = 0; p < COL_PARTS; ++p) {
+#pragma unroll 16
+  for (int p = 0; p < PARTS; ++p) {
     const uint32_t v = s_part[p][lane];
     run += p < part ? v : 0u;
     total += v;
@@ -136,28 +135,34 @@ __global__ void __launch_bounds__(64 * COL_PARTS) k_radix_colscan(uint32_t* __re
   }
 }
 
-__global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
+// WAVES waves rank a tile, each WKEYS = SORT_TILE / WAVES consecutive keys. The per-wave digit
+// counters take WAVES x bins words of dynamic LDS (plus WAVES words of wave sums): 128 KB for
+// 16 waves and 11-bit digits (one workgroup per CU), 32 KB for 4 waves (four per CU).
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, uint32_t n, int shift, int bits, uint32_t ntiles,
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ totals) {
-  constexpr int PER = MAX_BINS / SORT_THREADS;  // digits per thread for the digit bases
-  static_assert(MAX_BINS % SORT_THREADS == 0, "whole digits per thread");
-  __shared__ uint32_t s_cnt[SORT_WAVES][MAX_BINS];
-  __shared__ uint32_t s_wsum[SORT_WAVES];
+    uint32_t* __restrict__ vout, uint32_t n, int shift, int bits, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ totals) {
+  constexpr int THREADS = 64 * WAVES;
+  constexpr int ITEMS = SORT_TILE / THREADS;      // rounds of 64 keys per wave
+  constexpr int WKEYS = SORT_TILE / WAVES;
+  constexpr int PER = MAX_BINS / THREADS;          // digits per thread for the digit bases
+  static_assert(MAX_BINS % THREADS == 0 && SORT_TILE % THREADS == 0, "whole digits and keys per thread");
+  extern __shared__ uint32_t s_dyn[];
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int bins = 1 << bits;
   const uint32_t mask = uint32_t(bins - 1);
-  uint32_t* cnt = s_cnt[wave];
+  uint32_t* s_wsum = s_dyn + WAVES * bins;
+  uint32_t* cnt = s_dyn + wave * bins;
   {  // each wave clears its own counters (in-order LDS per wave: no workgroup barrier needed)
     uint4* c4 = reinterpret_cast<uint4*>(cnt);
-#pragma unroll
-    for (int i = lane; i < MAX_BINS / 4; i += 64) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int i = lane; i < bins / 4; i += 64) c4[i] = make_uint4(0u, 0u, 0u, 0u);
   }
-  const uint32_t base = blockIdx.x * uint32_t(SORT_TILE) + uint32_t(wave) * WAVE_KEYS + uint32_t(lane);
-  uint32_t kreg[SORT_ITEMS], vreg[SORT_ITEMS];
+  const uint32_t base = blockIdx.x * uint32_t(SORT_TILE) + uint32_t(wave) * WKEYS + uint32_t(lane);
+  uint32_t kreg[ITEMS], vreg[ITEMS];
 #pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const uint32_t i = base + uint32_t(r * 64);
     kreg[r] = i < n ? kin[i] : 0u;
     vreg[r] = i < n ? vin[i] : 0u;
@@ -177,9 +182,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
   __builtin_amdgcn_wave_barrier();
   // rank within the wave: running per-digit counters, one round of 64 keys at a time
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t loc[SORT_ITEMS];
+  uint32_t loc[ITEMS];
 #pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const bool valid = base + uint32_t(r * 64) < n;
     const uint32_t d = (kreg[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
@@ -213,16 +218,16 @@ __global__ void __launch_bounds__(SORT_THREADS) k_radix_scatter(
       uint32_t acc = run + row[j];
       run += tot[j];
 #pragma unroll
-      for (int w = 0; w < SORT_WAVES; ++w) {
-        const uint32_t c = s_cnt[w][d];
-        s_cnt[w][d] = acc;
+      for (int w = 0; w < WAVES; ++w) {
+        const uint32_t c = s_dyn[w * bins + d];
+        s_dyn[w * bins + d] = acc;
         acc += c;
       }
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < SORT_ITEMS; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     if (base + uint32_t(r * 64) < n) {
       const uint32_t d = (kreg[r] >> shift) & mask;
       const uint32_t pos = cnt[d] + loc[r];
@@ -258,7 +263,8 @@ SortLayout layout(uint32_t n) {
 // (and n > 0) the first launch also runs the step start *begin.
 inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
                                    const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit,
-                                   int digit_bits, hipStream_t stream, const StepBegin* begin = nullptr) {
+                                   int digit_bits, hipStream_t stream, int num_cu,
+                                   const StepBegin* begin = nullptr) {
   const SortLayout L = layout(n);
   if (!temp) {
     *temp_bytes = L.total;
@@ -275,6 +281,9 @@ inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_
   const int db = digit_bits < 4 ? 4 : (digit_bits > MAX_BITS ? MAX_BITS : digit_bits);
   const int passes = (eb + db - 1) / db;
   const int dbits = (eb + passes - 1) / passes;
+  // 16 ranking waves per tile while all tiles fit one workgroup per CU; beyond that 4 waves per
+  // tile, so that four tiles share a CU and no second round of workgroups is needed
+  const bool wide = L.ntiles <= uint32_t(num_cu > 0 ? num_cu : 1);
   // choose the first destination so that the last pass writes keys_out
   const uint32_t* kin = keys_in;
   const uint32_t* vin = vals_in;
@@ -286,11 +295,23 @@ inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_
     const int bits = eb - shift < dbits ? eb - shift : dbits;
     const int bins = 1 << bits;
     const int with_begin = (p == 0 && begin) ? 1 : 0;
-    k_radix_hist<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, n, shift, bits, L.ntiles, hist,
-                                                        with_begin ? *begin : StepBegin{}, with_begin);
-    k_radix_colscan<<<div_up(bins, 64), 64 * COL_PARTS, 0, stream>>>(hist, L.ntiles, bins, totals);
-    k_radix_scatter<<<L.ntiles, SORT_THREADS, 0, stream>>>(kin, vin, ko, vo, n, shift, bits, L.ntiles, hist,
-                                                           totals);
+    const StepBegin sb = with_begin ? *begin : StepBegin{};
+    if (wide)
+      k_radix_hist<1024><<<L.ntiles, 1024, 0, stream>>>(kin, n, shift, bits, hist, sb, with_begin);
+    else
+      k_radix_hist<256><<<L.ntiles, 256, 0, stream>>>(kin, n, shift, bits, hist, sb, with_begin);
+    if (L.ntiles <= uint32_t(16 * COL_CACHE))
+      k_radix_colscan<64><<<div_up(bins, 64), COL_THREADS, 0, stream>>>(hist, L.ntiles, bins, totals);
+    else if (L.ntiles <= uint32_t(32 * COL_CACHE))
+      k_radix_colscan<32><<<div_up(bins, 32), COL_THREADS, 0, stream>>>(hist, L.ntiles, bins, totals);
+    else
+      k_radix_colscan<16><<<div_up(bins, 16), COL_THREADS, 0, stream>>>(hist, L.ntiles, bins, totals);
+    if (wide)
+      k_radix_scatter<16><<<L.ntiles, 64 * 16, (16 * bins + 16) * sizeof(uint32_t), stream>>>(
+          kin, vin, ko, vo, n, shift, bits, hist, totals);
+    else
+      k_radix_scatter<4><<<L.ntiles, 64 * 4, (4 * bins + 4) * sizeof(uint32_t), stream>>>(
+          kin, vin, ko, vo, n, shift, bits, hist, totals);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     kin = ko;

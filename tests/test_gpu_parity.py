@@ -476,13 +476,16 @@ def _storage_ids(ctx, x_creation):
     return ids
 
 
-@pytest.mark.parametrize("name", ["snowball_drop", "two_snowballs"])
-def test_storage_order_is_stable_cell_sort(name):
+@pytest.mark.parametrize("name,n", [("snowball_drop", None), ("two_snowballs", 1 << 20),
+                                    ("two_snowballs", 1536000), ("two_snowballs", 4300000)])
+def test_storage_order_is_stable_cell_sort(name, n):
     """A step sorts the particles by the cell key of their positions at the start of the step (cell
     tiles of 4^3, tile-major, cells inside a tile x-fastest), stably: particles of one cell keep the
     order they had. Checked on the storage order before and after one step, for every particle
-    whose cell is unambiguous (u = (x - origin)/h further than 1e-4 from an integer)."""
-    sc = getattr(scenes, name)()
+    whose cell is unambiguous (u = (x - origin)/h further than 1e-4 from an integer). The sizes
+    cover sort tiles of 4096 keys that fit one workgroup per CU (16 ranking waves each), more tiles
+    than CUs (4 waves each), and more tiles than the column scan holds in registers."""
+    sc = getattr(scenes, name)() if n is None else getattr(scenes, name)(n_particles=n)
     with SnowContext(sc) as ctx:
         ctx.init()
         for _ in range(2):
@@ -504,7 +507,7 @@ def test_storage_order_is_stable_cell_sort(name):
     rank_before[before] = np.arange(len(before))
     seq = after[sure[after]]  # storage order after the step, unambiguous particles only
     k, r = key[seq], rank_before[seq]
-    print(f"\n[{name}] {len(seq)} of {len(after)} particles checked, {len(np.unique(k))} cells", flush=True)
+    print(f"\n[{name} {n}] {len(seq)} of {len(after)} particles checked, {len(np.unique(k))} cells", flush=True)
     assert (np.diff(k) >= 0).all()
     same = np.diff(k) == 0
     assert (np.diff(r)[same] > 0).all()
