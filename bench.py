@@ -73,6 +73,33 @@ def measured_traffic(kernel: str):
     return None
 
 
+INSTR_SUMMARY = os.path.join(ROOT, "profiles", "snow_step_two_snowballs_1M_128_instruction_counters.md")
+PLAIN_STEP_KERNEL = {"k_g2p_advance": "snow::k_g2p_advance<false, false>", "k_p2g_chunk": "snow::k_p2g_chunk",
+                     "k_grid": "snow::k_grid<false, true>"}  # the instantiations a plain FP32 step launches
+VALU_CYCLES = 4          # cycles per wave64 FP32 VALU instruction (16 lanes per SIMD)
+SIMDS = 256 * 4          # MI355X: 256 CUs of 4 SIMDs
+CLOCK_HZ = 2.4e9         # peak engine clock
+
+
+def valu_floor(kernel: str, measured_us: float):
+    """The vector-ALU floor of `kernel` on this workload: its SQ_INSTS_VALU per launch from the committed
+    rocprofv3 counter summary, at VALU_CYCLES per instruction on every SIMD at CLOCK_HZ, against the
+    measured launch time. None when the summary has no row for the kernel."""
+    try:
+        with open(INSTR_SUMMARY) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) < 3 or cells[0] != PLAIN_STEP_KERNEL.get(kernel):
+                    continue
+                valu = int(cells[2])
+                floor_us = valu * VALU_CYCLES / (SIMDS * CLOCK_HZ) * 1e6
+                return {"instructions": valu, "floor_us": floor_us, "measured_us": measured_us,
+                        "frac": floor_us / measured_us}
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def cpu_baseline(scene, budget_s: float = 12.0):
     """FP32 restatement of the step (oracle/ compiled with Real=float), multithreaded on the host."""
     from oracle import pyoracle
@@ -346,6 +373,9 @@ def main():
         avg_s = ms / calls / 1e3
         alg = bp * n + bn * active
         kernels[KERNEL_NAMES[st]] = {"avg_us": avg_s * 1e6, "alg_bytes": alg, "GBps": alg / avg_s / 1e9}
+        vf = valu_floor(KERNEL_NAMES[st], avg_s * 1e6)
+        if vf:
+            kernels[KERNEL_NAMES[st]]["valu_floor"] = vf
     sort_ms, sort_calls = times.get("#Sim:Sort Particles", (0.0, 0))
     if sort_calls:
         kernels["sort+cell_bounds"] = {"avg_us": sort_ms / sort_calls * 1e3}
@@ -368,7 +398,8 @@ def main():
         "config": {"workload": "two_snowballs_128^3", "particles": n, "grid": [128, 128, 128], "dt": dt,
                    "form": args.form, "active_nodes": active},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
+                     "valu_floor": kernels[dom].get("valu_floor")},
         "step_roofline_frac": (296 * n + 92 * active) * args.steps / el / (HBM_PEAK_GBS * 1e9),
         "kernels": kernels,
     }
