@@ -1149,9 +1149,6 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
   const float4 r1 = rec_in[4 * size_t(p) + 1];
   const float4 r2 = rec_in[4 * size_t(p) + 2];
   const float4 r3 = rec_in[4 * size_t(p) + 3];
-  float FE[9], FP[9];
-#pragma unroll
-  for (int e = 0; e < 9; ++e) { FE[e] = fe_in[size_t(e) * n + p]; FP[e] = fp_in[size_t(e) * n + p]; }
   const float rho0 = rho_in[p];
   const uint32_t id = pid_in[p];
 
@@ -1160,6 +1157,10 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
     g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, true);
   else
     g2p_gather_wave(S, sid, r0.x, r0.y, r0.z, g_node, s_box[threadIdx.x >> 6], vpn, dvp);
+  // F_E, F_P are loaded after the gather: not live through it, and in flight behind its LDS work
+  float FE[9], FP[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { FE[e] = fe_in[size_t(e) * n + p]; FP[e] = fp_in[size_t(e) * n + p]; }
 
   // F_E, F_P update; the next step's force matrix (p2gTransfer.glsl:109-141, once per particle)
   // takes R_E = W V^T from the factors of this update's SVD: the stored F_E is W S' V^T with
