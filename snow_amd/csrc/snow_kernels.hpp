@@ -570,11 +570,14 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       fetch();
       // the wave's own LDS writes complete before its reads (in-order LDS per wave)
       __builtin_amdgcn_wave_barrier();
+      // bit k: staged particle k starts a new cell (bit 0: against the cell of the register sums),
+      // so the loop tests a scalar bit and reads the cell only where it changes
+      const uint32_t prevc = uint32_t(__shfl_up(int(cellv), 1, 64));
+      const uint64_t chg = __ballot(uint32_t(lane) < cnt && (lane == 0 ? int(cellv) != cur : cellv != prevc));
       for (uint32_t k = 0; k < cnt; ++k) {
-        const int cell = __builtin_amdgcn_readlane(int(cellv), int(k));
-        if (cell != cur) {
+        if ((chg >> k) & 1ull) {
           if (cur >= 0) commit();
-          cur = cell;
+          cur = __builtin_amdgcn_readlane(int(cellv), int(k));
         }
         const float* d = stage + k * P2G_STAGE;
         const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
