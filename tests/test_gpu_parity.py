@@ -9,7 +9,8 @@ algorithm, so differences come from FP32 rounding, including the 1e-5 rounding o
 import numpy as np
 import pytest
 
-from snow_amd import (FLAG_KEEP_GRID, FLAG_MIXED, SPHERE, Collider, CollisionObjects, Material, SnowContext)
+from snow_amd import (FLAG_KEEP_GRID, FLAG_MIXED, SPHERE, Collider, CollisionObjects, Material, ParticleSystem,
+                      SnowContext)
 from snow_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -275,6 +276,38 @@ def test_rigid_colliders_advance():
             ctx.step(1e-3)
         cols = ctx.read_colliders()
     assert cols[1, 1] == pytest.approx(0.45 + 10 * 1e-3 * 0.5, abs=1e-6)
+
+
+def test_empty_scene_steps_and_advances_colliders():
+    """A scene without particles steps: alone (the step start then runs as its own launch) its
+    colliders advance and nothing else happens; batched with a scene that has particles (the step
+    start rides on the sort) both scenes' colliders advance and the other scene's particles match
+    its own context bit for bit."""
+    empty = with_colliders(scenes.tiny_block())
+    empty.particles = ParticleSystem()
+    with SnowContext(empty) as ctx:
+        ctx.init()
+        for _ in range(10):
+            ctx.step(1e-3)
+        cols = ctx.read_colliders()
+        assert ctx.read_particles()["x"].shape == (0, 3)
+    assert cols[1, 1] == pytest.approx(0.45 + 10 * 1e-3 * 0.5, abs=1e-6)
+    full = with_colliders(scenes.tiny_block(n_side=6, dims=16, seed=3))
+    with SnowContext(full) as ctx:
+        ctx.init()
+        for _ in range(4):
+            ctx.step(1e-4)
+        single = ctx.read_particles()
+    with SnowContext([empty, full]) as ctx:
+        ctx.init()
+        for _ in range(4):
+            ctx.step(1e-4)
+        batch = ctx.read_particles(1)
+        cols0, cols1 = ctx.read_colliders(0), ctx.read_colliders(1)
+    for k in ("x", "v", "fe", "fp"):
+        assert np.array_equal(single[k], batch[k]), k
+    for c in (cols0, cols1):
+        assert c[1, 1] == pytest.approx(0.45 + 4 * 1e-4 * 0.5, abs=1e-6)
 
 
 def test_nonfinite_detected():

@@ -14,11 +14,14 @@ EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
 
 
 @pytest.mark.parametrize("name,v_q999,v_max", [("snowBallSmash", 1e-3, 5e-3), ("snowBallsCrash", 1e-3, 5e-3),
-                                                ("snowPlowTrack", 5e-3, 2e-2)])
+                                                ("snowPlowTrack", 5e-3, 2e-2), ("snowSingleParticle", 1e-3, 1e-3),
+                                                ("snowPlowField", 1e-3, 1e-3)])
 def test_reference_scene_matches_oracle(oracle_mod, name, v_q999, v_max):
     """5 steps of the reference's scene, FP32 GPU vs FP64 oracle. Positions within 1e-4 of the domain
     size (201 * 0.05 m); velocity errors relative to max(max|v|, 1 m/s), for 99.9 % of the particles
-    and for all. The two snowballs fly freely. The plow track is a box of randomly placed particles
+    and for all. The two snowballs fly freely; the single particle and the two of the plow field
+    fall alone (one and two particles: the smallest sorts and chunks) while the plow field's three
+    sphere colliders move at 10 m/s. The plow track is a box of randomly placed particles
     at rest on the ground: the nodes just above its free top surface carry tiny masses, where f/m
     amplifies roundoff, so FP32 and FP64 part there by up to a few mm/s within 5 steps of 1 ms
     (the worst particles are the top layer), while the bulk agrees far better."""
