@@ -19,11 +19,14 @@ namespace snow {
 
 template <class R> struct M3 { R m[9]; };  // m[c*3+r]
 
+// SVD square roots: the hardware v_rsq_f32 / v_sqrt_f32 (1 ulp) without the library's rescaling of
+// denormal inputs, which the SVD never passes (the Jacobi step takes its exact rotation only for a
+// normal ch^2; the Givens pivot is at least EPS^2). FP64 uses the library functions.
 template <class R> __device__ __forceinline__ R rsq(R x);
-template <> __device__ __forceinline__ float rsq<float>(float x) { return rsqrtf(x); }
+template <> __device__ __forceinline__ float rsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <> __device__ __forceinline__ double rsq<double>(double x) { return rsqrt(x); }
 template <class R> __device__ __forceinline__ R sq_root(R x);
-template <> __device__ __forceinline__ float sq_root<float>(float x) { return sqrtf(x); }
+template <> __device__ __forceinline__ float sq_root<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 template <> __device__ __forceinline__ double sq_root<double>(double x) { return sqrt(x); }
 // GLSL round(): nearest integer; the direction of exact .5 ties is left to the implementation,
 // here the hardware's round-to-nearest-even (one instruction).
@@ -100,20 +103,18 @@ template <class R>
 __device__ __forceinline__ void jacobi_conj(int x, int y, int z, R& s00, R& s11, R& s22, R& s01,
                                             R& s02, R& s12, R q[4]) {
   const R GAMMA = R(5.828427124), CSTAR = R(0.923879532), SSTAR = R(0.3826834323);
-  // cos(2t), sin(2t) of the fixed angle pi/8 branch, evaluated as the reference does
-  const R ASTAR = (CSTAR * CSTAR - SSTAR * SSTAR) / (CSTAR * CSTAR + SSTAR * SSTAR);
-  const R BSTAR = (R(2) * SSTAR * CSTAR) / (CSTAR * CSTAR + SSTAR * SSTAR);
   const R ch = R(2) * (s00 - s11), sh = s01;
   const R ch2 = ch * ch, sh2 = sh * sh;
-  const bool flag = GAMMA * sh2 < ch2;
+  // exact rotation when ch^2 > gamma sh^2 (SVD.include.glsl:83-92), else the fixed pi/8 one; a
+  // denormal ch^2 (never reached by matrices with entries of order one) also takes the fixed one
+  const bool flag = GAMMA * sh2 < ch2 && ch2 >= R(1.17549435e-38);
   const R w = rsq(ch2 + sh2);
-  // the reference normalises (ch, sh) with w and then divides by ch^2 + sh^2 of the normalised
-  // pair (SVD.include.glsl:83-92); with w^2 = 1 / (ch^2 + sh^2) both steps fold into one scale
-  const R w2 = w * w;
   const R chn = flag ? w * ch : CSTAR;
   const R shn = flag ? w * sh : SSTAR;
-  const R a = flag ? (ch2 - sh2) * w2 : ASTAR;
-  const R b = flag ? R(2) * (sh * ch) * w2 : BSTAR;
+  // cos(2t), sin(2t) of the normalised half-angle pair; the reference also divides both by
+  // chn^2 + shn^2, which is 1 up to rounding in either branch
+  const R a = chn * chn - shn * shn;
+  const R b = R(2) * (chn * shn);
   const R a2 = a * a, b2 = b * b, ab = a * b;
   // S[1][0]=s01, S[2][0]=s02, S[2][1]=s12, S[1][2]=s12, S[0][2]=s02
   const R n0 = a2 * s00 + R(2) * ab * s01 + b2 * s11;
