@@ -260,6 +260,23 @@ __device__ __forceinline__ uint32_t* bound_slots(uint32_t* pb, int sid, uint32_t
   return pb + (size_t(sid) * 2 + parity) * (PB_SLOTS * PB_STRIDE);
 }
 
+// Wave-uniform minimum / maximum by DPP (no LDS traffic): row shifts by 1, 2, 4, 8 leave each
+// row's extreme in its lane 15, the row-15 and row-31 broadcasts carry them to lane 63 (lanes
+// without a source keep their own value), which is read as a scalar.
+template <bool MAX>
+__device__ __forceinline__ int wave_ext_i(int v) {
+  auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));  // row_bcast:15 to rows 1, 3
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));  // row_bcast:31 to rows 2, 3
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min_i(int v) { return wave_ext_i<false>(v); }
+__device__ __forceinline__ int wave_max_i(int v) { return wave_ext_i<true>(v); }
+
 // wave_full (uniform): all 64 lanes of the calling wave are live; otherwise each lane publishes
 // its own bound. NaN and infinite terms count as the largest float.
 __device__ __forceinline__ void publish_bound(uint32_t* __restrict__ pb, uint32_t parity, int sid, P2GBound b,
@@ -271,12 +288,10 @@ __device__ __forceinline__ void publish_bound(uint32_t* __restrict__ pb, uint32_
   if (wave_full) {
     const int sid0 = __builtin_amdgcn_readfirstlane(sid);
     if (__ballot(sid != sid0) == 0ull) {
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        m = fmaxf(m, __shfl_xor(m, off, 64));
-        p = fmaxf(p, __shfl_xor(p, off, 64));
-        f = fmaxf(f, __shfl_xor(f, off, 64));
-      }
+      // values in [0, FLT_MAX]: their bit patterns order as the floats do
+      m = __int_as_float(wave_max_i(__float_as_int(m)));
+      p = __int_as_float(wave_max_i(__float_as_int(p)));
+      f = __int_as_float(wave_max_i(__float_as_int(f)));
       if ((threadIdx.x & 63) == 0) {
         uint32_t* sl = bound_slots(pb, sid0, parity) + slot * PB_STRIDE;
         if (m > 0.f) atomicMax(sl, __float_as_uint(m));
@@ -317,12 +332,10 @@ __device__ __forceinline__ void publish_bound_block(uint32_t* __restrict__ pb, u
   float f = b.f <= __FLT_MAX__ ? b.f : __FLT_MAX__;
   const int lane = threadIdx.x & 63;
   if ((i | 63u) < n) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      m = fmaxf(m, __shfl_xor(m, off, 64));
-      p = fmaxf(p, __shfl_xor(p, off, 64));
-      f = fmaxf(f, __shfl_xor(f, off, 64));
-    }
+    // values in [0, FLT_MAX]: their bit patterns order as the floats do
+    m = __int_as_float(wave_max_i(__float_as_int(m)));
+    p = __int_as_float(wave_max_i(__float_as_int(p)));
+    f = __int_as_float(wave_max_i(__float_as_int(f)));
     if (lane == 0) {
       atomicMax(&s_pb[0], __float_as_uint(m));
       atomicMax(&s_pb[1], __float_as_uint(p));
@@ -436,18 +449,13 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     }
     const uint32_t par = step & 1u;
     if (wave == P2G_WAVES - 1) {  // max over the scene's bound slots of this step
+      // slots hold the bits of non-negative floats: the integer maximum is the float maximum
       const uint32_t* sl = bound_slots(pbound, sid, par) + lane * PB_STRIDE;
-      float bm = __uint_as_float(sl[0]), bp = __uint_as_float(sl[1]), bf = __uint_as_float(sl[2]);
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        bm = fmaxf(bm, __shfl_xor(bm, off, 64));
-        bp = fmaxf(bp, __shfl_xor(bp, off, 64));
-        bf = fmaxf(bf, __shfl_xor(bf, off, 64));
-      }
+      const int bm = wave_max_i(int(sl[0])), bp = wave_max_i(int(sl[1])), bf = wave_max_i(int(sl[2]));
       if (lane == 0) {
-        s_bound[0] = bm;
-        s_bound[1] = bp;
-        s_bound[2] = bf;
+        s_bound[0] = __int_as_float(bm);
+        s_bound[1] = __int_as_float(bp);
+        s_bound[2] = __int_as_float(bf);
       }
     }
     // the scene's first chunk clears the slots the writers of this step fill for the next one
@@ -990,17 +998,6 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
 // box use g2p_gather. The sums are the same as g2p_gather's, in the same order.
 constexpr int G2P_BOX = 384;
 
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
 __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, float x, float y, float z,
                                                 const float2* __restrict__ g_node, float2* box, float vpn[3],
                                                 float dvp[9]) {
@@ -1017,12 +1014,8 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
     return;
   }
   // node box [xmin, xmin+ex) x [ymin, ymin+ey) x [zmin, zmin+ez)
-  const int xmin = __builtin_amdgcn_readfirstlane(wave_min_i(bx)) - 1;
-  const int ymin = __builtin_amdgcn_readfirstlane(wave_min_i(by)) - 1;
-  const int zmin = __builtin_amdgcn_readfirstlane(wave_min_i(bz)) - 1;
-  const int ex = __builtin_amdgcn_readfirstlane(wave_max_i(bx)) + 3 - xmin;
-  const int ey = __builtin_amdgcn_readfirstlane(wave_max_i(by)) + 3 - ymin;
-  const int ez = __builtin_amdgcn_readfirstlane(wave_max_i(bz)) + 3 - zmin;
+  const int xmin = wave_min_i(bx) - 1, ymin = wave_min_i(by) - 1, zmin = wave_min_i(bz) - 1;
+  const int ex = wave_max_i(bx) + 3 - xmin, ey = wave_max_i(by) + 3 - ymin, ez = wave_max_i(bz) + 3 - zmin;
   const int vol = ex * ey * ez;
   if (vol > G2P_BOX) {
     g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
