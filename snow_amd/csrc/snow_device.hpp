@@ -77,8 +77,10 @@ __device__ __forceinline__ R det3(const R* m) {
 // ---- Lame parameters (lame.include.glsl:69-85) -----------------------------------------
 template <class R>
 __device__ __forceinline__ R hardening_factor(R xi, R JP);
+// FP32: the hardware exp2 (__expf, a few ulp for the small arguments xi (1 - J_P) here; GLSL's
+// exp allows 3 + 2|x| ulp)
 template <> __device__ __forceinline__ float hardening_factor<float>(float xi, float JP) {
-  return expf(xi * (1.0f - JP));
+  return __expf(xi * (1.0f - JP));
 }
 template <> __device__ __forceinline__ double hardening_factor<double>(double xi, double JP) {
   return exp(xi * (1.0 - JP));
@@ -268,7 +270,7 @@ __device__ __forceinline__ void force_matrix_wv(const R* FE, const R* FP, const 
   for (int i = 0; i < 9; ++i) D[i] = FE[i] - Rot[i];
   mul_bt(D, FE, P);  // (F_E - R_E) F_E^T
   const R lterm = la * (JE - R(1)) * JE;
-  const R k = rho0 > R(0) ? gs * (m / rho0) : R(0);
+  const R k = rho0 > R(0) ? gs * (m * recip(rho0)) : R(0);  // m / rho0 within 2 ulp (GLSL: 2.5)
 #pragma unroll
   for (int i = 0; i < 9; ++i) Aout[i] = k * (R(2) * mu * P[i] + ((i == 0 || i == 4 || i == 8) ? lterm : R(0)));
 }
