@@ -353,7 +353,8 @@ int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
   if (c->total_chunks == 0) return tm.done();
   const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu * 4)));
 #define GRID_ARGS                                                                                          \
-  c->d_scenes, c->d_tile_scene, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_ctl, c->d_tile_stamp,     \
+  c->d_scenes, c->nscenes == 1 ? nullptr : c->d_tile_scene, c->d_list_ctr, c->d_tile_list, c->total_ctiles,    \
+      c->d_ctl, c->d_tile_stamp,                                                                              \
       c->d_tpart, c->d_zero_part, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass, c->d_cols, dt, cnt
   if (keep && update)
     k_grid<true, true><<<blocks, GRID_THREADS, 0, c->stream>>>(GRID_ARGS);

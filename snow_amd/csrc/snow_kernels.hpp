@@ -685,6 +685,7 @@ __device__ __forceinline__ bool tile_live(const SceneDev& S, const uint32_t* __r
 
 constexpr int GRID_THREADS = 512;  // 8 waves: one per candidate node tile
 
+// tile_scene (scene of every cell tile) may be null for a context of one scene.
 template <bool KEEP, bool UPDATE>
 __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restrict__ scenes,
                                                        const uint16_t* __restrict__ tile_scene,
@@ -717,7 +718,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
     const int sub = __builtin_amdgcn_readfirstlane(__ffsll((long long)__ballot(incl > e)) - 1);
     const uint32_t first = __builtin_amdgcn_readfirstlane(__shfl(incl - cnt, sub, 64));
     const uint32_t k = tile_list[size_t(sub) * list_cap + (e - first)];
-    const int sid = tile_scene[k];
+    const int sid = tile_scene ? tile_scene[k] : 0;  // no table: one scene, whose record loads at once
     const SceneDev& S = scenes[sid];
     const uint32_t kl = k - S.ct_off;
     const int kx = int(kl % uint32_t(S.ct[0])), ky = int((kl / uint32_t(S.ct[0])) % uint32_t(S.ct[1]));
