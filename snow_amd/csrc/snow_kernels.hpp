@@ -1121,7 +1121,7 @@ __device__ __forceinline__ bool needs_fp64(const float dvp[9], float dt, const f
 // step's force matrix (from the SVD of the F_E update) and the next step's cell key. The
 // permutation of the sort is thereby applied for free by the writes of this kernel.
 template <bool MIXED, bool COMP>
-__global__ void __launch_bounds__(256) k_g2p_advance(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_g2p_advance(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ pscene, const uint32_t* __restrict__ vals,
     const float4* __restrict__ rec_in, const float* __restrict__ fe_in, const float* __restrict__ fp_in,
     const float* __restrict__ rho_in, const uint32_t* __restrict__ pid_in, float4* __restrict__ rec_out,
@@ -1138,69 +1138,77 @@ __global__ void __launch_bounds__(256) k_g2p_advance(
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // The body below runs with a wave-uniform scene (its record in scalar registers) when all lanes
+  // of the wave share one, which sorted particles almost always do, else with one scene per lane.
+  auto body = [&](const SceneDev& S, int sid) {
+    const float dt = dt_override > 0.0f ? dt_override : S.dt;
+    const uint32_t p = vals[i];
+    const float4 r0 = rec_in[4 * size_t(p)];
+    const float4 r1 = rec_in[4 * size_t(p) + 1];
+    const float4 r2 = rec_in[4 * size_t(p) + 2];
+    const float4 r3 = rec_in[4 * size_t(p) + 3];
+    const float rho0 = rho_in[p];
+    const uint32_t id = pid_in[p];
+
+    float vpn[3], dvp[9];
+    if (COMP)
+      g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, true);
+    else
+      g2p_gather_wave(S, sid, r0.x, r0.y, r0.z, g_node, s_box[threadIdx.x >> 6], vpn, dvp);
+    // F_E, F_P are loaded after the gather: not live through it, and in flight behind its LDS work
+    float FE[9], FP[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) { FE[e] = fe_in[size_t(e) * n + p]; FP[e] = fp_in[size_t(e) * n + p]; }
+
+    // F_E, F_P update; the next step's force matrix (p2gTransfer.glsl:109-141, once per particle)
+    // takes R_E = W V^T from the factors of this update's SVD: the stored F_E is W S' V^T with
+    // S' the clamped triangular factor, so its own SVD gives the same W V^T.
+    float FEo[9], FPo[9], A[9];
+    bool wide = false;
+    if (MIXED) wide = needs_fp64(dvp, dt, FE, S.crit_comp, S.crit_stretch);
+    if (wide) {
+      double W[9], V[9], FEd[9], FPd[9], Ad[9];
+      deform_update<double>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) { FEd[e] = FEo[e]; FPd[e] = FPo[e]; }
+      force_matrix_wv<double>(FEd, FPd, W, V, double(r0.w), double(rho0), double(S.mu0), double(S.lambda0),
+                              double(S.hardening), double(S.gscale), Ad);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) A[e] = float(Ad[e]);
+      if (fp64_count) atomicAdd(fp64_count, 1ull);
+    } else {
+      float W[9], V[9];
+      deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
+      force_matrix_wv<float>(FEo, FPo, W, V, r0.w, rho0, S.mu0, S.lambda0, S.hardening, S.gscale, A);
+    }
+
+    // velocity, collisions at the pre-advection position, advection (particleAdvance.glsl:318-359)
+    float vx = r1.w * S.alpha + vpn[0], vy = r2.w * S.alpha + vpn[1], vz = r3.w * S.alpha + vpn[2];
+    collide(r0.x, r0.y, r0.z, vx, vy, vz, cols + S.col_off, S.ncol);
+    const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
+
+    // rec rows: A row r = (A[r], A[3+r], A[6+r]) from the column-major A
+    const float4 o0 = make_float4(x, y, z, r0.w), o1 = make_float4(A[0], A[3], A[6], vx),
+                 o2 = make_float4(A[1], A[4], A[7], vy), o3 = make_float4(A[2], A[5], A[8], vz);
+    rec_out[4 * size_t(i)] = o0;
+    rec_out[4 * size_t(i) + 1] = o1;
+    rec_out[4 * size_t(i) + 2] = o2;
+    rec_out[4 * size_t(i) + 3] = o3;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) { fe_out[size_t(e) * n + i] = FEo[e]; fp_out[size_t(e) * n + i] = FPo[e]; }
+    rho_out[i] = rho0;
+    pid_out[i] = id;
+    const uint32_t key = cell_key(S, x, y, z);
+    keys_next[i] = key;
+    vals_next[i] = i;
+    publish_bound_block(pbound, next_parity, sid, record_bound(S, key, o0, o1, o2, o3, 1), s_pb, one_scene, i, n);
+  };
   const int sid = pscene[i];
-  const SceneDev& S = scenes[sid];
-  const float dt = dt_override > 0.0f ? dt_override : S.dt;
-  const uint32_t p = vals[i];
-  const float4 r0 = rec_in[4 * size_t(p)];
-  const float4 r1 = rec_in[4 * size_t(p) + 1];
-  const float4 r2 = rec_in[4 * size_t(p) + 2];
-  const float4 r3 = rec_in[4 * size_t(p) + 3];
-  const float rho0 = rho_in[p];
-  const uint32_t id = pid_in[p];
-
-  float vpn[3], dvp[9];
-  if (COMP)
-    g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, true);
+  const int sid0 = __builtin_amdgcn_readfirstlane(sid);
+  if (__ballot(sid != sid0) == 0ull)
+    body(scenes[sid0], sid0);
   else
-    g2p_gather_wave(S, sid, r0.x, r0.y, r0.z, g_node, s_box[threadIdx.x >> 6], vpn, dvp);
-  // F_E, F_P are loaded after the gather: not live through it, and in flight behind its LDS work
-  float FE[9], FP[9];
-#pragma unroll
-  for (int e = 0; e < 9; ++e) { FE[e] = fe_in[size_t(e) * n + p]; FP[e] = fp_in[size_t(e) * n + p]; }
-
-  // F_E, F_P update; the next step's force matrix (p2gTransfer.glsl:109-141, once per particle)
-  // takes R_E = W V^T from the factors of this update's SVD: the stored F_E is W S' V^T with
-  // S' the clamped triangular factor, so its own SVD gives the same W V^T.
-  float FEo[9], FPo[9], A[9];
-  bool wide = false;
-  if (MIXED) wide = needs_fp64(dvp, dt, FE, S.crit_comp, S.crit_stretch);
-  if (wide) {
-    double W[9], V[9], FEd[9], FPd[9], Ad[9];
-    deform_update<double>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) { FEd[e] = FEo[e]; FPd[e] = FPo[e]; }
-    force_matrix_wv<double>(FEd, FPd, W, V, double(r0.w), double(rho0), double(S.mu0), double(S.lambda0),
-                            double(S.hardening), double(S.gscale), Ad);
-#pragma unroll
-    for (int e = 0; e < 9; ++e) A[e] = float(Ad[e]);
-    if (fp64_count) atomicAdd(fp64_count, 1ull);
-  } else {
-    float W[9], V[9];
-    deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
-    force_matrix_wv<float>(FEo, FPo, W, V, r0.w, rho0, S.mu0, S.lambda0, S.hardening, S.gscale, A);
-  }
-
-  // velocity, collisions at the pre-advection position, advection (particleAdvance.glsl:318-359)
-  float vx = r1.w * S.alpha + vpn[0], vy = r2.w * S.alpha + vpn[1], vz = r3.w * S.alpha + vpn[2];
-  collide(r0.x, r0.y, r0.z, vx, vy, vz, cols + S.col_off, S.ncol);
-  const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
-
-  // rec rows: A row r = (A[r], A[3+r], A[6+r]) from the column-major A
-  const float4 o0 = make_float4(x, y, z, r0.w), o1 = make_float4(A[0], A[3], A[6], vx),
-               o2 = make_float4(A[1], A[4], A[7], vy), o3 = make_float4(A[2], A[5], A[8], vz);
-  rec_out[4 * size_t(i)] = o0;
-  rec_out[4 * size_t(i) + 1] = o1;
-  rec_out[4 * size_t(i) + 2] = o2;
-  rec_out[4 * size_t(i) + 3] = o3;
-#pragma unroll
-  for (int e = 0; e < 9; ++e) { fe_out[size_t(e) * n + i] = FEo[e]; fp_out[size_t(e) * n + i] = FPo[e]; }
-  rho_out[i] = rho0;
-  pid_out[i] = id;
-  const uint32_t key = cell_key(S, x, y, z);
-  keys_next[i] = key;
-  vals_next[i] = i;
-  publish_bound_block(pbound, next_parity, sid, record_bound(S, key, o0, o1, o2, o3, 1), s_pb, one_scene, i, n);
+    body(scenes[sid], sid);
 }
 
 // Split stages for the per-stage API: G2P alone writes (vpn, dvp) per particle slot;
