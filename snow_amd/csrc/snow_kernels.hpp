@@ -1115,13 +1115,19 @@ __device__ __forceinline__ bool needs_fp64(const float dvp[9], float dt, const f
 }
 
 // ---------------------------------------------------------------------------------------
+// Waves per SIMD the G2P register allocation aims at: 4 for the FP32 kernel (128 VGPRs;
+// uncapped, the per-lane scene path below takes it to 135 and 3 waves, which is slower than a
+// few spills), 3 with the FP64 path compiled in (168 VGPRs: at 4 waves it spills in the hot
+// path, at 2 waves latency is exposed).
+template <bool MIXED> struct G2PWaves { static constexpr int value = MIXED ? 3 : 4; };
+
 // Fused G2P + particle advance (g2pTransfer.glsl:53-143, particleAdvance.glsl:75-217).
 // Thread i handles sorted position i: it reads the particle from its previous slot
 // vals[i] (buffers *_in), writes the new state to slot i (buffers *_out), computes the next
 // step's force matrix (from the SVD of the F_E update) and the next step's cell key. The
 // permutation of the sort is thereby applied for free by the writes of this kernel.
 template <bool MIXED, bool COMP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_g2p_advance(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G2PWaves<MIXED>::value))) k_g2p_advance(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ pscene, const uint32_t* __restrict__ vals,
     const float4* __restrict__ rec_in, const float* __restrict__ fe_in, const float* __restrict__ fp_in,
     const float* __restrict__ rho_in, const uint32_t* __restrict__ pid_in, float4* __restrict__ rec_out,
