@@ -4,6 +4,8 @@ import os
 import subprocess
 import sys
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -27,3 +29,17 @@ def test_measured_traffic_reads_committed_summary():
     t = bench.measured_traffic("k_g2p_advance")
     assert t is None or t > 100_000_000
     assert bench.measured_traffic("no_such_kernel") is None
+
+
+def test_valu_floor_reads_the_instruction_profile():
+    """roofline.valu_floor prices the committed SQ_INSTS_VALU of a kernel at 4 cycles per wave64
+    instruction on 1024 SIMDs at 2.4 GHz, against the measured launch time."""
+    sys.path.insert(0, ROOT)
+    import bench
+    for kernel in ("k_g2p_advance", "k_p2g_chunk", "k_grid"):
+        vf = bench.valu_floor(kernel, 100.0)
+        assert vf is not None, kernel
+        assert vf["floor_us"] == pytest.approx(vf["instructions"] * 4 / (1024 * 2.4e9) * 1e6)
+        assert vf["frac"] == pytest.approx(vf["floor_us"] / 100.0)
+        assert 0.0 < vf["floor_us"] < 100.0
+    assert bench.valu_floor("k_no_such_kernel", 1.0) is None
