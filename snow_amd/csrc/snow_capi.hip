@@ -76,7 +76,7 @@ struct snow_ctx {
   float* d_rho[2] = {nullptr, nullptr};
   uint32_t* d_pid[2] = {nullptr, nullptr};
   uint32_t* d_keys[2] = {nullptr, nullptr};
-  uint32_t* d_vals[2] = {nullptr, nullptr};
+  uint32_t* d_slot = nullptr;  // slot of the particle at each sorted position (the sort's permutation)
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   uint32_t *d_tile_lo = nullptr, *d_tile_hi = nullptr;
@@ -286,7 +286,7 @@ int run_keys(snow_ctx* c) {
   if (c->keys_valid) return 0;
   if (c->N) {
     k_keys<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->d_keys[0],
-                                                       c->d_vals[0], c->N);
+                                                       c->N);
     LAUNCHCHK(c);
   }
   c->keys_valid = true;
@@ -318,7 +318,7 @@ int run_sort(snow_ctx* c, float dt_rigid) {
   StageTimer tm(c, ST_SORT);
   if (c->N) {
     size_t tb = c->sort_tmp_bytes;
-    HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->N,
+    HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], nullptr, c->d_slot, c->N,
                                c->key_bits, c->sort_bits, c->stream, c->num_cu, &begin));
   } else {
     k_step_begin<<<1, 256, 0, c->stream>>>(begin);
@@ -335,7 +335,7 @@ int run_p2g(snow_ctx* c, int mass_only) {
   StageTimer tm(c, ST_P2G);
   if (c->total_chunks) {
     k_p2g_chunk<<<c->total_chunks, 64 * P2G_WAVES, 0, c->stream>>>(
-        c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_vals[1], c->d_tile_lo, c->d_tile_hi, c->d_tile_stamp,
+        c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_slot, c->d_tile_lo, c->d_tile_hi, c->d_tile_stamp,
         c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_tpart, c->d_hpart,
         c->d_chunk_scene, c->d_pbound, mass_only ? 0 : 1);
     LAUNCHCHK(c);
@@ -375,8 +375,8 @@ int run_g2p_advance(snow_ctx* c, float dt) {
   const bool comp = (c->flags & SNOW_FLAG_COMPENSATED_G2P) != 0;
   if (c->N) {
 #define G2P_ARGS                                                                                        \
-  c->d_scenes, c->d_pscene, c->d_vals[1], c->d_rec[a], c->d_fe[a], c->d_fp[a], c->d_rho[a], c->d_pid[a], \
-      c->d_rec[b], c->d_fe[b], c->d_fp[b], c->d_rho[b], c->d_pid[b], c->d_keys[0], c->d_vals[0],         \
+  c->d_scenes, c->d_pscene, c->d_slot, c->d_rec[a], c->d_fe[a], c->d_fp[a], c->d_rho[a], c->d_pid[a], \
+      c->d_rec[b], c->d_fe[b], c->d_fp[b], c->d_rho[b], c->d_pid[b], c->d_keys[0],                        \
       c->d_g_node, c->d_cols, dt, c->N, f64, c->d_ctl, c->d_pbound
     const unsigned bs = unsigned(c->g2p_block);
     const unsigned g = grid_for(c->N, bs);
@@ -418,7 +418,7 @@ void release(snow_ctx* c) {
   }
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
-                  c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1], c->d_sort_tmp,
+                  c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp,
                   c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
@@ -586,11 +586,11 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     CREATECHK(dalloc(&c->d_rho[b], N));
     CREATECHK(dalloc(&c->d_pid[b], N));
     CREATECHK(dalloc(&c->d_keys[b], N));
-    CREATECHK(dalloc(&c->d_vals[b], N));
   }
+  CREATECHK(dalloc(&c->d_slot, N));
   {
     size_t tb = 0;
-    CREATECHK(radix_sort_pairs(nullptr, &tb, c->d_keys[0], c->d_keys[1], c->d_vals[0], c->d_vals[1],
+    CREATECHK(radix_sort_pairs(nullptr, &tb, c->d_keys[0], c->d_keys[1], nullptr, c->d_slot,
                                std::max<uint32_t>(N, 1), c->key_bits, 11, c->stream, c->num_cu));
     c->sort_tmp_bytes = tb;
     CREATECHK(hipMalloc(&c->d_sort_tmp, std::max<size_t>(tb, 16)));
@@ -840,7 +840,7 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
       {nullptr, c->d_rec[k], sizeof(float4) * 4 * N}, {nullptr, c->d_fe[k], sizeof(float) * 9 * N},
       {nullptr, c->d_fp[k], sizeof(float) * 9 * N},  {nullptr, c->d_rho[k], sizeof(float) * N},
       {nullptr, c->d_pid[k], sizeof(uint32_t) * N},   {nullptr, c->d_keys[0], sizeof(uint32_t) * N},
-      {nullptr, c->d_vals[0], sizeof(uint32_t) * N},  {nullptr, c->d_ctl, sizeof(StepCtl)},
+      {nullptr, c->d_ctl, sizeof(StepCtl)},
       {nullptr, c->d_cols, sizeof(Collider) * std::max<size_t>(1, c->cols.size())}};
   auto free_all = [&]() {
     for (auto& b : bufs)
@@ -1000,7 +1000,7 @@ int snow_stage_particle_advance(snow_ctx* c, double dt) {
   if (c->N) {
     k_advance_only<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[k], c->d_fe[k],
                                                                 c->d_fp[k], c->d_rho[k], c->d_aux, c->d_cols, fdt,
-                                                                c->d_keys[0], c->d_vals[0], c->N, c->d_ctl,
+                                                                c->d_keys[0], c->N, c->d_ctl,
                                                                 c->d_pbound);
     LAUNCHCHK(c);
   }

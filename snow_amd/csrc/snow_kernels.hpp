@@ -106,13 +106,12 @@ __device__ __forceinline__ uint32_t cell_key(const SceneDev& S, float x, float y
 __global__ void __launch_bounds__(256) k_keys(const SceneDev* __restrict__ scenes,
                                               const uint16_t* __restrict__ pscene,
                                               const float4* __restrict__ rec, uint32_t* __restrict__ keys,
-                                              uint32_t* __restrict__ vals, uint32_t n) {
+                                              uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SceneDev& S = scenes[pscene[i]];
   const float4 xm = rec[4 * size_t(i)];
   keys[i] = cell_key(S, xm.x, xm.y, xm.z);
-  vals[i] = i;
 }
 
 
@@ -1132,7 +1131,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G2PWav
     const float4* __restrict__ rec_in, const float* __restrict__ fe_in, const float* __restrict__ fp_in,
     const float* __restrict__ rho_in, const uint32_t* __restrict__ pid_in, float4* __restrict__ rec_out,
     float* __restrict__ fe_out, float* __restrict__ fp_out, float* __restrict__ rho_out,
-    uint32_t* __restrict__ pid_out, uint32_t* __restrict__ keys_next, uint32_t* __restrict__ vals_next,
+    uint32_t* __restrict__ pid_out, uint32_t* __restrict__ keys_next,
     const float2* __restrict__ g_node, const Collider* __restrict__ cols,
     float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count, const StepCtl* __restrict__ ctl,
     uint32_t* __restrict__ pbound) {
@@ -1206,7 +1205,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G2PWav
     pid_out[i] = id;
     const uint32_t key = cell_key(S, x, y, z);
     keys_next[i] = key;
-    vals_next[i] = i;
     publish_bound_block(pbound, next_parity, sid, record_bound(S, key, o0, o1, o2, o3, 1), s_pb, one_scene, i, n);
   };
   const int sid = pscene[i];
@@ -1241,7 +1239,7 @@ __global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict
                                                       float* __restrict__ fe, float* __restrict__ fp,
                                                       const float* __restrict__ rho, const float* __restrict__ aux,
                                                       const Collider* __restrict__ cols, float dt_override,
-                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                      uint32_t* __restrict__ keys,
                                                       uint32_t n, const StepCtl* __restrict__ ctl,
                                                       uint32_t* __restrict__ pbound) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1277,7 +1275,6 @@ __global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict
   for (int e = 0; e < 9; ++e) { fe[size_t(e) * n + i] = FEo[e]; fp[size_t(e) * n + i] = FPo[e]; }
   const uint32_t key = cell_key(S, x, y, z);
   keys[i] = key;
-  vals[i] = i;
   publish_bound(pbound, (ctl->step + 1u) & 1u, sid, record_bound(S, key, o0, o1, o2, o3, 1), (i | 63u) < n);
 }
 

@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_radix_scatter(
   for (int r = 0; r < ITEMS; ++r) {
     const uint32_t i = base + uint32_t(r * 64);
     kreg[r] = i < n ? kin[i] : 0u;
-    vreg[r] = i < n ? vin[i] : 0u;
+    vreg[r] = vin ? (i < n ? vin[i] : 0u) : i;  // no value array: the value is the position
   }
   // thread t owns digits [PER t, PER t + PER): their totals and this tile's row offsets load while
   // the wave ranks its keys
@@ -258,7 +258,8 @@ SortLayout layout(uint32_t n) {
 }  // namespace
 
 // Stable sort of (keys_in, vals_in) by the low end_bit bits of the key into (keys_out,
-// vals_out), in passes of at most digit_bits (4..11) bits. With temp == nullptr, returns the
+// vals_out), in passes of at most digit_bits (4..11) bits. vals_in == nullptr stands for the
+// values 0..n-1 (vals_out is then the sorting permutation). With temp == nullptr, returns the
 // temporary storage size in *temp_bytes. Arrays must be 16-byte aligned. With begin != nullptr
 // (and n > 0) the first launch also runs the step start *begin.
 inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
