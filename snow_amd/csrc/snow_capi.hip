@@ -124,6 +124,7 @@ struct snow_ctx {
   int g2p_block = 256;
   uint32_t total_chunks = 0;       // P2G workgroups (chunks of every scene)
   uint16_t* d_chunk_scene = nullptr;
+  uint32_t* d_chunk_lo = nullptr;  // first sorted position of each P2G chunk, as the P2G chose it
 };
 
 namespace {
@@ -337,10 +338,11 @@ int run_p2g(snow_ctx* c, int mass_only) {
     k_p2g_chunk<<<c->total_chunks, 64 * P2G_WAVES, 0, c->stream>>>(
         c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_slot, c->d_tile_lo, c->d_tile_hi, c->d_tile_stamp,
         c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_tpart, c->d_hpart,
-        c->d_chunk_scene, c->d_pbound, mass_only ? 0 : 1);
+        c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, mass_only ? 0 : 1);
     LAUNCHCHK(c);
     k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
-                                                            c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart);
+                                                            c->d_tile_lo, c->d_tile_hi, c->d_chunk_lo, c->d_tpart,
+                                                            c->d_hpart);
     LAUNCHCHK(c);
   }
   c->bound_valid = false;  // consumed; the slots of the other parity are cleared for the writers
@@ -419,7 +421,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp,
-                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
+                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -528,6 +530,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
       uint32_t seg = 256;
       while (seg < uint32_t(P2G_SEG_MAX) && uint64_t(seg) * 2 * 64 <= uint64_t(d.num_particles)) seg *= 2;
       S.seg = seg;
+      // bounds at tile starts where a chunk plus its search window fits the P2G's LDS lists
+      S.align_w = std::min(seg, uint32_t(P2G_SEG_MAX) - seg);
       S.chunk_off = uint32_t(chunk_off);
       chunk_off += (uint64_t(d.num_particles) + seg - 1) / seg;
     }
@@ -600,6 +604,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_tpart, size_t(c->total_ctiles) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_hpart, size_t(std::max<uint32_t>(1, c->total_chunks)) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_chunk_scene, std::max<uint32_t>(1, c->total_chunks)));
+  CREATECHK(dalloc(&c->d_chunk_lo, std::max<uint32_t>(1, c->total_chunks)));
   CREATECHK(dalloc(&c->d_zero_part, size_t(NQ) * HALO_NODES));
   CREATECHK(dalloc(&c->d_list_ctr, size_t(TILE_LISTS) * TILE_LIST_STRIDE));
   CREATECHK(hipMemset(c->d_list_ctr, 0, sizeof(uint32_t) * TILE_LISTS * TILE_LIST_STRIDE));

@@ -49,7 +49,8 @@ struct SceneDev {
   float inv_h;        // 1/h
   uint32_t chunk_off; // global index of the scene's first P2G chunk
   float uo[3];        // -origin/h: grid coordinate u = x/h + uo
-  float pad2;
+  uint32_t align_w;   // P2G chunk bounds move to the first cell-tile start within this many
+                      // positions of p_off + c*seg (0: they stay there)
 };
 
 // Grid coordinate of a position: u = (x - origin)/h evaluated as x*(1/h) - origin/h (one FMA).
@@ -195,9 +196,11 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
 // ---------------------------------------------------------------------------------------
 // P2G (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84) over chunks of the sorted particles.
 // A scene's particles are contiguous in sorted order; its chunk c (global index chunk_off + c)
-// is the sorted range [p_off + c*seg, p_off + (c+1)*seg), so every workgroup gets about the same
-// number of particles however unevenly they fill the cell tiles, and a scene's chunks are the
-// same alone or in a batch.
+// is the sorted range [b(c), b(c+1)) with b(c) = p_off + c*seg, so every workgroup gets about the
+// same number of particles however unevenly they fill the cell tiles, and a scene's chunks are
+// the same alone or in a batch. Where seg + align_w <= P2G_SEG_MAX (scenes with small chunks),
+// b(c) moves forward to the first cell-tile start within align_w positions, so that a tile
+// is split between chunks only when it is longer than that: no continuation partial, no fold.
 // The workgroup first reads the chunk's keys and slots once, all loads in flight together: it
 // keeps the slots and local cells in LDS and marks where the cell tile changes (the chunk falls
 // apart into runs of one tile, contiguous and cell by cell); the fixed-point bound is the
@@ -392,7 +395,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const StepCtl* __restrict__ ctl, uint32_t* __restrict__ list_ctr, uint32_t* __restrict__ tile_list,
     uint32_t list_cap, const float4* __restrict__ rec, float* __restrict__ tpart,
     float* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene, uint32_t* __restrict__ pbound,
-    int skip_heavy) {
+    uint32_t* __restrict__ chunk_lo, int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
   __shared__ double s_acc[NQ * HALO_NODES];  // integer-valued fixed-point sums
@@ -406,6 +409,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   __shared__ float s_bound[3];
   __shared__ uint32_t s_nrun;
   __shared__ uint32_t s_edge[2];
+  __shared__ uint32_t s_bnd[2];
 
   const int t = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -417,11 +421,29 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   const uint32_t step = ctl->step;
   const int sid = chunk_scene[chunk];
   const SceneDev& S = scenes[sid];
-  const uint32_t lo = uint32_t(S.p_off) + (chunk - S.chunk_off) * S.seg;
   const uint32_t end = uint32_t(S.p_off + S.p_cnt);
-  const uint32_t hi = min(end, lo + S.seg);
-  const uint32_t len = hi - lo;
-  if (t == 0) {  // tiles of the particles just before and after the chunk (same scene only)
+  const uint32_t w0 = uint32_t(S.p_off) + (chunk - S.chunk_off) * S.seg;  // < end
+  const uint32_t w1 = min(end, w0 + S.seg);
+  uint32_t lo = w0, hi = w1;
+  if (S.align_w) {  // uniform: both bounds to the first tile start in their window, if any
+    if (t < 2) s_bnd[t] = 0xffffffffu;
+    __syncthreads();
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const uint32_t w = side ? w1 : w0;
+      // the scene's start and end stay; a searched window starts at w >= p_off + seg, so
+      // keys[q - 1] is the scene's
+      const uint32_t qend = (side ? w1 < end : w0 > uint32_t(S.p_off)) ? min(end, w + S.align_w) : w;
+      for (uint32_t q = w + uint32_t(t); q < qend; q += NT)
+        if (keys[q] / TILE_CELLS != keys[q - 1] / TILE_CELLS) atomicMin(&s_bnd[side], q);
+    }
+    __syncthreads();
+    if (s_bnd[0] != 0xffffffffu) lo = s_bnd[0];
+    if (s_bnd[1] != 0xffffffffu) hi = s_bnd[1];
+  }
+  const uint32_t len = hi - lo;  // <= seg + align_w - 1 <= P2G_SEG_MAX
+  if (t == 0) {
+    chunk_lo[chunk] = lo;  // tiles of the particles just before and after the chunk (same scene only)
     s_edge[0] = lo > uint32_t(S.p_off) ? keys[lo - 1] / TILE_CELLS : 0xffffffffu;
     s_edge[1] = hi < end ? keys[hi] / TILE_CELLS : 0xffffffffu;
   }
@@ -641,18 +663,26 @@ __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restri
                                                        const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ tile_lo,
                                                        const uint32_t* __restrict__ tile_hi,
+                                                       const uint32_t* __restrict__ chunk_lo,
                                                        float* __restrict__ tpart,
                                                        const float* __restrict__ hpart) {
   const uint32_t chunk = blockIdx.x;
   const SceneDev& S = scenes[chunk_scene[chunk]];
   const uint32_t pbase = uint32_t(S.p_off);
-  const uint32_t lo = pbase + (chunk - S.chunk_off) * S.seg;
-  const uint32_t hi = min(uint32_t(S.p_off + S.p_cnt), lo + S.seg);
+  const uint32_t end = uint32_t(S.p_off + S.p_cnt);
+  const uint32_t j = chunk - S.chunk_off;
+  const uint32_t nch = (end - pbase + S.seg - 1) / S.seg;
+  const uint32_t lo = chunk_lo[chunk];  // the bounds the P2G used
+  const uint32_t hi = j + 1 < nch ? chunk_lo[chunk + 1] : end;
   const uint32_t tile = keys[lo] / TILE_CELLS;
   if (tile == S.dead_tile) return;
   const uint32_t tlo = tile_lo[tile], thi = tile_hi[tile];
   if (tlo == lo || thi > hi) return;  // no continuation here, or a later chunk folds
-  const uint32_t c_first = S.chunk_off + (tlo - pbase) / S.seg + 1u;
+  // the chunk holding tlo: the one of its window, or the one before if the window's bound moved
+  // past tlo
+  uint32_t jt = (tlo - pbase) / S.seg;
+  if (tlo < chunk_lo[S.chunk_off + jt]) --jt;
+  const uint32_t c_first = S.chunk_off + jt + 1u;
   float* dst = tpart + size_t(tile) * (NQ * HALO_NODES);
   for (int i = threadIdx.x; i < NQ * HALO_NODES; i += blockDim.x) {
     double acc = double(dst[i]);
