@@ -20,9 +20,13 @@ Prints ONE JSON line.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-errors] [--no-sweep]
                     [--no-second-form] [--no-stress] [--large]
 
-roofline.traffic is the HBM traffic per launch of the dominant kernel measured with rocprofv3
-FETCH_SIZE / WRITE_SIZE counters (calibrated against a plain copy), read from the committed summary
-profiles/snow_step_two_snowballs_1M_128_hbm_traffic.md; null when the summary has no entry for it.
+Kernel times come from HIP events around the reference's stages. The stage named k_p2g_chunk
+("Transfer to Grid") spans k_p2g_chunk and k_fold_partials; `launches` lists what each measured
+stage spans, and roofline.valu_floor counts the instructions of all of them.
+roofline.traffic is the HBM traffic per step of the dominant stage's launches measured with
+rocprofv3 FETCH_SIZE / WRITE_SIZE counters (calibrated against a plain copy), read from the
+committed summary profiles/snow_step_two_snowballs_1M_128_hbm_traffic.md; null when the summary
+has no entry for one of them.
 """
 from __future__ import annotations
 
@@ -53,6 +57,13 @@ KERNEL_NAMES = {
     "#Sim:Compute Derivatives": "k_grid",
     "#Sim:Transfer to Particles": "k_g2p_advance",
 }
+# The launches each stage timer spans: the P2G stage ("Transfer to Grid") also adds the
+# continuation partials of tiles split between chunks (k_fold_partials, a few us).
+STAGE_LAUNCHES = {
+    "k_p2g_chunk": ("k_p2g_chunk", "k_fold_partials"),
+    "k_grid": ("k_grid",),
+    "k_g2p_advance": ("k_g2p_advance",),
+}
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "snow_step_two_snowballs_1M_128_hbm_traffic.md")
@@ -73,8 +84,16 @@ def measured_traffic(kernel: str):
     return None
 
 
+def stage_traffic(kernel: str):
+    """HBM bytes per step of the launches the stage timer of `kernel` spans (STAGE_LAUNCHES), or None
+    when the summary lacks one of them."""
+    parts = [measured_traffic(k) for k in STAGE_LAUNCHES.get(kernel, (kernel,))]
+    return None if any(p is None for p in parts) else sum(parts)
+
+
 INSTR_SUMMARY = os.path.join(ROOT, "profiles", "snow_step_two_snowballs_1M_128_instruction_counters.md")
 PLAIN_STEP_KERNEL = {"k_g2p_advance": "snow::k_g2p_advance<false, false>", "k_p2g_chunk": "snow::k_p2g_chunk",
+                     "k_fold_partials": "snow::k_fold_partials",
                      "k_grid": "snow::k_grid<false, true>"}  # the instantiations a plain FP32 step launches
 VALU_CYCLES = 4          # cycles per wave64 FP32 VALU instruction (16 lanes per SIMD)
 SIMDS = 256 * 4          # MI355X: 256 CUs of 4 SIMDs
@@ -82,22 +101,25 @@ CLOCK_HZ = 2.4e9         # peak engine clock
 
 
 def valu_floor(kernel: str, measured_us: float):
-    """The vector-ALU floor of `kernel` on this workload: its SQ_INSTS_VALU per launch from the committed
-    rocprofv3 counter summary, at VALU_CYCLES per instruction on every SIMD at CLOCK_HZ, against the
-    measured launch time. None when the summary has no row for the kernel."""
+    """The vector-ALU floor of the stage measured as `kernel` on this workload: the SQ_INSTS_VALU per
+    step of the launches it spans (STAGE_LAUNCHES) from the committed rocprofv3 counter summary, at
+    VALU_CYCLES per instruction on every SIMD at CLOCK_HZ, against the measured time. None when the
+    summary has no row for one of them."""
+    rows = {}
     try:
         with open(INSTR_SUMMARY) as f:
             for line in f:
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cells) < 3 or cells[0] != PLAIN_STEP_KERNEL.get(kernel):
-                    continue
-                valu = int(cells[2])
-                floor_us = valu * VALU_CYCLES / (SIMDS * CLOCK_HZ) * 1e6
-                return {"instructions": valu, "floor_us": floor_us, "measured_us": measured_us,
-                        "frac": floor_us / measured_us}
+                if len(cells) >= 3:
+                    rows[cells[0]] = cells[2]
+        names = [PLAIN_STEP_KERNEL.get(k) for k in STAGE_LAUNCHES.get(kernel, (kernel,))]
+        if any(nm not in rows for nm in names):
+            return None
+        valu = sum(int(rows[nm]) for nm in names)
     except (OSError, ValueError):
-        pass
-    return None
+        return None
+    floor_us = valu * VALU_CYCLES / (SIMDS * CLOCK_HZ) * 1e6
+    return {"instructions": valu, "floor_us": floor_us, "measured_us": measured_us, "frac": floor_us / measured_us}
 
 
 def cpu_baseline(scene, budget_s: float = 12.0):
@@ -372,7 +394,8 @@ def main():
             continue
         avg_s = ms / calls / 1e3
         alg = bp * n + bn * active
-        kernels[KERNEL_NAMES[st]] = {"avg_us": avg_s * 1e6, "alg_bytes": alg, "GBps": alg / avg_s / 1e9}
+        kernels[KERNEL_NAMES[st]] = {"avg_us": avg_s * 1e6, "alg_bytes": alg, "GBps": alg / avg_s / 1e9,
+                                     "launches": list(STAGE_LAUNCHES[KERNEL_NAMES[st]])}
         vf = valu_floor(KERNEL_NAMES[st], avg_s * 1e6)
         if vf:
             kernels[KERNEL_NAMES[st]]["valu_floor"] = vf
@@ -397,9 +420,9 @@ def main():
         "data": "synthetic (seeded SplitMix64 particle sampling)",
         "config": {"workload": "two_snowballs_128^3", "particles": n, "grid": [128, 128, 128], "dt": dt,
                    "form": args.form, "active_nodes": active},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
-                     "valu_floor": kernels[dom].get("valu_floor")},
+        "roofline": {"bound": "hbm", "kernel": dom, "launches": kernels[dom]["launches"], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": stage_traffic(dom), "valu_floor": kernels[dom].get("valu_floor")},
         "step_roofline_frac": (296 * n + 92 * active) * args.steps / el / (HBM_PEAK_GBS * 1e9),
         "kernels": kernels,
     }

@@ -31,6 +31,27 @@ def test_measured_traffic_reads_committed_summary():
     assert bench.measured_traffic("no_such_kernel") is None
 
 
+def test_stage_figures_cover_every_launch_of_the_stage():
+    """The P2G stage timer spans the P2G and the fold of continuation partials: its traffic and VALU
+    floor are the sums over both launches."""
+    sys.path.insert(0, ROOT)
+    import bench
+    assert bench.STAGE_LAUNCHES["k_p2g_chunk"] == ("k_p2g_chunk", "k_fold_partials")
+    p2g, fold = bench.measured_traffic("k_p2g_chunk"), bench.measured_traffic("k_fold_partials")
+    if p2g is not None and fold is not None:
+        assert bench.stage_traffic("k_p2g_chunk") == p2g + fold
+    assert bench.stage_traffic("k_g2p_advance") == bench.measured_traffic("k_g2p_advance")
+    rows = {}
+    with open(bench.INSTR_SUMMARY) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 3 and cells[0].startswith("snow::"):
+                rows[cells[0]] = int(cells[2])
+    vf = bench.valu_floor("k_p2g_chunk", 100.0)
+    assert vf["instructions"] == rows["snow::k_p2g_chunk"] + rows["snow::k_fold_partials"]
+    assert bench.valu_floor("k_grid", 100.0)["instructions"] == rows["snow::k_grid<false, true>"]
+
+
 def test_valu_floor_reads_the_instruction_profile():
     """roofline.valu_floor prices the committed SQ_INSTS_VALU of a kernel at 4 cycles per wave64
     instruction on 1024 SIMDs at 2.4 GHz, against the measured launch time."""
