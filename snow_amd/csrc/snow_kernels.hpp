@@ -443,7 +443,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   }
   const uint32_t len = hi - lo;  // <= seg + align_w - 1 <= P2G_SEG_MAX
   if (t == 0) {
-    chunk_lo[chunk] = lo;  // tiles of the particles just before and after the chunk (same scene only)
+    chunk_lo[chunk] = lo;  // for the fold of continuation partials
+    // tiles of the particles just before and after the chunk (same scene only)
     s_edge[0] = lo > uint32_t(S.p_off) ? keys[lo - 1] / TILE_CELLS : 0xffffffffu;
     s_edge[1] = hi < end ? keys[hi] / TILE_CELLS : 0xffffffffu;
   }
