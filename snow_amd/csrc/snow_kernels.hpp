@@ -742,12 +742,18 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
     const uint32_t y = __shfl_up(incl, off, 64);
     if (lane >= off) incl += y;
   }
-  const uint32_t total = __shfl(incl, 63, 64);
-  for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
-    // (uniform values: the loads below become scalar loads)
+  const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+  // cell tile of list entry e (uniform values: the loads become scalar loads)
+  auto entry = [&](uint32_t e) -> uint32_t {
     const int sub = __builtin_amdgcn_readfirstlane(__ffsll((long long)__ballot(incl > e)) - 1);
-    const uint32_t first = __builtin_amdgcn_readfirstlane(__shfl(incl - cnt, sub, 64));
-    const uint32_t k = tile_list[size_t(sub) * list_cap + (e - first)];
+    const uint32_t first = uint32_t(__builtin_amdgcn_readlane(int(incl - cnt), sub));
+    return tile_list[size_t(sub) * list_cap + (e - first)];
+  };
+  uint32_t k_next = blockIdx.x < total ? entry(blockIdx.x) : 0u;
+  for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
+    const uint32_t k = k_next;
+    // the next entry's list slot loads while this tile is merged
+    if (e + gridDim.x < total) k_next = entry(e + gridDim.x);
     const int sid = tile_scene ? tile_scene[k] : 0;  // no table: one scene, whose record loads at once
     const SceneDev& S = scenes[sid];
     const uint32_t kl = k - S.ct_off;
