@@ -369,6 +369,30 @@ template <class R> inline void particle_in_grid(const Scene<R>& s, Vec3<R> x, Ve
   for (int k = 0; k < 3; ++k) base[k] = int(u[k]);  // C++ int() truncates toward zero like GLSL
 }
 
+// The factors of weighting3(d) and weightingGradient3(-d) for the 64 stencil nodes of one
+// particle, tabulated per axis: d[k] = (base[k] + o) - u[k] for the offsets o = -1..2 of getIJK.
+// weighting() is even in its argument (|x| and x*x only), so weighting(-d) = weighting(d) bit
+// for bit, and the per-node products below are the same products, in the same order, as the
+// vec3 overloads (cubic.include.glsl:36-40) evaluate: w = (wx*wy)*wz, grad.x = (gx*wy)*wz,
+// grad.y = (wx*gy)*wz, grad.z = (wx*wy)*gz.
+template <class R> struct StencilWeights {
+  R w[3][4], g[3][4];
+  StencilWeights(Vec3<R> u, const int base[3]) {
+    for (int k = 0; k < 3; ++k)
+      for (int o = 0; o < 4; ++o) {
+        const R d = R(base[k] + o - 1) - u[k];
+        w[k][o] = weighting(d);
+        g[k][o] = weightingGradient(-d);
+      }
+  }
+  // o: getIJK offsets in {-1..2}
+  R weight(const int o[3]) const { return w[0][o[0] + 1] * w[1][o[1] + 1] * w[2][o[2] + 1]; }
+  Vec3<R> gradient(const int o[3]) const {
+    const R wx = w[0][o[0] + 1], wy = w[1][o[1] + 1], wz = w[2][o[2] + 1];
+    return v3(g[0][o[0] + 1] * wy * wz, wx * g[1][o[1] + 1] * wz, wx * wy * g[2][o[2] + 1]);
+  }
+};
+
 // gReset.glsl:11-16 (bounds-checked)
 template <class R> void grid_reset(Scene<R>& s) {
   int64_t nn = s.nodes();
@@ -405,20 +429,20 @@ template <class R> void p2g_scatter(Scene<R>& s, bool mass_only, const std::vect
       Vec3<R> vp = v3(s.v[3*p], s.v[3*p+1], s.v[3*p+2]);
       Vec3<R> u; int base[3];
       particle_in_grid(s, s.xp(p), u, base);
+      const StencilWeights<R> sw(u, base);
       for (int sidx = 0; sidx < 64; ++sidx) {
         int o[3]; getIJK(sidx, o);
         int gi[3] = {base[0] + o[0], base[1] + o[1], base[2] + o[2]};
         if (!(gi[0] >= 0 && gi[1] >= 0 && gi[2] >= 0 && gi[0] < Dx && gi[1] < Dy && gi[2] < Dz)) continue;
         if (gi[2] < k0 || gi[2] >= k1) continue;
-        Vec3<R> d = v3(R(gi[0]) - u.x, R(gi[1]) - u.y, R(gi[2]) - u.z);
-        R wip = weighting3(d);
+        R wip = sw.weight(o);  // weighting3(gi - u)
         int64_t gI = gi[0] + int64_t(gi[1]) * Dx + int64_t(gi[2]) * Dx * Dy;
         s.gm[gI] += mp * wip;
         if (mass_only) continue;
         Vec3<R> vel = vp * mp * wip;
         s.gmv[3*gI] += vel.x; s.gmv[3*gI+1] += vel.y; s.gmv[3*gI+2] += vel.z;
         if (s.rho0[p] > R(0)) {  // p2gTransfer.glsl:126
-          Vec3<R> wipg = weightingGradient3(v3(-d.x, -d.y, -d.z)) * s.gscale;
+          Vec3<R> wipg = sw.gradient(o) * s.gscale;  // weightingGradient3(u - gi)
           Vec3<R> force = A[p] * wipg;
           s.gf[3*gI] -= force.x; s.gf[3*gI+1] -= force.y; s.gf[3*gI+2] -= force.z;
         }
@@ -448,12 +472,25 @@ template <class R> void p2g(Scene<R>& s, bool mass_only) {
     });
   }
   auto zb = z_buckets(s);
-  int T = std::max(1, std::min(s.nthreads, s.D[2]));
-  parallel_for(T, T, [&](int, int64_t tb, int64_t te) {
-    for (int64_t t = tb; t < te; ++t) {
-      int k0 = int(int64_t(s.D[2]) * t / T), k1 = int(int64_t(s.D[2]) * (t + 1) / T);
-      p2g_scatter(s, mass_only, A, zb, k0, k1);
+  // Thread t owns the node layers [kb[t], kb[t+1]), placed so that every thread visits about the
+  // same number of particles. Each node is summed by one thread, bucket by bucket in increasing
+  // base z and particle index order, so the sums do not depend on where the boundaries fall.
+  const int T = std::max(1, std::min(s.nthreads, s.D[2]));
+  std::vector<int> kb(T + 1, s.D[2]);
+  {
+    std::vector<int64_t> cum(s.D[2] + 1, 0);  // particles with base z < k (layers -2..D)
+    for (int k = 0; k < s.D[2]; ++k) cum[k + 1] = cum[k] + int64_t(zb[k + 2].size()) + (k == 0 ? int64_t(zb[0].size() + zb[1].size()) : 0);
+    const int64_t tot = cum[s.D[2]];
+    kb[0] = 0;
+    int k = 0;
+    for (int t = 1; t < T; ++t) {
+      const int64_t want = tot * t / T;
+      while (k < s.D[2] && cum[k] < want) ++k;
+      kb[t] = std::max(kb[t - 1], k);
     }
+  }
+  parallel_for(T, T, [&](int, int64_t tb, int64_t te) {
+    for (int64_t t = tb; t < te; ++t) p2g_scatter(s, mass_only, A, zb, kb[t], kb[t + 1]);
   });
 }
 
@@ -465,13 +502,13 @@ template <class R> void particle_volume(Scene<R>& s) {
     for (int64_t p = b; p < e; ++p) {
       Vec3<R> u; int base[3];
       particle_in_grid(s, s.xp(p), u, base);
+      const StencilWeights<R> sw(u, base);
       R acc = 0;
       for (int sidx = 0; sidx < 64; ++sidx) {
         int o[3]; getIJK(sidx, o);
         int gi[3] = {base[0] + o[0], base[1] + o[1], base[2] + o[2]};
         if (!(gi[0] >= 0 && gi[1] >= 0 && gi[2] >= 0 && gi[0] < Dx && gi[1] < Dy && gi[2] < Dz)) continue;
-        Vec3<R> d = v3(R(gi[0]) - u.x, R(gi[1]) - u.y, R(gi[2]) - u.z);
-        R wip = weighting3(d);
+        R wip = sw.weight(o);  // weighting3(gi - u)
         int64_t gI = gi[0] + int64_t(gi[1]) * Dx + int64_t(gi[2]) * Dx * Dy;
         acc += s.gm[gI] * wip / cell;
       }
@@ -525,15 +562,15 @@ template <class R> void g2p_advance_particle(Scene<R>& s, int64_t p, R dt) {
   Vec3<R> xp = s.xp(p);
   Vec3<R> u; int base[3];
   particle_in_grid(s, xp, u, base);
+  const StencilWeights<R> sw(u, base);
   Vec3<R> pvn = v3(R(0), R(0), R(0));
   Mat3<R> dvp = Mat3<R>::zero();
   for (int sidx = 0; sidx < 64; ++sidx) {
     int o[3]; getIJK(sidx, o);
     int gi[3] = {base[0] + o[0], base[1] + o[1], base[2] + o[2]};
     if (!(gi[0] >= 0 && gi[1] >= 0 && gi[2] >= 0 && gi[0] < Dx && gi[1] < Dy && gi[2] < Dz)) continue;
-    Vec3<R> d = v3(R(gi[0]) - u.x, R(gi[1]) - u.y, R(gi[2]) - u.z);
-    R wip = weighting3(d);
-    Vec3<R> gwip = weightingGradient3(v3(-d.x, -d.y, -d.z)) * s.gscale;
+    R wip = sw.weight(o);                           // weighting3(gi - u)
+    Vec3<R> gwip = sw.gradient(o) * s.gscale;       // weightingGradient3(u - gi)
     int64_t gI = gi[0] + int64_t(gi[1]) * Dx + int64_t(gi[2]) * Dx * Dy;
     Vec3<R> vin = v3(s.gvstar[3*gI], s.gvstar[3*gI+1], s.gvstar[3*gI+2]);
     Vec3<R> vi = v3(s.gmv[3*gI], s.gmv[3*gI+1], s.gmv[3*gI+2]);
