@@ -4,7 +4,7 @@
  * This library replaces the GLSL compute-shader dispatch chain of the explicit
  * snow MPM step (reference: src/snow/simulation/timeUpdate/explicittimeupdate.cpp:4-63)
  * and the SSBO buffer interface it reads and writes (src/snow/object/particlesystem.cpp:101-228,
- * src/snow/object/grid.cpp:181-255, src/snow/object/collisionObjects.cpp:25-88).
+ * src/snow/object/grid.cpp:74-148, src/snow/object/collisionObjects.cpp:25-88).
  *
  * All functions return 0 on success or a negative SNOW_E* code; snow_last_error()
  * returns the message of the last failure on a context (or of a failed snow_create*).
@@ -13,7 +13,7 @@
  * snow_read_*, snow_sync and snow_create*.
  *
  * Matrices are 3x3, column-major (element (row r, col c) at index c*3+r), the
- * order GLSL uses for mat3/mat4 (particle.hpp:8-11, particleAdvance.glsl:230).
+ * order GLSL uses for mat3/mat4 (particle.hpp:8-11, particleAdvance.glsl:82-85).
  */
 #ifndef SNOW_H
 #define SNOW_H
@@ -56,7 +56,7 @@ typedef struct snow_material {
 } snow_material;
 
 /* Dense node grid: node (i,j,k) sits at origin + spacing*(i,j,k), index
- * i + j*dim[0] + k*dim[0]*dim[1] (grid.cpp:205-215, gridIndex.include.glsl:7-9). */
+ * i + j*dim[0] + k*dim[0]*dim[1] (grid.cpp:98-108, gridIndex.include.glsl:7-9). */
 typedef struct snow_grid {
   int32_t dim[3];
   float origin[3];
@@ -109,7 +109,7 @@ void snow_material_defaults(snow_material* out);
 void snow_grid_defaults(snow_grid* out);
 
 /* Replaces ExplicitTimeUpdate's constructor + ParticleSystem/Grid/CollisionObjects::initSSBO
- * (explicittimeupdate.hpp:14-25, particlesystem.cpp:101-228, grid.cpp:181-255,
+ * (explicittimeupdate.hpp:14-25, particlesystem.cpp:101-228, grid.cpp:74-148,
  * collisionObjects.cpp:25-66). A single scene is a batch of one. */
 int snow_create(const snow_scene_desc* scene, uint32_t flags, snow_ctx** out);
 int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_t flags,
@@ -147,13 +147,19 @@ int snow_autotune(snow_ctx* ctx, int32_t steps, int32_t* g2p_block, int32_t* p2g
 
 /* ---- per-stage entry points (the Technique::dispatch of each compute shader) ---- */
 int snow_stage_grid_mass(snow_ctx* ctx);        /* gMassCompute.glsl:33-84 (init)      */
-int snow_stage_particle_volume(snow_ctx* ctx);  /* pVolumeCompute.glsl:116-166 (init)  */
+int snow_stage_particle_volume(snow_ctx* ctx);  /* pVolumeCompute.glsl:32-83 (init)    */
 int snow_stage_rigid_advance(snow_ctx* ctx, double dt); /* rigidAdvance.glsl:14-18    */
 int snow_stage_grid_reset(snow_ctx* ctx);       /* gReset.glsl:11-16                   */
 int snow_stage_p2g(snow_ctx* ctx);              /* p2gTransfer.glsl:44-151             */
 int snow_stage_grid_update(snow_ctx* ctx, double dt); /* gForceCompute.glsl:57-103     */
 int snow_stage_g2p(snow_ctx* ctx);              /* g2pTransfer.glsl:53-143 (+ advance) */
 int snow_stage_particle_advance(snow_ctx* ctx, double dt); /* particleAdvance.glsl:75-217 */
+/* g2pTransfer.glsl:53-143 and particleAdvance.glsl:75-217 as the one fused launch snow_step runs
+ * (it also computes the next P2G's force matrices and cell keys), on the current particles and the
+ * grid velocities of the last grid stage or snow_write_grid. The reference dispatches the two
+ * shaders separately (explicittimeupdate.cpp:55-62); this entry point exists so the fused kernel
+ * can be run from a given state. It also sets the count of snow_fp64_particles. */
+int snow_stage_g2p_advance(snow_ctx* ctx, double dt);
 
 /* ---- state transfer --------------------------------------------------------- */
 
@@ -165,7 +171,7 @@ int snow_read_particles(snow_ctx* ctx, int32_t scene, float* x, float* v, float*
 int snow_write_particles(snow_ctx* ctx, int32_t scene, const float* x, const float* v,
                          const float* fe, const float* fp, const float* rho0);
 /* Dense grid fields of one scene ([D^3] m, [D^3][3] mv, f, vstar), as left by the last
- * P2G / grid update (gv, gf, gvn buffers, grid.cpp:226-254). NULL fields are skipped. */
+ * P2G / grid update (gv, gf, gvn buffers, grid.cpp:119-147). NULL fields are skipped. */
 int snow_read_grid(snow_ctx* ctx, int32_t scene, float* m, float* mv, float* f, float* vstar);
 /* Overwrite the dense grid fields read by the grid update and G2P stages. */
 int snow_write_grid(snow_ctx* ctx, int32_t scene, const float* m, const float* mv,

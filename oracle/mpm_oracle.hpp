@@ -10,10 +10,10 @@
 //   weighting / weightingGradient ..... shader/compute/interpolation/cubic.include.glsl:7-40
 //   getIJK / getIndex .................. shader/compute/indexing/neighborIndex.include.glsl:6-12,
 //                                        shader/compute/indexing/gridIndex.include.glsl:7-9
-//   mu / lambda ........................ shader/compute/lame/lame.include.glsl:69-85
+//   mu / lambda ........................ shader/compute/lame/lame.include.glsl:11-27
 //   SVD (McAdams et al. 2011) ........... shader/compute/svd/SVD.include.glsl:24-385
 //   grid mass / particle volume ........ shader/compute/mpm/gMassCompute.glsl:33-84,
-//                                        shader/compute/mpm/pVolumeCompute.glsl:116-166
+//                                        shader/compute/mpm/pVolumeCompute.glsl:32-83
 //   P2G ................................ shader/compute/mpm/p2gTransfer.glsl:44-151
 //   grid update + collisions ........... shader/compute/mpm/gForceCompute.glsl:40-103
 //   G2P ................................ shader/compute/mpm/g2pTransfer.glsl:53-143
@@ -131,7 +131,7 @@ inline void getIJK(int index, int ijk[3]) {
   ijk[2] = index / (width * width) - 1;
 }
 
-// ---- lame.include.glsl:69-85 ----------------------------------------------------
+// ---- lame.include.glsl:11-27 ----------------------------------------------------
 template <class R> inline R mu(R young, R poisson, R hardening, R JP) {
   return young / (R(2) * (R(1) + poisson)) * std::exp(hardening * (R(1) - JP));
 }
@@ -279,7 +279,7 @@ template <class R> inline void computePD(const Mat3<R>& A, Mat3<R>& Rot) {
   Rot = W * transpose(V);
 }
 
-// ---- collisions: gForceCompute.glsl:40-53, particleAdvance.glsl:203-216 -------------
+// ---- collisions: gForceCompute.glsl:40-53, particleAdvance.glsl:60-73 -------------
 template <class R> struct Collider { Vec3<R> pos; R type; Vec3<R> vel; R radius; Vec3<R> normal; R friction; };
 
 // Returns whether x lies in collider c; sets n to the contact normal.
@@ -307,7 +307,7 @@ template <class R> inline bool collides(Vec3<R> x, const Collider<R>& c, Vec3<R>
   return false;
 }
 
-// Sequential Coulomb response of gForceCompute.glsl:72-93 / particleAdvance.glsl:325-347.
+// Sequential Coulomb response of gForceCompute.glsl:72-93 / particleAdvance.glsl:182-204.
 template <class R> inline Vec3<R> collide_velocity(Vec3<R> x, Vec3<R> vin, const Collider<R>* cs, int nc) {
   for (int i = 0; i < nc; ++i) {
     Vec3<R> n = cs[i].normal;
@@ -494,7 +494,7 @@ template <class R> void p2g(Scene<R>& s, bool mass_only) {
   });
 }
 
-// pVolumeCompute.glsl:116-166: rho_p0 = sum_i m_i w_ip / h^3.
+// pVolumeCompute.glsl:32-83: rho_p0 = sum_i m_i w_ip / h^3.
 template <class R> void particle_volume(Scene<R>& s) {
   const int Dx = s.D[0], Dy = s.D[1], Dz = s.D[2];
   const R cell = s.h * s.h * s.h;
@@ -582,7 +582,7 @@ template <class R> void g2p_advance_particle(Scene<R>& s, int64_t p, R dt) {
         for (int r = 0; r < 3; ++r) dvp.m[c][r] += vin[r] * gwip[c];
     }
   }
-  // particleAdvance.glsl:225-283
+  // particleAdvance.glsl:95-140
   Mat3<R> FEp = s.FE(p), FPp = s.FP(p);
   Mat3<R> L = Mat3<R>::ident() + dt * dvp;
   Mat3<R> FEpn = L * FEp;
@@ -603,7 +603,7 @@ template <class R> void g2p_advance_particle(Scene<R>& s, int64_t p, R dt) {
     for (int j = 0; j < 3; ++j) FPpn.m[i][j] = round_to(FPpn.m[i][j], R(1e9));
   std::memcpy(s.fe + 9*p, FEpn.m, sizeof(FEpn.m));
   std::memcpy(s.fp + 9*p, FPpn.m, sizeof(FPpn.m));
-  // particleAdvance.glsl:318-359
+  // particleAdvance.glsl:173-210
   Vec3<R> vp = v3(s.v[3*p], s.v[3*p+1], s.v[3*p+2]);
   Vec3<R> vpn1 = vp * alpha + pvn;
   vpn1 = collide_velocity(xp, vpn1, s.col, s.nc);  // pre-advection position (Q11)

@@ -79,7 +79,7 @@ class Grid:
         return g
 
     def node_positions(self) -> np.ndarray:
-        """Node positions x0 + h*(i,j,k), x fastest (grid.cpp:205-215)."""
+        """Node positions x0 + h*(i,j,k), x fastest (grid.cpp:98-108)."""
         k, j, i = np.meshgrid(np.arange(self.dimz), np.arange(self.dimy), np.arange(self.dimx), indexing="ij")
         pos = np.stack([self.x_off + self.h * i, self.y_off + self.h * j, self.z_off + self.h * k], -1)
         return pos.reshape(-1, 3).astype(np.float32)

@@ -2,7 +2,7 @@
 //
 // Replaces ExplicitTimeUpdate (src/snow/simulation/timeUpdate/explicittimeupdate.cpp:4-63), its
 // eight Technique wrappers (src/snow/simulation/computingTechnique/*.cpp) and the SSBO set-up of
-// ParticleSystem / Grid / CollisionObjects (particlesystem.cpp:101-228, grid.cpp:181-255,
+// ParticleSystem / Grid / CollisionObjects (particlesystem.cpp:101-228, grid.cpp:74-148,
 // collisionObjects.cpp:25-66). One context owns one HIP stream and all device memory of a
 // batch of scenes; every launch of a step covers all scenes at once.
 #include <hip/hip_runtime.h>
@@ -70,11 +70,11 @@ struct snow_ctx {
 
   SceneDev* d_scenes = nullptr;
   uint16_t *d_pscene = nullptr, *d_tile_scene = nullptr, *d_ntile_scene = nullptr;
-  float4* d_rec[2] = {nullptr, nullptr};
+  float4* d_rec[2] = {nullptr, nullptr};  // 2 float4 per particle: (x, m), (v, rho0)
   float* d_fe[2] = {nullptr, nullptr};
   float* d_fp[2] = {nullptr, nullptr};
-  float* d_rho[2] = {nullptr, nullptr};
   uint32_t* d_pid[2] = {nullptr, nullptr};
+  float* d_fa = nullptr;  // force matrices, 9 planes, in the slot order of the current buffer
   uint32_t* d_keys[2] = {nullptr, nullptr};
   uint32_t* d_slot = nullptr;  // slot of the particle at each sorted position (the sort's permutation)
   void* d_sort_tmp = nullptr;
@@ -125,6 +125,10 @@ struct snow_ctx {
   uint32_t total_chunks = 0;       // P2G workgroups (chunks of every scene)
   uint16_t* d_chunk_scene = nullptr;
   uint32_t* d_chunk_lo = nullptr;  // first sorted position of each P2G chunk, as the P2G chose it
+  // G2P waves: each scene's sorted range cut into waves of 64 positions, (first position, scene)
+  // per wave; only for batches of several scenes (one scene: wave w starts at 64 w)
+  uint2* d_waves = nullptr;
+  uint32_t nwaves = 0;
 };
 
 namespace {
@@ -275,8 +279,8 @@ int publish_bounds(snow_ctx* c, uint32_t ahead, int skip_heavy) {
   int rc;
   if ((rc = clear_bounds(c))) return rc;
   if (c->N) {
-    k_particle_bounds<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->d_ctl,
-                                                                  c->d_pbound, ahead, skip_heavy, c->N);
+    k_particle_bounds<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->d_fa,
+                                                                  c->d_ctl, c->d_pbound, ahead, skip_heavy, c->N);
     LAUNCHCHK(c);
   }
   c->bound_valid = true;
@@ -301,7 +305,7 @@ int run_force_prepass(snow_ctx* c) {
   if (c->N) {
     const int k = c->cur;
     k_force_prepass<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[k], c->d_fe[k],
-                                                                 c->d_fp[k], c->d_rho[k], c->N, c->d_ctl, c->d_pbound);
+                                                                 c->d_fp[k], c->d_fa, c->N, c->d_ctl, c->d_pbound);
     LAUNCHCHK(c);
   }
   c->force_valid = true;
@@ -337,8 +341,8 @@ int run_p2g(snow_ctx* c, int mass_only) {
   if (c->total_chunks) {
     k_p2g_chunk<<<c->total_chunks, 64 * P2G_WAVES, 0, c->stream>>>(
         c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_slot, c->d_tile_lo, c->d_tile_hi, c->d_tile_stamp,
-        c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_tpart, c->d_hpart,
-        c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, mass_only ? 0 : 1);
+        c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_fa, c->N, c->d_tpart,
+        c->d_hpart, c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, mass_only ? 0 : 1);
     LAUNCHCHK(c);
     k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
                                                             c->d_tile_lo, c->d_tile_hi, c->d_chunk_lo, c->d_tpart,
@@ -376,12 +380,12 @@ int run_g2p_advance(snow_ctx* c, float dt) {
   const bool mixed = (c->flags & SNOW_FLAG_MIXED) != 0;
   const bool comp = (c->flags & SNOW_FLAG_COMPENSATED_G2P) != 0;
   if (c->N) {
-#define G2P_ARGS                                                                                        \
-  c->d_scenes, c->d_pscene, c->d_slot, c->d_rec[a], c->d_fe[a], c->d_fp[a], c->d_rho[a], c->d_pid[a], \
-      c->d_rec[b], c->d_fe[b], c->d_fp[b], c->d_rho[b], c->d_pid[b], c->d_keys[0],                        \
-      c->d_g_node, c->d_cols, dt, c->N, f64, c->d_ctl, c->d_pbound
+#define G2P_ARGS                                                                                          \
+  c->d_scenes, c->nscenes > 1 ? c->d_waves : nullptr, c->nwaves, c->d_slot, c->d_rec[a], c->d_fe[a], c->d_fp[a], \
+      c->d_pid[a], c->d_rec[b], c->d_fe[b], c->d_fp[b], c->d_pid[b], c->d_fa, c->d_keys[0], c->d_g_node,       \
+      c->d_cols, dt, c->N, f64, c->d_ctl, c->d_pbound
     const unsigned bs = unsigned(c->g2p_block);
-    const unsigned g = grid_for(c->N, bs);
+    const unsigned g = grid_for(c->nwaves, bs / 64u);
     if (mixed && comp) k_g2p_advance<true, true><<<g, bs, 0, c->stream>>>(G2P_ARGS);
     else if (mixed) k_g2p_advance<true, false><<<g, bs, 0, c->stream>>>(G2P_ARGS);
     else if (comp) k_g2p_advance<false, true><<<g, bs, 0, c->stream>>>(G2P_ARGS);
@@ -419,9 +423,9 @@ void release(snow_ctx* c) {
     if (c->graph[p]) (void)hipGraphDestroy(c->graph[p]);
   }
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
-                  c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_rho[0], c->d_rho[1], c->d_pid[0],
+                  c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_fa, c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp,
-                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
+                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -475,7 +479,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     if (validate_scene(scenes[s], s, msg)) return fail(nullptr, SNOW_E_ARG, msg);
     ntot += scenes[s].num_particles;
   }
-  if (ntot >= (int64_t(1) << 31)) return fail(nullptr, SNOW_E_CAPACITY, "more than 2^31 particles in a batch");
+  // SoA planes are addressed with 32-bit byte offsets in the kernels
+  if (ntot >= (int64_t(1) << 30)) return fail(nullptr, SNOW_E_CAPACITY, "2^30 or more particles in a batch");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(nullptr, SNOW_E_NODEVICE, "no HIP device available");
@@ -584,13 +589,13 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_tile_scene, c->total_ctiles));
   CREATECHK(dalloc(&c->d_ntile_scene, std::max<uint32_t>(1, c->total_ntiles)));
   for (int b = 0; b < 2; ++b) {
-    CREATECHK(dalloc(&c->d_rec[b], size_t(4) * N));
+    CREATECHK(dalloc(&c->d_rec[b], size_t(2) * N));
     CREATECHK(dalloc(&c->d_fe[b], size_t(9) * N));
     CREATECHK(dalloc(&c->d_fp[b], size_t(9) * N));
-    CREATECHK(dalloc(&c->d_rho[b], N));
     CREATECHK(dalloc(&c->d_pid[b], N));
     CREATECHK(dalloc(&c->d_keys[b], N));
   }
+  CREATECHK(dalloc(&c->d_fa, size_t(9) * N));
   CREATECHK(dalloc(&c->d_slot, N));
   {
     size_t tb = 0;
@@ -605,6 +610,17 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_hpart, size_t(std::max<uint32_t>(1, c->total_chunks)) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_chunk_scene, std::max<uint32_t>(1, c->total_chunks)));
   CREATECHK(dalloc(&c->d_chunk_lo, std::max<uint32_t>(1, c->total_chunks)));
+  {
+    std::vector<uint2> waves;
+    for (int s = 0; s < num_scenes; ++s) {
+      const SceneDev& S = c->scenes[s];
+      for (int64_t q = 0; q < S.p_cnt; q += 64) waves.push_back(make_uint2(uint32_t(S.p_off + q), uint32_t(s)));
+    }
+    c->nwaves = uint32_t(waves.size());
+    CREATECHK(dalloc(&c->d_waves, std::max<size_t>(1, waves.size())));
+    if (!waves.empty())
+      CREATECHK(hipMemcpy(c->d_waves, waves.data(), sizeof(uint2) * waves.size(), hipMemcpyHostToDevice));
+  }
   CREATECHK(dalloc(&c->d_zero_part, size_t(NQ) * HALO_NODES));
   CREATECHK(dalloc(&c->d_list_ctr, size_t(TILE_LISTS) * TILE_LIST_STRIDE));
   CREATECHK(hipMemset(c->d_list_ctr, 0, sizeof(uint32_t) * TILE_LISTS * TILE_LIST_STRIDE));
@@ -646,9 +662,9 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     if (!c->cols.empty())
       CREATECHK(hipMemcpy(c->d_cols, c->cols.data(), sizeof(Collider) * c->cols.size(), hipMemcpyHostToDevice));
   }
-  // particle upload (creation order = slot order, pid = identity)
+  // particle upload (creation order = slot order, pid = identity; rest density 0 until snow_init)
   {
-    std::vector<float4> rec(size_t(4) * N);
+    std::vector<float4> rec(size_t(2) * N);
     std::vector<float> fe(size_t(9) * N), fp(size_t(9) * N);
     std::vector<uint32_t> pid(N);
     for (int s = 0; s < num_scenes; ++s) {
@@ -656,10 +672,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
       const int64_t off = c->scenes[s].p_off;
       for (int64_t k = 0; k < d.num_particles; ++k) {
         const size_t i = size_t(off + k);
-        rec[4 * i] = make_float4(d.x[3 * k], d.x[3 * k + 1], d.x[3 * k + 2], d.mass[k]);
-        rec[4 * i + 1] = make_float4(0.f, 0.f, 0.f, d.v[3 * k]);
-        rec[4 * i + 2] = make_float4(0.f, 0.f, 0.f, d.v[3 * k + 1]);
-        rec[4 * i + 3] = make_float4(0.f, 0.f, 0.f, d.v[3 * k + 2]);
+        rec[2 * i] = make_float4(d.x[3 * k], d.x[3 * k + 1], d.x[3 * k + 2], d.mass[k]);
+        rec[2 * i + 1] = make_float4(d.v[3 * k], d.v[3 * k + 1], d.v[3 * k + 2], 0.f);
         for (int e = 0; e < 9; ++e) {
           const float id = (e == 0 || e == 4 || e == 8) ? 1.0f : 0.0f;
           fe[size_t(e) * N + i] = d.fe ? d.fe[9 * k + e] : id;
@@ -673,7 +687,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
       CREATECHK(hipMemcpy(c->d_fe[0], fe.data(), sizeof(float) * fe.size(), hipMemcpyHostToDevice));
       CREATECHK(hipMemcpy(c->d_fp[0], fp.data(), sizeof(float) * fp.size(), hipMemcpyHostToDevice));
       CREATECHK(hipMemcpy(c->d_pid[0], pid.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice));
-      CREATECHK(hipMemset(c->d_rho[0], 0, sizeof(float) * N));
+      CREATECHK(hipMemset(c->d_fa, 0, sizeof(float) * 9 * size_t(N)));
     }
   }
   CREATECHK(hipMemset(c->d_tile_lo, 0, sizeof(uint32_t) * c->total_ctiles));
@@ -715,8 +729,8 @@ int snow_stage_grid_mass(snow_ctx* c) {
   // A = 0 during the mass pass: zero the force part of the records (gMassCompute only adds m)
   if ((rc = clear_bounds(c))) return rc;
   if (c->N) {
-    k_zero_force<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->N,
-                                                             c->d_ctl, c->d_pbound);
+    k_zero_force<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur], c->d_fa,
+                                                             c->N, c->d_ctl, c->d_pbound);
     LAUNCHCHK(c);
   }
   c->bound_valid = true;
@@ -735,7 +749,7 @@ int snow_stage_particle_volume(snow_ctx* c) {
     return fail(c, SNOW_E_STATE, "particle volume needs grid masses (run snow_stage_grid_mass)");
   if (c->N) {
     k_particle_volume<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[c->cur],
-                                                                   c->d_g_mmv, c->d_rho[c->cur], c->N);
+                                                                   c->d_g_mmv, c->N);
     LAUNCHCHK(c);
   }
   c->force_valid = false;
@@ -842,8 +856,8 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
   const int k = c->cur;
   struct Buf { void* dst; const void* src; size_t bytes; };
   std::vector<Buf> bufs = {
-      {nullptr, c->d_rec[k], sizeof(float4) * 4 * N}, {nullptr, c->d_fe[k], sizeof(float) * 9 * N},
-      {nullptr, c->d_fp[k], sizeof(float) * 9 * N},  {nullptr, c->d_rho[k], sizeof(float) * N},
+      {nullptr, c->d_rec[k], sizeof(float4) * 2 * N}, {nullptr, c->d_fe[k], sizeof(float) * 9 * N},
+      {nullptr, c->d_fp[k], sizeof(float) * 9 * N},  {nullptr, c->d_fa, sizeof(float) * 9 * N},
       {nullptr, c->d_pid[k], sizeof(uint32_t) * N},   {nullptr, c->d_keys[0], sizeof(uint32_t) * N},
       {nullptr, c->d_ctl, sizeof(StepCtl)},
       {nullptr, c->d_cols, sizeof(Collider) * std::max<size_t>(1, c->cols.size())}};
@@ -860,6 +874,10 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
   }
   auto restore = [&]() -> int {
     for (auto& b : bufs) HIPCHK(c, hipMemcpyAsync(const_cast<void*>(b.src), b.dst, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+    // The restored control block rewinds the step stamp, so a later step can carry the number a
+    // trial already stamped tiles with: clear the stamps (0 is never a step's stamp).
+    k_fill_u32<<<grid_for(c->total_ctiles, 256), 256, 0, c->stream>>>(c->d_tile_stamp, 0u, int64_t(c->total_ctiles));
+    LAUNCHCHK(c);
     c->cur = k;
     c->keys_valid = true;
     c->force_valid = true;
@@ -968,7 +986,9 @@ int snow_stage_grid_update(snow_ctx* c, double dt) {
   if (!c) return SNOW_E_ARG;
   const float fdt = float(dt);
   if (!(fdt > 0.0f)) return fail(c, SNOW_E_ARG, "dt must be > 0");
-  if (c->grid_state == 1) return run_grid(c, true, true, fdt);
+  // after snow_stage_p2g: the tile merge + update of snow_step (the dense node fields the P2G stage
+  // kept stay valid)
+  if (c->grid_state == 1) return run_grid(c, false, true, fdt);
   if (c->grid_state != 2) return fail(c, SNOW_E_STATE, "grid update needs a P2G result or a written grid");
   StageTimer tm(c, ST_GRID);
   if (c->total_nodes) {
@@ -1004,9 +1024,8 @@ int snow_stage_particle_advance(snow_ctx* c, double dt) {
   const int k = c->cur;
   if (c->N) {
     k_advance_only<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_scenes, c->d_pscene, c->d_rec[k], c->d_fe[k],
-                                                                c->d_fp[k], c->d_rho[k], c->d_aux, c->d_cols, fdt,
-                                                                c->d_keys[0], c->N, c->d_ctl,
-                                                                c->d_pbound);
+                                                                c->d_fp[k], c->d_fa, c->d_aux, c->d_cols, fdt,
+                                                                c->d_keys[0], c->N, c->d_ctl, c->d_pbound);
     LAUNCHCHK(c);
   }
   c->keys_valid = true;
@@ -1014,6 +1033,27 @@ int snow_stage_particle_advance(snow_ctx* c, double dt) {
   c->bound_valid = true;
   c->aux_valid = false;
   return tm.done();
+}
+
+int snow_stage_g2p_advance(snow_ctx* c, double dt) {
+  if (!c) return SNOW_E_ARG;
+  const float fdt = float(dt);
+  if (!(fdt > 0.0f) || !std::isfinite(fdt)) return fail(c, SNOW_E_ARG, "dt must be > 0");
+  if (c->grid_state == 0) return fail(c, SNOW_E_STATE, "G2P needs grid velocities");
+  int rc;
+  if ((rc = run_keys(c))) return rc;
+  // the cell sort of the current positions gives the slot permutation the fused kernel applies;
+  // it runs without the step start, so the grid of the last P2G stays readable
+  if (c->N) {
+    size_t tb = c->sort_tmp_bytes;
+    HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], nullptr, c->d_slot, c->N,
+                               c->key_bits, c->sort_bits, c->stream, c->num_cu));
+  }
+  // the FP64-path counter of this call; the next P2G's bound from this call's records alone
+  k_fill_u32<<<1, 64, 0, c->stream>>>(reinterpret_cast<uint32_t*>(c->d_counts + c->nscenes), 0u, 2);
+  LAUNCHCHK(c);
+  if ((rc = clear_bounds(c))) return rc;
+  return run_g2p_advance(c, fdt);
 }
 
 int snow_read_particles(snow_ctx* c, int32_t s, float* x, float* v, float* fe, float* fp, float* rho0) {
@@ -1026,9 +1066,8 @@ int snow_read_particles(snow_ctx* c, int32_t s, float* x, float* v, float* fe, f
   float* st = static_cast<float*>(c->d_stage);
   float *ox = st, *ov = ox + 3 * n, *ofe = ov + 3 * n, *ofp = ofe + 9 * n, *orho = ofp + 9 * n, *om = orho + n;
   const int k = c->cur;
-  k_gather_out<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_rec[k], c->d_fe[k], c->d_fp[k], c->d_rho[k],
-                                                         c->d_pid[k], c->scenes[s].p_off, n, c->N, ox, ov, ofe,
-                                                         ofp, orho, om);
+  k_gather_out<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_rec[k], c->d_fe[k], c->d_fp[k], c->d_pid[k],
+                                                         c->scenes[s].p_off, n, c->N, ox, ov, ofe, ofp, orho, om);
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (x) HIPCHK(c, hipMemcpy(x, ox, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
@@ -1057,9 +1096,9 @@ int snow_write_particles(snow_ctx* c, int32_t s, const float* x, const float* v,
   if (fp) { HIPCHK(c, hipMemcpy(ifp, fp, sizeof(float) * 9 * n, hipMemcpyHostToDevice)); mask |= 8; }
   if (rho0) { HIPCHK(c, hipMemcpy(irho, rho0, sizeof(float) * n, hipMemcpyHostToDevice)); mask |= 16; }
   const int k = c->cur;
-  k_scatter_in<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_rec[k], c->d_fe[k], c->d_fp[k], c->d_rho[k],
-                                                         c->d_pid[k], c->scenes[s].p_off, n, c->N, ix, iv, ife,
-                                                         ifp, irho, nullptr, mask);
+  k_scatter_in<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_rec[k], c->d_fe[k], c->d_fp[k], c->d_pid[k],
+                                                         c->scenes[s].p_off, n, c->N, ix, iv, ife, ifp, irho, nullptr,
+                                                         mask);
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (mask & 1) c->keys_valid = false;
@@ -1144,7 +1183,7 @@ int snow_import_reference_particles(int64_t n, const float* pxm, const int32_t* 
   for (int64_t i = 0; i < n; ++i) {
     if (x) { x[3 * i] = pxm[4 * i]; x[3 * i + 1] = pxm[4 * i + 1]; x[3 * i + 2] = pxm[4 * i + 2]; }
     if (mass) mass[i] = pxm[4 * i + 3];
-    // pv.xyz = v * 1e8 (particleAdvance.glsl:349); decode as p2gTransfer.glsl:67 does
+    // pv.xyz = v * 1e8 (particleAdvance.glsl:206); decode as p2gTransfer.glsl:67 does
     if (v) for (int k = 0; k < 3; ++k) v[3 * i + k] = float(pv[4 * i + k]) * 1e-8f;
     for (int cc = 0; cc < 3; ++cc)
       for (int r = 0; r < 3; ++r) {
@@ -1193,7 +1232,8 @@ int snow_check_finite(snow_ctx* c) {
   if (!c->N) return 0;
   HIPCHK(c, hipMemsetAsync(c->d_bad, 0, sizeof(int), c->stream));
   const int k = c->cur;
-  k_check_finite<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_rec[k], c->d_fe[k], c->d_fp[k], c->N, c->d_bad);
+  k_check_finite<<<grid_for(c->N, 256), 256, 0, c->stream>>>(c->d_rec[k], c->d_fe[k], c->d_fp[k], c->d_fa, c->N,
+                                                              c->d_bad);
   LAUNCHCHK(c);
   int bad = 0;
   HIPCHK(c, hipMemcpyAsync(&bad, c->d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1270,8 +1310,8 @@ int snow_positions_view(snow_ctx* c, int32_t scene, const float** device_ptr, in
                         int64_t* count) {
   if (!c || !device_ptr || scene < 0 || scene >= c->nscenes) return SNOW_E_ARG;
   const SceneDev& S = c->scenes[scene];
-  *device_ptr = reinterpret_cast<const float*>(c->d_rec[c->cur] + 4 * size_t(S.p_off));
-  if (stride_bytes) *stride_bytes = int64_t(4 * sizeof(float4));
+  *device_ptr = reinterpret_cast<const float*>(c->d_rec[c->cur] + 2 * size_t(S.p_off));
+  if (stride_bytes) *stride_bytes = int64_t(2 * sizeof(float4));
   if (count) *count = S.p_cnt;
   return 0;
 }

@@ -2,7 +2,7 @@
 //
 // Restates, for registers on CDNA4, the GLSL include files of the reference:
 //   (cubic B-spline weights, cubic.include.glsl:7-40, are in snow_kernels.hpp: spline_w/q)
-//   Lame parameters with hardening  shader/compute/lame/lame.include.glsl:69-85
+//   Lame parameters with hardening  shader/compute/lame/lame.include.glsl:11-27
 //   3x3 SVD / polar decomposition .. shader/compute/svd/SVD.include.glsl:24-385
 //                                   (McAdams et al. 2011, UW-Madison TR1690: quaternion
 //                                   Jacobi eigenanalysis of A^T A, 4 sweeps, then a Givens QR)
@@ -39,7 +39,7 @@ template <class R> __device__ __forceinline__ R recip(R x);
 template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <> __device__ __forceinline__ double recip<double>(double x) { return 1.0 / x; }
 // round(scale*x)/scale of the reference shaders (SVD.include.glsl:356, p2gTransfer.glsl:114,
-// particleAdvance.glsl:265/281), with the division by the constant done as a multiplication.
+// particleAdvance.glsl:122/138), with the division by the constant done as a multiplication.
 template <class R> __device__ __forceinline__ R round_to(R x, R scale, R inv_scale) { return rnd(scale * x) * inv_scale; }
 
 // C = A * B (column-major 3x3)
@@ -74,7 +74,7 @@ __device__ __forceinline__ R det3(const R* m) {
          m[6] * (m[1] * m[5] - m[4] * m[2]);
 }
 
-// ---- Lame parameters (lame.include.glsl:69-85) -----------------------------------------
+// ---- Lame parameters (lame.include.glsl:11-27) -----------------------------------------
 template <class R>
 __device__ __forceinline__ R hardening_factor(R xi, R JP);
 // FP32: the hardware exp2 (__expf, a few ulp for the small arguments xi (1 - J_P) here; GLSL's
@@ -284,7 +284,7 @@ __device__ __forceinline__ void force_matrix(const R* FE, const R* FP, R m, R rh
   force_matrix_wv(FE, FP, W, V, m, rho0, mu0, lambda0, xi, gs, Aout);
 }
 
-// ---- colliders (gForceCompute.glsl:40-93, particleAdvance.glsl:203-216) ----------------
+// ---- colliders (gForceCompute.glsl:40-93, particleAdvance.glsl:60-73/182-204) ----------------
 struct Collider {
   float px, py, pz, type;
   float vx, vy, vz, radius;

@@ -2,10 +2,11 @@
 //
 // Data layout (all scenes of a batch concatenated; see snow_capi.hip for allocation):
 //  particles, ping-pong buffers in "sorted" order after each step:
-//    rec[i]   : 4 x float4 = { x y z m | A00 A01 A02 vx | A10 A11 A12 vy | A20 A21 A22 vz }
-//               (A = per-particle force matrix, row-major, f_node = -A grad w)
+//    rec[i]   : 2 x float4 = { x y z m | vx vy vz rho0 } (rho0: rest density, pv.w of the reference)
 //    fe/fp    : 9 SoA float planes, plane e = c*3+r (column-major 3x3)
-//    rho0,pid : rest density (pv.w of the reference), original index
+//    pid      : original (creation) index
+//  one buffer, in the slot order of the current particle buffer (written by G2P, read by P2G only):
+//    fa       : 9 SoA float planes of the force matrix A, plane r*3+c = A(r, c), f_node = -A grad w
 //  grid, dense per scene (node i + j*Dx + k*Dx*Dy):
 //    g_node   : 3 x float2 per node: (v*.x v*.y | v*.z u.x | u.y u.z), u = v* - alpha p/m,
 //               the gForceCompute output gvn and the FLIP/PIC blend term read together by G2P
@@ -52,6 +53,16 @@ struct SceneDev {
   uint32_t align_w;   // P2G chunk bounds move to the first cell-tile start within this many
                       // positions of p_off + c*seg (0: they stay there)
 };
+
+// Element i of an SoA plane addressed by the byte offset 4*i in 32 bits (a batch holds fewer than
+// 2^30 particles, checked at creation): with the plane base uniform, every plane access is a
+// scalar base plus one per-lane offset register, instead of a 64-bit address per plane.
+__device__ __forceinline__ float ld_plane(const float* __restrict__ base, uint32_t i) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (i << 2));
+}
+__device__ __forceinline__ void st_plane(float* __restrict__ base, uint32_t i, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (i << 2)) = v;
+}
 
 // Grid coordinate of a position: u = (x - origin)/h evaluated as x*(1/h) - origin/h (one FMA).
 __device__ __forceinline__ float grid_u(const SceneDev& S, float x, int k) { return fmaf(x, S.inv_h, S.uo[k]); }
@@ -111,7 +122,7 @@ __global__ void __launch_bounds__(256) k_keys(const SceneDev* __restrict__ scene
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SceneDev& S = scenes[pscene[i]];
-  const float4 xm = rec[4 * size_t(i)];
+  const float4 xm = rec[2 * size_t(i)];
   keys[i] = cell_key(S, xm.x, xm.y, xm.z);
 }
 
@@ -221,16 +232,17 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
 // adds tpart[tile] and the hpart of the tile's later chunks in chunk order.
 struct P2GBound { float m, p, f; };
 
-__device__ __forceinline__ P2GBound particle_bound(const float4& r0, const float4& r1, const float4& r2,
-                                                   const float4& r3, int skip_heavy) {
+// The largest per-particle terms of the P2G sums: |m|, |m v_k| and the absolute row sums of A
+// (a row-major 3x3), which bound |A grad w| for |grad w| components <= 1 times grad_scale.
+__device__ __forceinline__ P2GBound particle_bound(float m, float vx, float vy, float vz, const float Arm[9],
+                                                   int skip_heavy) {
   P2GBound b;
-  const float m = r0.w;
   const bool skip = skip_heavy && fabsf(m) > 100.0f;
   b.m = skip ? 0.f : fabsf(m);
-  b.p = skip ? 0.f : fmaxf(fabsf(r1.w * m), fmaxf(fabsf(r2.w * m), fabsf(r3.w * m)));
-  const float f0 = fabsf(r1.x) + fabsf(r1.y) + fabsf(r1.z);
-  const float f1 = fabsf(r2.x) + fabsf(r2.y) + fabsf(r2.z);
-  const float f2 = fabsf(r3.x) + fabsf(r3.y) + fabsf(r3.z);
+  b.p = skip ? 0.f : fmaxf(fabsf(vx * m), fmaxf(fabsf(vy * m), fabsf(vz * m)));
+  const float f0 = fabsf(Arm[0]) + fabsf(Arm[1]) + fabsf(Arm[2]);
+  const float f1 = fabsf(Arm[3]) + fabsf(Arm[4]) + fabsf(Arm[5]);
+  const float f2 = fabsf(Arm[6]) + fabsf(Arm[7]) + fabsf(Arm[8]);
   b.f = skip ? 0.f : fmaxf(f0, fmaxf(f1, f2));
   return b;
 }
@@ -279,15 +291,15 @@ __device__ __forceinline__ int wave_ext_i(int v) {
 __device__ __forceinline__ int wave_min_i(int v) { return wave_ext_i<false>(v); }
 __device__ __forceinline__ int wave_max_i(int v) { return wave_ext_i<true>(v); }
 
-// wave_full (uniform): all 64 lanes of the calling wave are live; otherwise each lane publishes
-// its own bound. NaN and infinite terms count as the largest float.
-__device__ __forceinline__ void publish_bound(uint32_t* __restrict__ pb, uint32_t parity, int sid, P2GBound b,
-                                              bool wave_full) {
+// A wave whose 64 lanes are all active and of one scene publishes its maxima once (DPP
+// reductions); otherwise each active lane publishes its own bound. NaN and infinite terms count
+// as the largest float.
+__device__ __forceinline__ void publish_bound(uint32_t* __restrict__ pb, uint32_t parity, int sid, P2GBound b) {
   float m = b.m <= __FLT_MAX__ ? b.m : __FLT_MAX__;
   float p = b.p <= __FLT_MAX__ ? b.p : __FLT_MAX__;
   float f = b.f <= __FLT_MAX__ ? b.f : __FLT_MAX__;
   const uint32_t slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % PB_SLOTS;
-  if (wave_full) {
+  if (__ballot(true) == ~0ull) {
     const int sid0 = __builtin_amdgcn_readfirstlane(sid);
     if (__ballot(sid != sid0) == 0ull) {
       // values in [0, FLT_MAX]: their bit patterns order as the floats do
@@ -309,31 +321,22 @@ __device__ __forceinline__ void publish_bound(uint32_t* __restrict__ pb, uint32_
   if (f > 0.f) atomicMax(sl + 2, __float_as_uint(f));
 }
 
-// The same, reduced over the workgroup first (4x fewer global atomics for 256-thread blocks):
-// s_pb (4 words, zeroed before a barrier at kernel start, before any thread returns) collects
-// the block maxima and counts the waves done; the last live wave publishes them. Threads
-// i >= n have returned (whole waves, the live lanes of a wave being its low lanes). Blocks
-// spanning several scenes (one_scene false, from block_one_scene at kernel start, so the load
-// is not at the end of every wave) publish wave by wave.
-__device__ __forceinline__ bool block_one_scene(const uint16_t* __restrict__ pscene, uint32_t n) {
-  const uint32_t b0 = blockIdx.x * blockDim.x;
-  const uint32_t last = min(n, b0 + blockDim.x) - 1u;
-  return b0 >= n || pscene[b0] == pscene[last];
-}
-
+// The same, reduced over the workgroup first (4x fewer global atomics for 256-thread blocks) when
+// all its waves hold one scene (one_scene, uniform): s_pb (4 words, zeroed before a barrier at
+// kernel start, before any thread returns) collects the block maxima and counts the waves done,
+// and the last of the block's `waves` live waves publishes them. Otherwise each wave publishes
+// on its own.
 __device__ __forceinline__ void publish_bound_block(uint32_t* __restrict__ pb, uint32_t parity, int sid, P2GBound b,
-                                                    uint32_t* s_pb, bool one_scene, uint32_t i, uint32_t n) {
-  const uint32_t b0 = blockIdx.x * blockDim.x;
-  const uint32_t live = min(n, b0 + blockDim.x) - b0;
+                                                    uint32_t* s_pb, bool one_scene, uint32_t waves) {
   if (!one_scene) {
-    publish_bound(pb, parity, sid, b, (i | 63u) < n);
+    publish_bound(pb, parity, sid, b);
     return;
   }
   float m = b.m <= __FLT_MAX__ ? b.m : __FLT_MAX__;
   float p = b.p <= __FLT_MAX__ ? b.p : __FLT_MAX__;
   float f = b.f <= __FLT_MAX__ ? b.f : __FLT_MAX__;
   const int lane = threadIdx.x & 63;
-  if ((i | 63u) < n) {
+  if (__ballot(true) == ~0ull) {
     // values in [0, FLT_MAX]: their bit patterns order as the floats do
     m = __int_as_float(wave_max_i(__float_as_int(m)));
     p = __int_as_float(wave_max_i(__float_as_int(p)));
@@ -343,13 +346,13 @@ __device__ __forceinline__ void publish_bound_block(uint32_t* __restrict__ pb, u
       atomicMax(&s_pb[1], __float_as_uint(p));
       atomicMax(&s_pb[2], __float_as_uint(f));
     }
-  } else {  // the partial last wave of the launch
+  } else {  // a partial wave (its live lanes are its low lanes)
     atomicMax(&s_pb[0], __float_as_uint(m));
     atomicMax(&s_pb[1], __float_as_uint(p));
     atomicMax(&s_pb[2], __float_as_uint(f));
   }
   __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's maxima are in LDS
-  if (lane == 0 && atomicAdd(&s_pb[3], 1u) == (live + 63u) / 64u - 1u) {
+  if (lane == 0 && atomicAdd(&s_pb[3], 1u) == waves - 1u) {
     const uint32_t slot = blockIdx.x % PB_SLOTS;
     uint32_t* sl = bound_slots(pb, sid, parity) + slot * PB_STRIDE;
     const uint32_t bm = s_pb[0], bp = s_pb[1], bf = s_pb[2];
@@ -359,29 +362,43 @@ __device__ __forceinline__ void publish_bound_block(uint32_t* __restrict__ pb, u
   }
 }
 
-// Bound of a particle record (x, m | A row 0, vx | A row 1, vy | A row 2, vz) whose cell key is
-// key; zero when the particle is out of the grid.
-__device__ __forceinline__ P2GBound record_bound(const SceneDev& S, uint32_t key, const float4& r0, const float4& r1,
-                                                 const float4& r2, const float4& r3, int skip_heavy) {
+// Bound of a particle (mass m, velocity v, row-major force matrix Arm) whose cell key is key;
+// zero when the particle is out of the grid.
+__device__ __forceinline__ P2GBound record_bound(const SceneDev& S, uint32_t key, float m, float vx, float vy,
+                                                 float vz, const float Arm[9], int skip_heavy) {
   if (key / TILE_CELLS == S.dead_tile) return P2GBound{0.f, 0.f, 0.f};
-  return particle_bound(r0, r1, r2, r3, skip_heavy);
+  return particle_bound(m, vx, vy, vz, Arm, skip_heavy);
+}
+
+// Force matrix of particle slot i from the planes of fa (row-major order).
+__device__ __forceinline__ void load_force(const float* __restrict__ fa, size_t n, uint32_t i, float Arm[9]) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Arm[e] = ld_plane(fa + e * n, i);
+}
+// Store of a column-major force matrix A into the row-major planes of fa.
+__device__ __forceinline__ void store_force(float* __restrict__ fa, size_t n, uint32_t i, const float A[9]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) st_plane(fa + (r * 3 + c) * n, i, A[c * 3 + r]);
 }
 
 // Fallback publisher for records no writer has published (host uploads, restored snapshots):
 // parity (step + ahead) & 1.
 __global__ void __launch_bounds__(256) k_particle_bounds(const SceneDev* __restrict__ scenes,
                                                          const uint16_t* __restrict__ pscene,
-                                                         const float4* __restrict__ rec, const StepCtl* __restrict__ ctl,
-                                                         uint32_t* __restrict__ pb, uint32_t ahead, int skip_heavy,
-                                                         uint32_t n) {
+                                                         const float4* __restrict__ rec, const float* __restrict__ fa,
+                                                         const StepCtl* __restrict__ ctl, uint32_t* __restrict__ pb,
+                                                         uint32_t ahead, int skip_heavy, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int sid = pscene[i];
   const SceneDev& S = scenes[sid];
-  const float4 r0 = rec[4 * size_t(i)], r1 = rec[4 * size_t(i) + 1], r2 = rec[4 * size_t(i) + 2],
-               r3 = rec[4 * size_t(i) + 3];
-  const P2GBound b = record_bound(S, cell_key(S, r0.x, r0.y, r0.z), r0, r1, r2, r3, skip_heavy);
-  publish_bound(pb, (ctl->step + ahead) & 1u, sid, b, (i | 63u) < n);
+  const float4 xm = rec[2 * size_t(i)], vr = rec[2 * size_t(i) + 1];
+  float Arm[9];
+  load_force(fa, n, i, Arm);
+  const P2GBound b = record_bound(S, cell_key(S, xm.x, xm.y, xm.z), xm.w, vr.x, vr.y, vr.z, Arm, skip_heavy);
+  publish_bound(pb, (ctl->step + ahead) & 1u, sid, b);
 }
 
 constexpr int P2G_CHUNK = 32;        // particles staged per wave at a time
@@ -393,9 +410,9 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
     uint32_t* __restrict__ tile_lo, uint32_t* __restrict__ tile_hi, uint32_t* __restrict__ tile_stamp,
     const StepCtl* __restrict__ ctl, uint32_t* __restrict__ list_ctr, uint32_t* __restrict__ tile_list,
-    uint32_t list_cap, const float4* __restrict__ rec, float* __restrict__ tpart,
-    float* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene, uint32_t* __restrict__ pbound,
-    uint32_t* __restrict__ chunk_lo, int skip_heavy) {
+    uint32_t list_cap, const float4* __restrict__ rec, const float* __restrict__ fa, uint32_t np,
+    float* __restrict__ tpart, float* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene,
+    uint32_t* __restrict__ pbound, uint32_t* __restrict__ chunk_lo, int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
   __shared__ double s_acc[NQ * HALO_NODES];  // integer-valued fixed-point sums
@@ -529,16 +546,16 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     nc = __builtin_amdgcn_readfirstlane(nc);
     ne = __builtin_amdgcn_readfirstlane(ne);
   };
-  float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0, q3 = q0;
+  float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;  // (x, m), (v, rho0)
+  float qa[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // A, row-major
   uint32_t qcell = 0u;
   auto fetch = [&]() {
     if (nr < nrun && uint32_t(lane) < min(uint32_t(P2G_CHUNK), ne - nc)) {
       const uint32_t p = s_val[nc + lane];
       qcell = s_cell[nc + lane];
-      q0 = rec[4 * size_t(p)];
-      q1 = rec[4 * size_t(p) + 1];
-      q2 = rec[4 * size_t(p) + 2];
-      q3 = rec[4 * size_t(p) + 3];
+      q0 = rec[2 * size_t(p)];
+      q1 = rec[2 * size_t(p) + 1];
+      load_force(fa, np, p, qa);
     }
   };
   seek(0);
@@ -588,10 +605,10 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
           d4[3] = make_float4(spline_w(2, fy), spline_q(2, fy), spline_w(3, fy), spline_q(3, fy));
           d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
           d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
-          d4[6] = make_float4(m, q1.w * m, q2.w * m, q3.w * m);
-          d4[7] = make_float4(q1.x * use, q1.y * use, q1.z * use, q2.x * use);
-          d4[8] = make_float4(q2.y * use, q2.z * use, q3.x * use, q3.y * use);
-          d4[9] = make_float4(q3.z * use, 0.f, 0.f, 0.f);
+          d4[6] = make_float4(m, q1.x * m, q1.y * m, q1.z * m);
+          d4[7] = make_float4(qa[0] * use, qa[1] * use, qa[2] * use, qa[3] * use);
+          d4[8] = make_float4(qa[4] * use, qa[5] * use, qa[6] * use, qa[7] * use);
+          d4[9] = make_float4(qa[8] * use, 0.f, 0.f, 0.f);
         }
       }
       // advance the cursor and load the next batch's records while this one is summed
@@ -871,7 +888,7 @@ __global__ void __launch_bounds__(256) k_grid_import(const SceneDev* __restrict_
     store_node(g_node, g_mass, node, vs.x, vs.y, vs.z, vs.x - S.alpha * (mm.x / m), vs.y - S.alpha * (mm.y / m),
                vs.z - S.alpha * (mm.z / m), m);
   } else {
-    store_node(g_node, g_mass, node, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, m);  // gvn = 0 where m <= 0 (gForceCompute.glsl:251-253)
+    store_node(g_node, g_mass, node, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, m);  // gvn = 0 where m <= 0 (gForceCompute.glsl:100-102)
   }
 }
 
@@ -1035,6 +1052,14 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
 // box use g2p_gather. The sums are the same as g2p_gather's, in the same order.
 constexpr int G2P_BOX = 384;
 
+// The per-lane gather of g2p_gather_wave's fallback, kept out of line: it is rare, and inlined it
+// holds registers through the wave path.
+__device__ __attribute__((noinline)) void g2p_gather_lane(const SceneDev& S, float x, float y, float z,
+                                                          const float2* __restrict__ g_node, float vpn[3],
+                                                          float dvp[9]) {
+  g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
+}
+
 __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, float x, float y, float z,
                                                 const float2* __restrict__ g_node, float2* box, float vpn[3],
                                                 float dvp[9]) {
@@ -1047,7 +1072,7 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
   const int sid0 = __builtin_amdgcn_readfirstlane(sid);
   const bool full = act == ~0ull;
   if (!full || __ballot(interior && sid == sid0) != act) {
-    g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
+    g2p_gather_lane(S, x, y, z, g_node, vpn, dvp);
     return;
   }
   // node box [xmin, xmin+ex) x [ymin, ymin+ey) x [zmin, zmin+ez)
@@ -1055,7 +1080,7 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
   const int ex = wave_max_i(bx) + 3 - xmin, ey = wave_max_i(by) + 3 - ymin, ez = wave_max_i(bz) + 3 - zmin;
   const int vol = ex * ey * ez;
   if (vol > G2P_BOX) {
-    g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
+    g2p_gather_lane(S, x, y, z, g_node, vpn, dvp);
     return;
   }
   const int lane = threadIdx.x & 63;
@@ -1088,11 +1113,13 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
   for (int k = 0; k < 9; ++k) dvp[k] *= gs;  // weight gradient scale (1 in the reference)
 }
 
-// particleAdvance.glsl:225-283 in precision R: F_E, F_P update with the singular-value clamp.
+// particleAdvance.glsl:95-140 in precision R: F_E, F_P update with the singular-value clamp.
+// Wo, Vo receive the SVD factors of F_E' = (I + dt L) F_E and so[3] its diagonal factor before the
+// clamp (S[0][0], S[1][1], S[2][2]).
 template <class R>
 __device__ __forceinline__ void deform_update(const float dvp[9], float dt, const float FEf[9], const float FPf[9],
                                               float crit_c, float crit_s, float FEo[9], float FPo[9], R Wo[9],
-                                              R Vo[9]) {
+                                              R Vo[9], R so[3]) {
   R L[9], FE[9], FP[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) { L[i] = R(dt) * R(dvp[i]); FE[i] = R(FEf[i]); FP[i] = R(FPf[i]); }
@@ -1103,6 +1130,7 @@ __device__ __forceinline__ void deform_update(const float dvp[9], float dt, cons
   mul(FEn, FP, Fn);
   R W[9], S[9], V[9];
   svd3(FEn, W, S, V);
+  so[0] = S[0]; so[1] = S[4]; so[2] = S[8];
   const R lb = R(1) - R(crit_c), ub = R(1) + R(crit_s);
   S[0] = S[0] < lb ? lb : (S[0] > ub ? ub : S[0]);
   S[4] = S[4] < lb ? lb : (S[4] > ub ? ub : S[4]);
@@ -1110,7 +1138,7 @@ __device__ __forceinline__ void deform_update(const float dvp[9], float dt, cons
   R WS[9], Fe2[9];
   mul(W, S, WS);
   mul_bt(WS, V, Fe2);
-  // F_P = V diag(1/S) W^T F  (particleAdvance.glsl:270-275)
+  // F_P = V diag(1/S) W^T F  (particleAdvance.glsl:128-132)
   R VSi[9];
   const R i0 = recip(S[0]), i1 = recip(S[4]), i2 = recip(S[8]);
 #pragma unroll
@@ -1127,21 +1155,13 @@ __device__ __forceinline__ void deform_update(const float dvp[9], float dt, cons
   }
 }
 
-// Singular values of F_E' near the clamp bounds or an ill-conditioned F_E' send the
-// particle to the FP64 path (mixed precision).
-__device__ __forceinline__ bool needs_fp64(const float dvp[9], float dt, const float FE[9], float crit_c,
-                                           float crit_s) {
-  float L[9], FEn[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) L[i] = dt * dvp[i];
-  L[0] += 1.f; L[4] += 1.f; L[8] += 1.f;
-  mul(L, FE, FEn);
-  float W[9], S[9], V[9];
-  svd3(FEn, W, S, V);
+// Mixed precision: a particle whose F_E' has a singular value (the diagonal d of the FP32 update's
+// own SVD, before the clamp) within 2e-4 of 1 - theta_c or 1 + theta_s, or whose F_E' is ill-
+// conditioned (min |d| <= 1e-3 max |d|), is updated again on the FP64 path.
+__device__ __forceinline__ bool needs_fp64(const float d[3], float crit_c, float crit_s) {
   const float tol = 2e-4f;
   const float lb = 1.f - crit_c, ub = 1.f + crit_s;
   bool near = false;
-  const float d[3] = {S[0], S[4], S[8]};
 #pragma unroll
   for (int k = 0; k < 3; ++k) near |= fabsf(fabsf(d[k]) - lb) < tol || fabsf(fabsf(d[k]) - ub) < tol;
   const float smax = fmaxf(fabsf(d[0]), fmaxf(fabsf(d[1]), fabsf(d[2])));
@@ -1151,105 +1171,102 @@ __device__ __forceinline__ bool needs_fp64(const float dvp[9], float dt, const f
 }
 
 // ---------------------------------------------------------------------------------------
-// Waves per SIMD the G2P register allocation aims at: 4 for the FP32 kernel (128 VGPRs;
-// uncapped, the per-lane scene path below takes it to 135 and 3 waves, which is slower than a
-// few spills), 3 with the FP64 path compiled in (168 VGPRs: at 4 waves it spills in the hot
-// path, at 2 waves latency is exposed).
+// Waves per SIMD the G2P register allocation aims at: 4 for the FP32 kernel (128 VGPRs), 3 with
+// the FP64 path compiled in (168 VGPRs: at 4 waves it spills in the hot path, at 2 waves latency
+// is exposed).
 template <bool MIXED> struct G2PWaves { static constexpr int value = MIXED ? 3 : 4; };
 
 // Fused G2P + particle advance (g2pTransfer.glsl:53-143, particleAdvance.glsl:75-217).
 // Thread i handles sorted position i: it reads the particle from its previous slot
 // vals[i] (buffers *_in), writes the new state to slot i (buffers *_out), computes the next
-// step's force matrix (from the SVD of the F_E update) and the next step's cell key. The
-// permutation of the sort is thereby applied for free by the writes of this kernel.
+// step's force matrix (from the SVD of the F_E update) into slot i of fa and the next step's cell
+// key. The permutation of the sort is thereby applied for free by the writes of this kernel.
+// The force matrix is not read here: fa is the next P2G's input only.
+// Every wave holds particles of one scene, so the scene record is wave-uniform (scalar loads):
+// waves[w] = (first sorted position, scene) of wave w, each scene's sorted range cut into waves of
+// 64 positions (its last one partial); null for a context of one scene, whose wave w starts at 64w.
 template <bool MIXED, bool COMP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G2PWaves<MIXED>::value))) k_g2p_advance(
-    const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ pscene, const uint32_t* __restrict__ vals,
+    const SceneDev* __restrict__ scenes, const uint2* __restrict__ waves, uint32_t nwaves,
+    const uint32_t* __restrict__ vals,
     const float4* __restrict__ rec_in, const float* __restrict__ fe_in, const float* __restrict__ fp_in,
-    const float* __restrict__ rho_in, const uint32_t* __restrict__ pid_in, float4* __restrict__ rec_out,
-    float* __restrict__ fe_out, float* __restrict__ fp_out, float* __restrict__ rho_out,
-    uint32_t* __restrict__ pid_out, uint32_t* __restrict__ keys_next,
-    const float2* __restrict__ g_node, const Collider* __restrict__ cols,
+    const uint32_t* __restrict__ pid_in, float4* __restrict__ rec_out, float* __restrict__ fe_out,
+    float* __restrict__ fp_out, uint32_t* __restrict__ pid_out, float* __restrict__ fa_out,
+    uint32_t* __restrict__ keys_next, const float2* __restrict__ g_node, const Collider* __restrict__ cols,
     float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count, const StepCtl* __restrict__ ctl,
     uint32_t* __restrict__ pbound) {
   __shared__ float2 s_box[4][3 * G2P_BOX];  // one node box per wave (blocks of up to 256 threads)
   __shared__ uint32_t s_pb[4];               // block maxima of the P2G bound, waves done
+  const uint32_t wpb = blockDim.x >> 6;      // waves per block
+  const uint32_t w0 = blockIdx.x * wpb;
+  const uint32_t wb = w0 + wpb <= nwaves ? wpb : nwaves - w0;  // live waves of the block
   if (threadIdx.x < 4) s_pb[threadIdx.x] = 0u;
-  const bool one_scene = block_one_scene(pscene, n);
+  // all waves of the block in one scene (the scene of a wave is constant along the table)
+  const bool one_scene = !waves || waves[w0].y == waves[w0 + wb - 1].y;
   const uint32_t next_parity = (ctl->step + 1u) & 1u;
   __syncthreads();
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  // The body below runs with a wave-uniform scene (its record in scalar registers) when all lanes
-  // of the wave share one, which sorted particles almost always do, else with one scene per lane.
-  auto body = [&](const SceneDev& S, int sid) {
+  const uint32_t w = w0 + uint32_t(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
+  if (w >= nwaves) return;
+  const uint2 ws = waves ? waves[w] : make_uint2(64u * w, 0u);
+  const int s0 = int(ws.y);
+  const uint32_t i = ws.x + (threadIdx.x & 63);
+  if (int64_t(i) >= scenes[s0].p_off + scenes[s0].p_cnt) return;
+  {
+    const SceneDev& S = scenes[s0];
     const float dt = dt_override > 0.0f ? dt_override : S.dt;
     const uint32_t p = vals[i];
-    const float4 r0 = rec_in[4 * size_t(p)];
-    const float4 r1 = rec_in[4 * size_t(p) + 1];
-    const float4 r2 = rec_in[4 * size_t(p) + 2];
-    const float4 r3 = rec_in[4 * size_t(p) + 3];
-    const float rho0 = rho_in[p];
+    const float4 xm = rec_in[2 * size_t(p)];      // x, m
+    const float4 vr = rec_in[2 * size_t(p) + 1];  // v, rho0 (in flight through the gather)
     const uint32_t id = pid_in[p];
 
     float vpn[3], dvp[9];
     if (COMP)
-      g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, true);
+      g2p_gather(S, xm.x, xm.y, xm.z, g_node, vpn, dvp, true);
     else
-      g2p_gather_wave(S, sid, r0.x, r0.y, r0.z, g_node, s_box[threadIdx.x >> 6], vpn, dvp);
-    // F_E, F_P are loaded after the gather: not live through it, and in flight behind its LDS work
+      g2p_gather_wave(S, s0, xm.x, xm.y, xm.z, g_node, s_box[threadIdx.x >> 6], vpn, dvp);
+    // F_E, F_P are loaded after the gather: not live through it, and in flight behind the
+    // velocity update below
     float FE[9], FP[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) { FE[e] = fe_in[size_t(e) * n + p]; FP[e] = fp_in[size_t(e) * n + p]; }
+    for (int e = 0; e < 9; ++e) { FE[e] = ld_plane(fe_in + size_t(e) * n, p); FP[e] = ld_plane(fp_in + size_t(e) * n, p); }
+
+    // velocity, collisions at the pre-advection position, advection (particleAdvance.glsl:180-210);
+    // independent of the F_E/F_P update below, so done and stored first (fewer live registers)
+    float vx = vr.x * S.alpha + vpn[0], vy = vr.y * S.alpha + vpn[1], vz = vr.z * S.alpha + vpn[2];
+    collide(xm.x, xm.y, xm.z, vx, vy, vz, cols + S.col_off, S.ncol);
+    const float x = xm.x + dt * vx, y = xm.y + dt * vy, z = xm.z + dt * vz;
+    rec_out[2 * size_t(i)] = make_float4(x, y, z, xm.w);
+    rec_out[2 * size_t(i) + 1] = make_float4(vx, vy, vz, vr.w);
+    pid_out[i] = id;
+    const uint32_t key = cell_key(S, x, y, z);
+    keys_next[i] = key;
 
     // F_E, F_P update; the next step's force matrix (p2gTransfer.glsl:109-141, once per particle)
     // takes R_E = W V^T from the factors of this update's SVD: the stored F_E is W S' V^T with
     // S' the clamped triangular factor, so its own SVD gives the same W V^T.
     float FEo[9], FPo[9], A[9];
-    bool wide = false;
-    if (MIXED) wide = needs_fp64(dvp, dt, FE, S.crit_comp, S.crit_stretch);
-    if (wide) {
-      double W[9], V[9], FEd[9], FPd[9], Ad[9];
-      deform_update<double>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
+    float W[9], V[9], sv[3];
+    deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V, sv);
+    if (MIXED && needs_fp64(sv, S.crit_comp, S.crit_stretch)) {
+      double Wd[9], Vd[9], svd[3], FEd[9], FPd[9], Ad[9];
+      deform_update<double>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, Wd, Vd, svd);
 #pragma unroll
       for (int e = 0; e < 9; ++e) { FEd[e] = FEo[e]; FPd[e] = FPo[e]; }
-      force_matrix_wv<double>(FEd, FPd, W, V, double(r0.w), double(rho0), double(S.mu0), double(S.lambda0),
+      force_matrix_wv<double>(FEd, FPd, Wd, Vd, double(xm.w), double(vr.w), double(S.mu0), double(S.lambda0),
                               double(S.hardening), double(S.gscale), Ad);
 #pragma unroll
       for (int e = 0; e < 9; ++e) A[e] = float(Ad[e]);
       if (fp64_count) atomicAdd(fp64_count, 1ull);
     } else {
-      float W[9], V[9];
-      deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
-      force_matrix_wv<float>(FEo, FPo, W, V, r0.w, rho0, S.mu0, S.lambda0, S.hardening, S.gscale, A);
+      force_matrix_wv<float>(FEo, FPo, W, V, xm.w, vr.w, S.mu0, S.lambda0, S.hardening, S.gscale, A);
     }
-
-    // velocity, collisions at the pre-advection position, advection (particleAdvance.glsl:318-359)
-    float vx = r1.w * S.alpha + vpn[0], vy = r2.w * S.alpha + vpn[1], vz = r3.w * S.alpha + vpn[2];
-    collide(r0.x, r0.y, r0.z, vx, vy, vz, cols + S.col_off, S.ncol);
-    const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
-
-    // rec rows: A row r = (A[r], A[3+r], A[6+r]) from the column-major A
-    const float4 o0 = make_float4(x, y, z, r0.w), o1 = make_float4(A[0], A[3], A[6], vx),
-                 o2 = make_float4(A[1], A[4], A[7], vy), o3 = make_float4(A[2], A[5], A[8], vz);
-    rec_out[4 * size_t(i)] = o0;
-    rec_out[4 * size_t(i) + 1] = o1;
-    rec_out[4 * size_t(i) + 2] = o2;
-    rec_out[4 * size_t(i) + 3] = o3;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) { fe_out[size_t(e) * n + i] = FEo[e]; fp_out[size_t(e) * n + i] = FPo[e]; }
-    rho_out[i] = rho0;
-    pid_out[i] = id;
-    const uint32_t key = cell_key(S, x, y, z);
-    keys_next[i] = key;
-    publish_bound_block(pbound, next_parity, sid, record_bound(S, key, o0, o1, o2, o3, 1), s_pb, one_scene, i, n);
-  };
-  const int sid = pscene[i];
-  const int sid0 = __builtin_amdgcn_readfirstlane(sid);
-  if (__ballot(sid != sid0) == 0ull)
-    body(scenes[sid0], sid0);
-  else
-    body(scenes[sid], sid);
+    for (int e = 0; e < 9; ++e) { st_plane(fe_out + size_t(e) * n, i, FEo[e]); st_plane(fp_out + size_t(e) * n, i, FPo[e]); }
+    store_force(fa_out, n, i, A);
+    const float Arm[9] = {A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8]};
+    publish_bound_block(pbound, next_parity, s0, record_bound(S, key, xm.w, vx, vy, vz, Arm, 1), s_pb, one_scene,
+                        wb);
+  }
 }
 
 // Split stages for the per-stage API: G2P alone writes (vpn, dvp) per particle slot;
@@ -1262,7 +1279,7 @@ __global__ void __launch_bounds__(256) k_g2p_only(const SceneDev* __restrict__ s
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SceneDev& S = scenes[pscene[i]];
-  const float4 r0 = rec[4 * size_t(i)];
+  const float4 r0 = rec[2 * size_t(i)];
   float vpn[3], dvp[9];
   g2p_gather(S, r0.x, r0.y, r0.z, g_node, vpn, dvp, false);
 #pragma unroll
@@ -1274,7 +1291,7 @@ __global__ void __launch_bounds__(256) k_g2p_only(const SceneDev* __restrict__ s
 __global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict__ scenes,
                                                       const uint16_t* __restrict__ pscene, float4* __restrict__ rec,
                                                       float* __restrict__ fe, float* __restrict__ fp,
-                                                      const float* __restrict__ rho, const float* __restrict__ aux,
+                                                      float* __restrict__ fa, const float* __restrict__ aux,
                                                       const Collider* __restrict__ cols, float dt_override,
                                                       uint32_t* __restrict__ keys,
                                                       uint32_t n, const StepCtl* __restrict__ ctl,
@@ -1284,10 +1301,8 @@ __global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict
   const int sid = pscene[i];
   const SceneDev& S = scenes[sid];
   const float dt = dt_override > 0.0f ? dt_override : S.dt;
-  const float4 r0 = rec[4 * size_t(i)];
-  const float4 r1 = rec[4 * size_t(i) + 1];
-  const float4 r2 = rec[4 * size_t(i) + 2];
-  const float4 r3 = rec[4 * size_t(i) + 3];
+  const float4 xm = rec[2 * size_t(i)];
+  const float4 vr = rec[2 * size_t(i) + 1];
   float FE[9], FP[9], vpn[3], dvp[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) { FE[e] = fe[size_t(e) * n + i]; FP[e] = fp[size_t(e) * n + i]; }
@@ -1295,38 +1310,35 @@ __global__ void __launch_bounds__(256) k_advance_only(const SceneDev* __restrict
   for (int k = 0; k < 3; ++k) vpn[k] = aux[size_t(k) * n + i];
 #pragma unroll
   for (int k = 0; k < 9; ++k) dvp[k] = aux[size_t(3 + k) * n + i];
-  float FEo[9], FPo[9], W[9], V[9];
-  deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V);
-  float vx = r1.w * S.alpha + vpn[0], vy = r2.w * S.alpha + vpn[1], vz = r3.w * S.alpha + vpn[2];
-  collide(r0.x, r0.y, r0.z, vx, vy, vz, cols + S.col_off, S.ncol);
-  const float x = r0.x + dt * vx, y = r0.y + dt * vy, z = r0.z + dt * vz;
+  float FEo[9], FPo[9], W[9], V[9], sv[3];
+  deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V, sv);
+  float vx = vr.x * S.alpha + vpn[0], vy = vr.y * S.alpha + vpn[1], vz = vr.z * S.alpha + vpn[2];
+  collide(xm.x, xm.y, xm.z, vx, vy, vz, cols + S.col_off, S.ncol);
+  const float x = xm.x + dt * vx, y = xm.y + dt * vy, z = xm.z + dt * vz;
   float A[9];
-  force_matrix_wv<float>(FEo, FPo, W, V, r0.w, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
-  const float4 o0 = make_float4(x, y, z, r0.w), o1 = make_float4(A[0], A[3], A[6], vx),
-               o2 = make_float4(A[1], A[4], A[7], vy), o3 = make_float4(A[2], A[5], A[8], vz);
-  rec[4 * size_t(i)] = o0;
-  rec[4 * size_t(i) + 1] = o1;
-  rec[4 * size_t(i) + 2] = o2;
-  rec[4 * size_t(i) + 3] = o3;
+  force_matrix_wv<float>(FEo, FPo, W, V, xm.w, vr.w, S.mu0, S.lambda0, S.hardening, S.gscale, A);
+  rec[2 * size_t(i)] = make_float4(x, y, z, xm.w);
+  rec[2 * size_t(i) + 1] = make_float4(vx, vy, vz, vr.w);
 #pragma unroll
   for (int e = 0; e < 9; ++e) { fe[size_t(e) * n + i] = FEo[e]; fp[size_t(e) * n + i] = FPo[e]; }
+  store_force(fa, n, i, A);
   const uint32_t key = cell_key(S, x, y, z);
   keys[i] = key;
-  publish_bound(pbound, (ctl->step + 1u) & 1u, sid, record_bound(S, key, o0, o1, o2, o3, 1), (i | 63u) < n);
+  const float Arm[9] = {A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8]};
+  publish_bound(pbound, (ctl->step + 1u) & 1u, sid, record_bound(S, key, xm.w, vx, vy, vz, Arm, 1));
 }
 
 // ---------------------------------------------------------------------------------------
-// Init: rest density rho_p0 = sum_i m_i w_ip / h^3 (pVolumeCompute.glsl:116-166), gathered
-// per particle (the reference scatters with a 64-way per-particle atomic).
+// Init: rest density rho_p0 = sum_i m_i w_ip / h^3 (pVolumeCompute.glsl:32-83), gathered per
+// particle (the reference scatters with a 64-way per-particle atomic); stored in rec[2i+1].w.
 __global__ void __launch_bounds__(256) k_particle_volume(const SceneDev* __restrict__ scenes,
                                                          const uint16_t* __restrict__ pscene,
-                                                         const float4* __restrict__ rec,
-                                                         const float4* __restrict__ g_mmv,
-                                                         float* __restrict__ rho, uint32_t n) {
+                                                         float4* __restrict__ rec,
+                                                         const float4* __restrict__ g_mmv, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SceneDev& S = scenes[pscene[i]];
-  const float4 r0 = rec[4 * size_t(i)];
+  const float4 r0 = rec[2 * size_t(i)];
   const float ux = grid_u(S, r0.x, 0), uy = grid_u(S, r0.y, 1), uz = grid_u(S, r0.z, 2);
   float acc = 0.f;
   if (in_stencil_range(S, ux, uy, uz)) {
@@ -1350,15 +1362,16 @@ __global__ void __launch_bounds__(256) k_particle_volume(const SceneDev* __restr
       }
     }
   }
-  rho[i] = acc;
+  reinterpret_cast<float*>(rec)[8 * size_t(i) + 7] = acc;
 }
 
 // Force matrices of the current state (after init or host writes).
 __global__ void __launch_bounds__(256) k_force_prepass(const SceneDev* __restrict__ scenes,
-                                                       const uint16_t* __restrict__ pscene, float4* __restrict__ rec,
-                                                       const float* __restrict__ fe, const float* __restrict__ fp,
-                                                       const float* __restrict__ rho, uint32_t n,
-                                                       const StepCtl* __restrict__ ctl, uint32_t* __restrict__ pbound) {
+                                                       const uint16_t* __restrict__ pscene,
+                                                       const float4* __restrict__ rec, const float* __restrict__ fe,
+                                                       const float* __restrict__ fp, float* __restrict__ fa,
+                                                       uint32_t n, const StepCtl* __restrict__ ctl,
+                                                       uint32_t* __restrict__ pbound) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int sid = pscene[i];
@@ -1366,108 +1379,94 @@ __global__ void __launch_bounds__(256) k_force_prepass(const SceneDev* __restric
   float FE[9], FP[9], A[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) { FE[e] = fe[size_t(e) * n + i]; FP[e] = fp[size_t(e) * n + i]; }
-  const float4 r0 = rec[4 * size_t(i)];
-  force_matrix<float>(FE, FP, r0.w, rho[i], S.mu0, S.lambda0, S.hardening, S.gscale, A);
-  float4 r1 = rec[4 * size_t(i) + 1], r2 = rec[4 * size_t(i) + 2], r3 = rec[4 * size_t(i) + 3];
-  r1.x = A[0]; r1.y = A[3]; r1.z = A[6];
-  r2.x = A[1]; r2.y = A[4]; r2.z = A[7];
-  r3.x = A[2]; r3.y = A[5]; r3.z = A[8];
-  rec[4 * size_t(i) + 1] = r1;
-  rec[4 * size_t(i) + 2] = r2;
-  rec[4 * size_t(i) + 3] = r3;
+  const float4 xm = rec[2 * size_t(i)], vr = rec[2 * size_t(i) + 1];
+  force_matrix<float>(FE, FP, xm.w, vr.w, S.mu0, S.lambda0, S.hardening, S.gscale, A);
+  store_force(fa, n, i, A);
+  const float Arm[9] = {A[0], A[3], A[6], A[1], A[4], A[7], A[2], A[5], A[8]};
   publish_bound(pbound, (ctl->step + 1u) & 1u, sid,
-                record_bound(S, cell_key(S, r0.x, r0.y, r0.z), r0, r1, r2, r3, 1), (i | 63u) < n);
+                record_bound(S, cell_key(S, xm.x, xm.y, xm.z), xm.w, vr.x, vr.y, vr.z, Arm, 1));
 }
 
 // Zero the force matrices (init: the grid-mass pass needs m only).
 __global__ void __launch_bounds__(256) k_zero_force(const SceneDev* __restrict__ scenes,
-                                                    const uint16_t* __restrict__ pscene, float4* __restrict__ rec,
+                                                    const uint16_t* __restrict__ pscene,
+                                                    const float4* __restrict__ rec, float* __restrict__ fa,
                                                     uint32_t n, const StepCtl* __restrict__ ctl,
                                                     uint32_t* __restrict__ pbound) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int sid = pscene[i];
   const SceneDev& S = scenes[sid];
-  const float4 r0 = rec[4 * size_t(i)];
-  float4 r[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    r[k] = rec[4 * size_t(i) + 1 + k];
-    r[k].x = 0.f; r[k].y = 0.f; r[k].z = 0.f;
-    rec[4 * size_t(i) + 1 + k] = r[k];
-  }
+  const float4 xm = rec[2 * size_t(i)], vr = rec[2 * size_t(i) + 1];
+  const float Arm[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  store_force(fa, n, i, Arm);
   // the grid-mass P2G keeps particles of any mass (gMassCompute.glsl has no |m| > 100 skip)
   publish_bound(pbound, (ctl->step + 1u) & 1u, sid,
-                record_bound(S, cell_key(S, r0.x, r0.y, r0.z), r0, r[0], r[1], r[2], 0), (i | 63u) < n);
+                record_bound(S, cell_key(S, xm.x, xm.y, xm.z), xm.w, vr.x, vr.y, vr.z, Arm, 0));
 }
 
 // ---------------------------------------------------------------------------------------
 // Host transfer helpers: original order <-> current slot order through pid.
 __global__ void __launch_bounds__(256) k_gather_out(const float4* __restrict__ rec, const float* __restrict__ fe,
-                                                    const float* __restrict__ fp, const float* __restrict__ rho,
-                                                    const uint32_t* __restrict__ pid, int64_t p_off, int64_t cnt,
-                                                    uint32_t n, float* __restrict__ ox, float* __restrict__ ov,
-                                                    float* __restrict__ ofe, float* __restrict__ ofp,
-                                                    float* __restrict__ orho, float* __restrict__ om) {
+                                                    const float* __restrict__ fp, const uint32_t* __restrict__ pid,
+                                                    int64_t p_off, int64_t cnt, uint32_t n, float* __restrict__ ox,
+                                                    float* __restrict__ ov, float* __restrict__ ofe,
+                                                    float* __restrict__ ofp, float* __restrict__ orho,
+                                                    float* __restrict__ om) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= cnt) return;
   const uint32_t i = uint32_t(p_off + k);
   const int64_t o = int64_t(pid[i]) - p_off;
-  const float4 r0 = rec[4 * size_t(i)];
-  ox[3 * o] = r0.x; ox[3 * o + 1] = r0.y; ox[3 * o + 2] = r0.z;
-  om[o] = r0.w;
-  ov[3 * o] = rec[4 * size_t(i) + 1].w; ov[3 * o + 1] = rec[4 * size_t(i) + 2].w; ov[3 * o + 2] = rec[4 * size_t(i) + 3].w;
+  const float4 xm = rec[2 * size_t(i)], vr = rec[2 * size_t(i) + 1];
+  ox[3 * o] = xm.x; ox[3 * o + 1] = xm.y; ox[3 * o + 2] = xm.z;
+  om[o] = xm.w;
+  ov[3 * o] = vr.x; ov[3 * o + 1] = vr.y; ov[3 * o + 2] = vr.z;
 #pragma unroll
   for (int e = 0; e < 9; ++e) { ofe[9 * o + e] = fe[size_t(e) * n + i]; ofp[9 * o + e] = fp[size_t(e) * n + i]; }
-  orho[o] = rho[i];
+  orho[o] = vr.w;
 }
 
 // mask bits: 1 x, 2 v, 4 fe, 8 fp, 16 rho, 32 mass
 __global__ void __launch_bounds__(256) k_scatter_in(float4* __restrict__ rec, float* __restrict__ fe,
-                                                    float* __restrict__ fp, float* __restrict__ rho,
-                                                    const uint32_t* __restrict__ pid, int64_t p_off, int64_t cnt,
-                                                    uint32_t n, const float* __restrict__ ix,
-                                                    const float* __restrict__ iv, const float* __restrict__ ife,
-                                                    const float* __restrict__ ifp, const float* __restrict__ irho,
-                                                    const float* __restrict__ im, int mask) {
+                                                    float* __restrict__ fp, const uint32_t* __restrict__ pid,
+                                                    int64_t p_off, int64_t cnt, uint32_t n,
+                                                    const float* __restrict__ ix, const float* __restrict__ iv,
+                                                    const float* __restrict__ ife, const float* __restrict__ ifp,
+                                                    const float* __restrict__ irho, const float* __restrict__ im,
+                                                    int mask) {
   const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= cnt) return;
   const uint32_t i = uint32_t(p_off + k);
   const int64_t o = int64_t(pid[i]) - p_off;
-  float4 r0 = rec[4 * size_t(i)];
-  if (mask & 1) { r0.x = ix[3 * o]; r0.y = ix[3 * o + 1]; r0.z = ix[3 * o + 2]; }
-  if (mask & 32) r0.w = im[o];
-  rec[4 * size_t(i)] = r0;
-  if (mask & 2) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float4 r = rec[4 * size_t(i) + 1 + c];
-      r.w = iv[3 * o + c];
-      rec[4 * size_t(i) + 1 + c] = r;
-    }
-  }
+  float4 xm = rec[2 * size_t(i)], vr = rec[2 * size_t(i) + 1];
+  if (mask & 1) { xm.x = ix[3 * o]; xm.y = ix[3 * o + 1]; xm.z = ix[3 * o + 2]; }
+  if (mask & 32) xm.w = im[o];
+  if (mask & 2) { vr.x = iv[3 * o]; vr.y = iv[3 * o + 1]; vr.z = iv[3 * o + 2]; }
+  if (mask & 16) vr.w = irho[o];
+  rec[2 * size_t(i)] = xm;
+  rec[2 * size_t(i) + 1] = vr;
 #pragma unroll
   for (int e = 0; e < 9; ++e) {
     if (mask & 4) fe[size_t(e) * n + i] = ife[9 * o + e];
     if (mask & 8) fp[size_t(e) * n + i] = ifp[9 * o + e];
   }
-  if (mask & 16) rho[i] = irho[o];
 }
 
-// Non-finite scan of the particle state.
+// Non-finite scan of the particle state (record, F_E, F_P and the force matrix).
 __global__ void __launch_bounds__(256) k_check_finite(const float4* __restrict__ rec, const float* __restrict__ fe,
-                                                      const float* __restrict__ fp, uint32_t n,
-                                                      int* __restrict__ bad) {
+                                                      const float* __restrict__ fp, const float* __restrict__ fa,
+                                                      uint32_t n, int* __restrict__ bad) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   bool ok = true;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const float4 v = rec[4 * size_t(i) + r];
+  for (int r = 0; r < 2; ++r) {
+    const float4 v = rec[2 * size_t(i) + r];
     ok &= isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w);
   }
 #pragma unroll
-  for (int e = 0; e < 9; ++e) ok &= isfinite(fe[size_t(e) * n + i]) && isfinite(fp[size_t(e) * n + i]);
+  for (int e = 0; e < 9; ++e)
+    ok &= isfinite(fe[size_t(e) * n + i]) && isfinite(fp[size_t(e) * n + i]) && isfinite(fa[size_t(e) * n + i]);
   if (!ok) atomicOr(bad, 1);
 }
 

@@ -9,6 +9,7 @@ first N accepted. Mass m_p = rho0 * V_shape / N, F_E = F_P = I.
 Workloads (unit cube domain, node i at i*h):
 * ``snowball_drop``      64^3, h=1/64, sphere r=0.15 at (0.5,0.6,0.5), 8 ppc, dt=1e-4
 * ``two_snowballs``      128^3, h=1/128, spheres r=0.12 at x=0.3/0.7 moving at +-3 m/s, N=2^20, dt=5e-5
+* ``two_snowballs_8ppc`` the same geometry at 8 particles per cell (~2.4e5 particles)
 * ``slab_impact``        256^3, h=1/256, slab [0.2,0.8]x[0.5,0.7]x[0.2,0.8], box collider, sphere impactor
 * ``parameter_sweep``    64 scenes, log-uniform N in [4k, 256k], grids {32,48,64,96}^3, random materials
 The reference's own scene programs (src/exe/*.cpp) are available unchanged through ``reference``:
@@ -123,6 +124,23 @@ def two_snowballs(n_particles=1 << 20, seed=1002, grad_scale=1.0, dims=128) -> P
                          snow_material(grad_scale=grad_scale), dt=5e-5)
 
 
+def two_snowballs_8ppc(seed=1006, grad_scale=1.0, dims=128) -> PhysicalScene:
+    """The two-snowball geometry of two_snowballs at 8 particles per cell (stratified 2x2x2 jitter in
+    each cell of a sphere's bounding box; about 2.4e5 particles on 128^3)."""
+    h = 1.0 / dims
+    r = 0.12
+    parts, vels = [], []
+    for k, (c, vx) in enumerate((((0.3, 0.5, 0.5), 3.0), ((0.7, 0.5, 0.5), -3.0))):
+        x = stratified_sphere(seed, k, c, r, h)
+        parts.append(x)
+        vels.append(np.tile([vx, 0.0, 0.0], (len(x), 1)))
+    x, v = np.concatenate(parts), np.concatenate(vels)
+    m = 400.0 * 2 * (4.0 / 3.0 * math.pi * r ** 3) / len(x)
+    ps = ParticleSystem(x=x.astype(np.float32), v=v.astype(np.float32), mass=np.full(len(x), m, np.float32))
+    return PhysicalScene(ps, Grid(dims, dims, dims, h, 0.0, 0.0, 0.0), CollisionObjects([ground()]),
+                         snow_material(grad_scale=grad_scale), dt=5e-5)
+
+
 def slab_impact(n_particles=1 << 24, seed=1003, grad_scale=1.0, dims=256, impactor_y=0.85) -> PhysicalScene:
     """Slab [0.2,0.8]x[0.5,0.7]x[0.2,0.8] dropped onto a box collider, hit by a sphere impactor of
     radius 0.1 (centre height impactor_y; 0.8 starts it in contact with the slab) at 10 m/s."""
@@ -165,8 +183,10 @@ def parameter_sweep(num_scenes=64, seed=4000, nmin=4096, nmax=262144, dims_choic
         m = 400.0 * vol / n
         ps = ParticleSystem(x=x.astype(np.float32), v=np.tile(vel, (n, 1)).astype(np.float32),
                             mass=np.full(n, m, np.float32))
-        out.append(PhysicalScene(ps, Grid(dims, dims, dims, h, 0.0, 0.0, 0.0), CollisionObjects([ground()]), mat,
-                                 dt=1e-4 * 64 / dims))
+        scene = PhysicalScene(ps, Grid(dims, dims, dims, h, 0.0, 0.0, 0.0), CollisionObjects([ground()]), mat,
+                              dt=1e-4 * 64 / dims)
+        scene.shape = "sphere" if sphere else "box"
+        out.append(scene)
     return out
 
 

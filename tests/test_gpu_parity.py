@@ -9,8 +9,8 @@ algorithm, so differences come from FP32 rounding, including the 1e-5 rounding o
 import numpy as np
 import pytest
 
-from snow_amd import (FLAG_KEEP_GRID, FLAG_MIXED, SPHERE, Collider, CollisionObjects, Material, ParticleSystem,
-                      SnowContext)
+from snow_amd import (FLAG_COMPENSATED_G2P, FLAG_KEEP_GRID, FLAG_MIXED, SPHERE, Collider, CollisionObjects, Material,
+                      ParticleSystem, SnowContext)
 from snow_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -32,6 +32,29 @@ def deformed(scene, amp=0.02, seed=5):
     # column-major per particle: element (r, c) at c*3 + r
     scene.particles.fe = np.ascontiguousarray(fe.transpose(0, 2, 1).reshape(n, 9), np.float32)
     scene.particles.fp = np.ascontiguousarray(fp.transpose(0, 2, 1).reshape(n, 9), np.float32)
+    return scene
+
+
+def clamp_deformed(scene, seed=11, spread=0.01):
+    """F_E = U diag(s) V^T with random rotations U, V and singular values s spread uniformly over
+    [1 - theta_c - spread, 1 + theta_s + spread]: the update's singular values then fall on both sides
+    of the clamp bounds (particleAdvance.glsl:112-117), and some within the FP64 path's 2e-4 of them.
+    F_P = I."""
+    rng = np.random.default_rng(seed)
+    n = len(scene.particles)
+    mat = scene.material
+    lo, hi = 1.0 - mat.crit_comp - spread, 1.0 + mat.crit_stretch + spread
+
+    def rotations(k):
+        q, r = np.linalg.qr(rng.normal(size=(k, 3, 3)))
+        q = q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
+        q[np.linalg.det(q) < 0, :, 0] *= -1.0
+        return q
+    u, v = rotations(n), rotations(n)
+    sv = rng.uniform(lo, hi, (n, 3))
+    fe = u @ (sv[:, :, None] * v.transpose(0, 2, 1))
+    scene.particles.fe = np.ascontiguousarray(fe.transpose(0, 2, 1).reshape(n, 9), np.float32)
+    scene.particles.fp = np.tile(np.eye(3, dtype=np.float32).reshape(1, 9), (n, 1))
     return scene
 
 
@@ -164,6 +187,86 @@ def test_g2p_advance_teacher_forced(oracle_mod):
     assert rel_err(g["v"], o.v) < 1e-5
     assert np.abs(g["fe"] - o.fe).max() < 1e-4
     assert np.abs(g["fp"] - o.fp).max() < 1e-4
+
+
+def test_grid_update_production_kernel(oracle_mod):
+    """The tiled grid merge + update that snow_step runs (k_grid<false, true>), from a GPU P2G,
+    against the oracle's gForceCompute on exactly the node fields that P2G produced."""
+    sc = with_colliders(clamp_deformed(scenes.snowball_drop()))
+    sc.particles.v[:] = np.random.default_rng(4).normal(0, 0.5, sc.particles.v.shape).astype(np.float32)
+    dt = 1e-4
+    o = oracle_mod.Oracle(sc)
+    o.init()
+    with SnowContext(sc) as ctx:
+        ctx.write_particles(rho0=o.rho0.astype(np.float32))
+        ctx.stage("p2g")
+        g = ctx.read_grid(fields=("m", "mv", "f"))
+        ctx.stage("grid_update", dt)
+        gv = ctx.read_grid(fields=("vstar",))
+    o.gm[:], o.gmv[:], o.gf[:] = g["m"], g["mv"], g["f"]
+    o.grid_update(dt)
+    assert (g["m"] > 0).sum() > 1000
+    assert rel_err(gv["vstar"], o.gvstar) < 1e-5  # FP32 vs FP64 on the same input
+
+
+def _g2p_advance_production(oracle_mod, flags):
+    """One fused G2P + particle advance (the kernel snow_step runs) from the oracle's grid and the
+    same particle state, singular values of F_E around the clamp bounds. Returns the GPU state, the
+    oracle and the number of particles that took the FP64 path."""
+    sc = with_colliders(clamp_deformed(scenes.snowball_drop()))
+    sc.particles.v[:] = np.random.default_rng(3).normal(0, 0.5, sc.particles.v.shape).astype(np.float32)
+    dt = 1e-4
+    o = oracle_mod.Oracle(sc)
+    o.init()
+    o.p2g()
+    o.grid_update(dt)
+    gm, gmv, gvs = o.gm.copy(), o.gmv.copy(), o.gvstar.copy()
+    with SnowContext(sc, flags) as ctx:
+        ctx.write_particles(rho0=o.rho0.astype(np.float32))
+        ctx.write_grid(m=gm, mv=gmv, vstar=gvs)
+        ctx.stage("g2p_advance", dt)
+        g = ctx.read_particles()
+        fp64 = ctx.fp64_particles()
+    o.g2p_advance(dt)
+    return g, o, fp64
+
+
+@pytest.mark.parametrize("flags", [0, FLAG_COMPENSATED_G2P, FLAG_MIXED, FLAG_MIXED | FLAG_COMPENSATED_G2P],
+                         ids=["plain", "compensated", "mixed", "mixed_compensated"])
+def test_g2p_advance_production_kernel(oracle_mod, flags):
+    g, o, fp64 = _g2p_advance_production(oracle_mod, flags)
+    # the stored F_E is W S' V^T with V rounded to 1e-5 (SVD.include.glsl:354-358), so the singular
+    # values of a clamped F_E lie within a few 1e-5 of the bound
+    sv = np.linalg.svd(o.fe.reshape(-1, 3, 3), compute_uv=False)
+    mat = scenes.snowball_drop().material
+    at_clamp = (np.abs(sv - (1 - mat.crit_comp)) < 5e-5) | (np.abs(sv - (1 + mat.crit_stretch)) < 5e-5)
+    assert at_clamp.any(-1).mean() > 0.3  # the clamp is active for many particles
+    assert np.abs(g["x"] - o.x).max() < 1e-6
+    assert rel_err(g["v"], o.v) < 1e-5
+    assert np.abs(g["fe"] - o.fe).max() < 1e-4
+    assert np.abs(g["fp"] - o.fp).max() < 1e-4
+    if flags & FLAG_MIXED:
+        assert fp64 > 0
+    else:
+        assert fp64 == 0
+
+
+def test_mixed_fp64_path_not_worse_teacher_forced(oracle_mod):
+    """From the same state, the mixed form (FP64 SVD and force matrix for particles near the clamp)
+    has no larger F_E / F_P error against the FP64 oracle than the plain FP32 form: the particles it
+    does not send to the FP64 path are computed as in the plain form."""
+    gp, o, _ = _g2p_advance_production(oracle_mod, 0)
+    gm, _, fp64 = _g2p_advance_production(oracle_mod, FLAG_MIXED)
+    assert fp64 > 0
+    print(f"\nFP64 path: {fp64} of {len(o.x)} particles")
+    for k in ("fe", "fp"):
+        ep = np.sqrt(((gp[k] - getattr(o, k)) ** 2).sum(-1))
+        em = np.sqrt(((gm[k] - getattr(o, k)) ** 2).sum(-1))
+        print(f"{k}: plain max {ep.max():.3e} rms {np.sqrt((ep ** 2).mean()):.3e}; "
+              f"mixed max {em.max():.3e} rms {np.sqrt((em ** 2).mean()):.3e}")
+        assert np.sqrt((em ** 2).mean()) <= np.sqrt((ep ** 2).mean())
+        assert em.max() <= ep.max() + 1e-6
+    assert np.array_equal(gp["x"], gm["x"])  # velocity and advection do not depend on the form
 
 
 def test_fused_step_matches_stage_chain(oracle_mod):
@@ -408,6 +511,31 @@ def test_autotune_keeps_state_and_results():
                 assert np.array_equal(pa[k], pb[k]), (i, k)
 
 
+def test_autotune_then_new_state_matches_untuned():
+    """After autotuning (whose trials step a snapshot and rewind the step counter), a host write of
+    new positions and steps with another dt give the same bits as a context that never tuned: no
+    cell tile keeps a stamp from a trial step."""
+    sc = _sweep_small()
+    with SnowContext(sc) as a, SnowContext(sc) as b:
+        a.init()
+        b.init()
+        for ctx in (a, b):
+            ctx.step()
+        b.autotune(steps=3)
+        for i in range(len(sc)):
+            st = a.read_particles(i)
+            x2 = (st["x"] + np.float32(0.004)).astype(np.float32)
+            for ctx in (a, b):
+                ctx.write_particles(i, x=x2)
+        for ctx in (a, b):
+            for _ in range(3):
+                ctx.step(7e-5)
+        for i in range(len(sc)):
+            pa, pb = a.read_particles(i), b.read_particles(i)
+            for k in ("x", "v", "fe", "fp"):
+                assert np.array_equal(pa[k], pb[k]), (i, k)
+
+
 def test_batched_sweep_matches_oracle(oracle_mod):
     """Per-scene parameters and dt in one batch: each scene agrees with its own FP64 oracle."""
     sc = _sweep_small()[:3]
@@ -478,8 +606,8 @@ def test_positions_view_is_the_particle_state(c1_scene):
             ctx.step(c1_scene.dt)
         ctx.sync()
         ptr, stride, n = ctx.positions_view(0)
-        assert n == len(c1_scene.particles) and stride == 64
-        raw = np.empty((n, 16), np.float32)
+        assert n == len(c1_scene.particles) and stride == 32
+        raw = np.empty((n, stride // 4), np.float32)
         assert hip.hipMemcpy(raw.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(n * stride), 2) == 0
         p = ctx.read_particles()
     got = raw[:, :4]
