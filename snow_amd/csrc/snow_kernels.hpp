@@ -1052,14 +1052,6 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
 // box use g2p_gather. The sums are the same as g2p_gather's, in the same order.
 constexpr int G2P_BOX = 384;
 
-// The per-lane gather of g2p_gather_wave's fallback, kept out of line: it is rare, and inlined it
-// holds registers through the wave path.
-__device__ __attribute__((noinline)) void g2p_gather_lane(const SceneDev& S, float x, float y, float z,
-                                                          const float2* __restrict__ g_node, float vpn[3],
-                                                          float dvp[9]) {
-  g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
-}
-
 __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, float x, float y, float z,
                                                 const float2* __restrict__ g_node, float2* box, float vpn[3],
                                                 float dvp[9]) {
@@ -1072,7 +1064,7 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
   const int sid0 = __builtin_amdgcn_readfirstlane(sid);
   const bool full = act == ~0ull;
   if (!full || __ballot(interior && sid == sid0) != act) {
-    g2p_gather_lane(S, x, y, z, g_node, vpn, dvp);
+    g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
     return;
   }
   // node box [xmin, xmin+ex) x [ymin, ymin+ey) x [zmin, zmin+ez)
@@ -1080,7 +1072,7 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
   const int ex = wave_max_i(bx) + 3 - xmin, ey = wave_max_i(by) + 3 - ymin, ez = wave_max_i(bz) + 3 - zmin;
   const int vol = ex * ey * ez;
   if (vol > G2P_BOX) {
-    g2p_gather_lane(S, x, y, z, g_node, vpn, dvp);
+    g2p_gather(S, x, y, z, g_node, vpn, dvp, false);
     return;
   }
   const int lane = threadIdx.x & 63;
@@ -1103,9 +1095,11 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
   for (int k = 0; k < 9; ++k) dvp[k] = 0.f;
   const float2* p0 = box + 3 * ((bx - 1 - xmin) + ex * ((by - 1 - ymin) + ey * (bz - 1 - zmin)));
   const int sy = 3 * ex, sz = 3 * ex * ey;
+  // two rows of LDS reads in flight: enough for the LDS latency, and it leaves the registers that
+  // four rows would hold to the rest of the kernel
 #pragma unroll 1
   for (int oz = 0; oz < 4; ++oz)
-#pragma unroll
+#pragma unroll 2
     for (int oy = 0; oy < 4; ++oy)
       g2p_row(p0 + oz * sz + oy * sy, wx, qx, wy[oy] * wz[oz], qy[oy] * wz[oz], wy[oy] * qz[oz], vpn, dvp, 0, 3);
   const float gs = S.gscale;
