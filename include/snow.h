@@ -167,6 +167,14 @@ int snow_stage_g2p_advance(snow_ctx* ctx, double dt);
  * NULL to skip that field. rho0 is the rest density from snow_init (pv.w*1e-6). */
 int snow_read_particles(snow_ctx* ctx, int32_t scene, float* x, float* v, float* fe, float* fp,
                         float* rho0);
+/* The particles with creation index in [first, first + count) of one scene, as snow_read_particles
+ * gives them, staged through a fixed device array of the library instead of device memory it
+ * allocates: this call allocates nothing (no hipMalloc). No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (serving loops, code that forbids allocation).
+ * count <= 65536 (SNOW_E_CAPACITY above). The staging array is shared by every context on a GPU:
+ * calls that overlap on one GPU race. */
+int snow_read_particles_noalloc(snow_ctx* ctx, int32_t scene, int64_t first, int64_t count, float* x, float* v,
+                                float* fe, float* fp, float* rho0);
 /* Overwrite particle state of one scene (original order); NULL fields are kept. */
 int snow_write_particles(snow_ctx* ctx, int32_t scene, const float* x, const float* v,
                          const float* fe, const float* fp, const float* rho0);

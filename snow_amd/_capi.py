@@ -83,6 +83,8 @@ SIGNATURES = [
     ("snow_stage_particle_advance", C.c_int, [C.c_void_p, C.c_double]),
     ("snow_stage_g2p_advance", C.c_int, [C.c_void_p, C.c_double]),
     ("snow_read_particles", C.c_int, [C.c_void_p, C.c_int32, f32p, f32p, f32p, f32p, f32p]),
+    ("snow_read_particles_noalloc", C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int64, f32p, f32p, f32p, f32p,
+                                             f32p]),
     ("snow_write_particles", C.c_int, [C.c_void_p, C.c_int32, f32p, f32p, f32p, f32p, f32p]),
     ("snow_read_grid", C.c_int, [C.c_void_p, C.c_int32, f32p, f32p, f32p, f32p]),
     ("snow_write_grid", C.c_int, [C.c_void_p, C.c_int32, f32p, f32p, f32p, f32p]),

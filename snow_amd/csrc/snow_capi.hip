@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,13 @@
 #include "snow_sort.hpp"
 
 using namespace snow;
+
+// Staging of snow_read_particles_noalloc: a fixed device array instead of memory the library
+// allocates. Each GPU the library runs on holds its own copy until the process ends; every call on
+// that GPU shares it (the window it reads is written in full by each call, so nothing is cleared).
+constexpr int64_t NOALLOC_MAX_PARTICLES = 65536;
+constexpr int64_t NOALLOC_FLOATS = 25;  // x 3, v 3, F_E 9, F_P 9, rho0 1 per particle
+static __device__ __align__(256) float scratch[NOALLOC_MAX_PARTICLES * NOALLOC_FLOATS];
 
 namespace {
 
@@ -117,6 +125,8 @@ struct snow_ctx {
   int64_t stage_calls[ST_COUNT] = {0};
   double stage_min_ms[ST_COUNT] = {0}, stage_max_ms[ST_COUNT] = {0}, stage_sq_ms[ST_COUNT] = {0};
 
+  float* noalloc_stage = nullptr;  // device address of `scratch` on this context's GPU
+
   hipGraph_t graph[2] = {nullptr, nullptr};      // batched step starting from buffer parity 0 / 1
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
 
@@ -150,6 +160,23 @@ int fail(snow_ctx* c, int code, const std::string& msg) {
     hipError_t _e = hipGetLastError();                                                           \
     if (_e != hipSuccess) return fail(ctx, SNOW_E_HIP, std::string("launch: ") + hipGetErrorString(_e)); \
   } while (0)
+
+// The device address of `scratch` on GPU `dev` (the current device), looked up once per device.
+std::mutex g_scratch_mu;
+float* g_scratch_addr[256] = {};
+
+hipError_t scratch_address(int dev, float** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= 256) return hipErrorInvalidDevice;
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return hipSuccess;
+}
 
 template <class T>
 hipError_t dalloc(T** p, size_t count) {
@@ -581,6 +608,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   } while (0)
 
   CREATECHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  CREATECHK(scratch_address(c->device, &c->noalloc_stage));
   CREATECHK(hipEventCreate(&c->ev[0]));
   CREATECHK(hipEventCreate(&c->ev[1]));
   const uint32_t N = c->N;
@@ -1075,6 +1103,31 @@ int snow_read_particles(snow_ctx* c, int32_t s, float* x, float* v, float* fe, f
   if (fe) HIPCHK(c, hipMemcpy(fe, ofe, sizeof(float) * 9 * n, hipMemcpyDeviceToHost));
   if (fp) HIPCHK(c, hipMemcpy(fp, ofp, sizeof(float) * 9 * n, hipMemcpyDeviceToHost));
   if (rho0) HIPCHK(c, hipMemcpy(rho0, orho, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int snow_read_particles_noalloc(snow_ctx* c, int32_t s, int64_t first, int64_t count, float* x, float* v,
+                                float* fe, float* fp, float* rho0) {
+  if (!c || s < 0 || s >= c->nscenes) return SNOW_E_ARG;
+  const int64_t n = c->sh[s].n;
+  if (first < 0 || count < 0 || first > n || count > n - first)
+    return fail(c, SNOW_E_ARG, "particle window outside the scene");
+  if (count > NOALLOC_MAX_PARTICLES)
+    return fail(c, SNOW_E_CAPACITY, "snow_read_particles_noalloc reads at most 65536 particles per call");
+  if (count == 0) return 0;
+  float* st = c->noalloc_stage;
+  float *ox = st, *ov = ox + 3 * count, *ofe = ov + 3 * count, *ofp = ofe + 9 * count, *orho = ofp + 9 * count;
+  const int k = c->cur;
+  k_gather_window<<<grid_for(n, 256), 256, 0, c->stream>>>(c->d_rec[k], c->d_fe[k], c->d_fp[k], c->d_pid[k],
+                                                            c->scenes[s].p_off, n, c->N, first, count, ox, ov, ofe,
+                                                            ofp, orho);
+  LAUNCHCHK(c);
+  if (x) HIPCHK(c, hipMemcpyAsync(x, ox, sizeof(float) * 3 * count, hipMemcpyDeviceToHost, c->stream));
+  if (v) HIPCHK(c, hipMemcpyAsync(v, ov, sizeof(float) * 3 * count, hipMemcpyDeviceToHost, c->stream));
+  if (fe) HIPCHK(c, hipMemcpyAsync(fe, ofe, sizeof(float) * 9 * count, hipMemcpyDeviceToHost, c->stream));
+  if (fp) HIPCHK(c, hipMemcpyAsync(fp, ofp, sizeof(float) * 9 * count, hipMemcpyDeviceToHost, c->stream));
+  if (rho0) HIPCHK(c, hipMemcpyAsync(rho0, orho, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 

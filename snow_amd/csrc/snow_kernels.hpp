@@ -1420,6 +1420,27 @@ __global__ void __launch_bounds__(256) k_gather_out(const float4* __restrict__ r
   orho[o] = vr.w;
 }
 
+// The particles with creation index in [first, first + count) of one scene (p_off, cnt), from
+// the current slot order, into window order: o = pid - p_off - first.
+__global__ void __launch_bounds__(256) k_gather_window(const float4* __restrict__ rec, const float* __restrict__ fe,
+                                                       const float* __restrict__ fp, const uint32_t* __restrict__ pid,
+                                                       int64_t p_off, int64_t cnt, uint32_t n, int64_t first,
+                                                       int64_t count, float* __restrict__ ox, float* __restrict__ ov,
+                                                       float* __restrict__ ofe, float* __restrict__ ofp,
+                                                       float* __restrict__ orho) {
+  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  const uint32_t i = uint32_t(p_off + k);
+  const int64_t o = int64_t(pid[i]) - p_off - first;
+  if (o < 0 || o >= count) return;
+  const float4 xm = rec[2 * size_t(i)], vr = rec[2 * size_t(i) + 1];
+  ox[3 * o] = xm.x; ox[3 * o + 1] = xm.y; ox[3 * o + 2] = xm.z;
+  ov[3 * o] = vr.x; ov[3 * o + 1] = vr.y; ov[3 * o + 2] = vr.z;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { ofe[9 * o + e] = fe[size_t(e) * n + i]; ofp[9 * o + e] = fp[size_t(e) * n + i]; }
+  orho[o] = vr.w;
+}
+
 // mask bits: 1 x, 2 v, 4 fe, 8 fp, 16 rho, 32 mass
 __global__ void __launch_bounds__(256) k_scatter_in(float4* __restrict__ rec, float* __restrict__ fe,
                                                     float* __restrict__ fp, const uint32_t* __restrict__ pid,
