@@ -566,7 +566,6 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     if (s_cell[r_lo] == P2G_DEAD) continue;  // out-of-grid particles (uniform)
     const uint32_t tile = keys[lo + r_lo] / TILE_CELLS;
     const bool starts_tile = r > 0 || s_edge[0] != tile;
-    const bool ends_tile = r + 1 < nrun || s_edge[1] != tile;
 
     // Lane l sums the contributions of the staged particles to its stencil node o in registers
     // (mass, momentum and A grad(w), p2gTransfer.glsl:84-146) while the cell stays the same;
@@ -651,15 +650,6 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     if (cur >= 0) commit();
     __syncthreads();
     float* out = starts_tile ? tpart + size_t(tile) * (NQ * HALO_NODES) : hpart + size_t(chunk) * (NQ * HALO_NODES);
-    if (t == 0) {
-      if (starts_tile) {
-        tile_lo[tile] = lo + r_lo;
-        tile_stamp[tile] = step;
-        const uint32_t sub = chunk % TILE_LISTS;
-        tile_list[size_t(sub) * list_cap + atomicAdd(&list_ctr[sub * TILE_LIST_STRIDE], 1u)] = tile;
-      }
-      if (ends_tile) tile_hi[tile] = lo + r_hi;
-    }
     // write the run sums out and clear the accumulator for the next run
     for (int i = t; i < NQ * HALO_NODES; i += NT) {
       const int q = i / HALO_NODES;
@@ -669,6 +659,37 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       out[i] = float(ldexp(v, -e));  // one rounding to float
     }
     __syncthreads();
+  }
+
+  // ---- the tiles of the chunk's runs, once for the whole chunk, so that no run waits on the list
+  // atomic: the tiles a run starts get their first position and this step's stamp and join a tile
+  // sublist (one atomic per 64 runs), the tiles a run ends their end position
+  if (wave == 0) {
+    const uint32_t sub = chunk % TILE_LISTS;
+    for (uint32_t r0 = 0; r0 < nrun; r0 += 64) {
+      const uint32_t r = r0 + uint32_t(lane);
+      bool starts = false;
+      uint32_t tile = 0u;
+      if (r < nrun) {
+        const uint32_t r_lo = s_run[r], r_hi = s_run[r + 1];
+        if (s_cell[r_lo] != P2G_DEAD) {
+          tile = keys[lo + r_lo] / TILE_CELLS;
+          starts = r > 0 || s_edge[0] != tile;
+          if (starts) {
+            tile_lo[tile] = lo + r_lo;
+            tile_stamp[tile] = step;
+          }
+          if (r + 1 < nrun || s_edge[1] != tile) tile_hi[tile] = lo + r_hi;
+        }
+      }
+      const uint64_t mask = __ballot(starts);
+      if (mask) {
+        uint32_t base = 0u;
+        if (lane == 0) base = atomicAdd(&list_ctr[sub * TILE_LIST_STRIDE], uint32_t(__popcll(mask)));
+        base = uint32_t(__builtin_amdgcn_readfirstlane(int(base)));
+        if (starts) tile_list[size_t(sub) * list_cap + base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)))] = tile;
+      }
+    }
   }
 }
 
