@@ -135,6 +135,9 @@ struct snow_ctx {
   uint32_t total_chunks = 0;       // P2G workgroups (chunks of every scene)
   uint16_t* d_chunk_scene = nullptr;
   uint32_t* d_chunk_lo = nullptr;  // first sorted position of each P2G chunk, as the P2G chose it
+  uint32_t* d_chunk_w = nullptr;   // planned window start of each P2G chunk (before tile alignment)
+  uint32_t* d_chunk_runs = nullptr;  // runs (tiles) of each chunk in the last P2G
+  unsigned long long* d_plan_pref = nullptr;  // cost prefix scratch of the chunk plan
   // G2P waves: each scene's sorted range cut into waves of 64 positions, (first position, scene)
   // per wave; only for batches of several scenes (one scene: wave w starts at 64 w)
   uint2* d_waves = nullptr;
@@ -359,7 +362,9 @@ int run_sort(snow_ctx* c, float dt_rigid) {
   return tm.done();
 }
 
-int run_p2g(snow_ctx* c, int mass_only) {
+// plan: also plan the chunk windows of the next P2G from this one's runs (the fused step and
+// init do; the per-stage API keeps the current plan, so its P2G leaves later steps unchanged)
+int run_p2g(snow_ctx* c, int mass_only, bool plan) {
   if (!c->bound_valid) {  // records no writer published (after the step start: this step's parity)
     const int rc = publish_bounds(c, 0u, mass_only ? 0 : 1);
     if (rc) return rc;
@@ -369,11 +374,14 @@ int run_p2g(snow_ctx* c, int mass_only) {
     k_p2g_chunk<<<c->total_chunks, 64 * P2G_WAVES, 0, c->stream>>>(
         c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_slot, c->d_tile_lo, c->d_tile_hi, c->d_tile_stamp,
         c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_fa, c->N, c->d_tpart,
-        c->d_hpart, c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, mass_only ? 0 : 1);
+        c->d_hpart, c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs,
+        mass_only ? 0 : 1);
     LAUNCHCHK(c);
-    k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
-                                                            c->d_tile_lo, c->d_tile_hi, c->d_chunk_lo, c->d_tpart,
-                                                            c->d_hpart);
+    // the fold of continuation partials, and one more block per scene that plans the chunk
+    // windows of the next P2G
+    k_fold_partials<<<c->total_chunks + (plan ? uint32_t(c->nscenes) : 0u), 1024, 0, c->stream>>>(
+        c->d_scenes, c->d_chunk_scene, c->d_keys[1], c->d_tile_lo, c->d_tile_hi, c->d_chunk_lo, c->d_tpart,
+        c->d_hpart, c->total_chunks, c->d_chunk_runs, c->d_plan_pref, c->d_chunk_w);
     LAUNCHCHK(c);
   }
   c->bound_valid = false;  // consumed; the slots of the other parity are cleared for the writers
@@ -434,7 +442,7 @@ int step_impl(snow_ctx* c, float dt) {
   int rc;
   if ((rc = run_force_prepass(c))) return rc;
   if ((rc = run_sort(c, dt))) return rc;
-  if ((rc = run_p2g(c, 0))) return rc;
+  if ((rc = run_p2g(c, 0, true))) return rc;
   const bool keep = (c->flags & SNOW_FLAG_KEEP_GRID) != 0;
   if ((rc = run_grid(c, keep, true, dt))) return rc;
   if ((rc = run_g2p_advance(c, dt))) return rc;
@@ -452,7 +460,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_fa, c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp,
-                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
+                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -555,15 +563,16 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     S.p_cnt = d.num_particles;
     S.inv_h = 1.0f / d.grid.spacing;
     for (int k = 0; k < 3; ++k) S.uo[k] = -d.grid.origin[k] / d.grid.spacing;
-    // P2G chunk: a power of two from 256 to P2G_SEG_MAX giving at least 64 chunks where it can
-    // (small scenes alone still fill many workgroups, big ones keep few tile splits); the P2G
-    // rounding depends on it, so it depends on the scene's particle count only
+    // P2G chunks: the mean chunk is a power of two from 256 to P2G_SEG_MEAN giving at least 64
+    // chunks where it can (small scenes alone still fill many workgroups, big ones keep few tile
+    // splits); the windows are planned from the previous P2G's runs, up to P2G_SEG_MAX - align_w
+    // long. The chunking depends on the scene's own particles only.
     {
       uint32_t seg = 256;
-      while (seg < uint32_t(P2G_SEG_MAX) && uint64_t(seg) * 2 * 64 <= uint64_t(d.num_particles)) seg *= 2;
+      while (seg < uint32_t(P2G_SEG_MEAN) && uint64_t(seg) * 2 * 64 <= uint64_t(d.num_particles)) seg *= 2;
       S.seg = seg;
-      // bounds at tile starts where a chunk plus its search window fits the P2G's LDS lists
-      S.align_w = std::min(seg, uint32_t(P2G_SEG_MAX) - seg);
+      // bounds move to the first tile start within align_w positions of the planned window
+      S.align_w = std::min(seg, 512u);
       S.chunk_off = uint32_t(chunk_off);
       chunk_off += (uint64_t(d.num_particles) + seg - 1) / seg;
     }
@@ -638,6 +647,9 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_hpart, size_t(std::max<uint32_t>(1, c->total_chunks)) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_chunk_scene, std::max<uint32_t>(1, c->total_chunks)));
   CREATECHK(dalloc(&c->d_chunk_lo, std::max<uint32_t>(1, c->total_chunks)));
+  CREATECHK(dalloc(&c->d_chunk_w, std::max<uint32_t>(1, c->total_chunks)));
+  CREATECHK(dalloc(&c->d_chunk_runs, std::max<uint32_t>(1, c->total_chunks)));
+  CREATECHK(dalloc(&c->d_plan_pref, std::max<uint32_t>(1, c->total_chunks)));
   {
     std::vector<uint2> waves;
     for (int s = 0; s < num_scenes; ++s) {
@@ -680,12 +692,17 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     CREATECHK(hipMemcpy(c->d_pscene, ps.data(), sizeof(uint16_t) * N, hipMemcpyHostToDevice));
     CREATECHK(hipMemcpy(c->d_tile_scene, ts.data(), sizeof(uint16_t) * ts.size(), hipMemcpyHostToDevice));
     std::vector<uint16_t> cs(std::max<uint32_t>(1, c->total_chunks), 0);
+    std::vector<uint32_t> cw(std::max<uint32_t>(1, c->total_chunks), 0);
     for (int s = 0; s < num_scenes; ++s) {
       const SceneDev& S = c->scenes[s];
       const uint32_t nch = uint32_t((uint64_t(S.p_cnt) + S.seg - 1) / S.seg);
-      for (uint32_t k = 0; k < nch; ++k) cs[S.chunk_off + k] = uint16_t(s);
+      for (uint32_t k = 0; k < nch; ++k) {
+        cs[S.chunk_off + k] = uint16_t(s);
+        cw[S.chunk_off + k] = uint32_t(S.p_off) + k * S.seg;  // until the first plan: even windows
+      }
     }
     CREATECHK(hipMemcpy(c->d_chunk_scene, cs.data(), sizeof(uint16_t) * cs.size(), hipMemcpyHostToDevice));
+    CREATECHK(hipMemcpy(c->d_chunk_w, cw.data(), sizeof(uint32_t) * cw.size(), hipMemcpyHostToDevice));
     CREATECHK(hipMemcpy(c->d_ntile_scene, nts.data(), sizeof(uint16_t) * nts.size(), hipMemcpyHostToDevice));
     if (!c->cols.empty())
       CREATECHK(hipMemcpy(c->d_cols, c->cols.data(), sizeof(Collider) * c->cols.size(), hipMemcpyHostToDevice));
@@ -764,7 +781,7 @@ int snow_stage_grid_mass(snow_ctx* c) {
   c->bound_valid = true;
   c->force_valid = false;
   if ((rc = run_sort(c, -1.0f))) return rc;
-  if ((rc = run_p2g(c, 1))) return rc;
+  if ((rc = run_p2g(c, 1, true))) return rc;
   if ((rc = run_grid(c, true, false, 0.0f))) return rc;
   c->grid_state = 1;
   c->keep_valid = true;
@@ -830,7 +847,7 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
       c->cur = p;
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       rc = run_sort(c, 0.0f);
-      if (!rc) rc = run_p2g(c, 0);
+      if (!rc) rc = run_p2g(c, 0, true);
       if (!rc) rc = run_grid(c, (c->flags & SNOW_FLAG_KEEP_GRID) != 0, true, 0.0f);
       if (!rc) rc = run_g2p_advance(c, 0.0f);
       hipGraph_t g = nullptr;
@@ -888,6 +905,7 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
       {nullptr, c->d_fp[k], sizeof(float) * 9 * N},  {nullptr, c->d_fa, sizeof(float) * 9 * N},
       {nullptr, c->d_pid[k], sizeof(uint32_t) * N},   {nullptr, c->d_keys[0], sizeof(uint32_t) * N},
       {nullptr, c->d_ctl, sizeof(StepCtl)},
+      {nullptr, c->d_chunk_w, sizeof(uint32_t) * std::max<uint32_t>(1, c->total_chunks)},
       {nullptr, c->d_cols, sizeof(Collider) * std::max<size_t>(1, c->cols.size())}};
   auto free_all = [&]() {
     for (auto& b : bufs)
@@ -1003,7 +1021,7 @@ int snow_stage_p2g(snow_ctx* c) {
   int rc;
   if ((rc = run_force_prepass(c))) return rc;
   if ((rc = run_sort(c, -1.0f))) return rc;
-  if ((rc = run_p2g(c, 0))) return rc;
+  if ((rc = run_p2g(c, 0, false))) return rc;
   if ((rc = run_grid(c, true, false, 0.0f))) return rc;
   c->grid_state = 1;
   c->keep_valid = true;

@@ -26,7 +26,11 @@ constexpr int HALO = 7;              // TILE + 3 nodes per edge
 constexpr int HALO_NODES = 343;      // 7^3
 constexpr int NQ = 7;                // m, p.xyz, f.xyz
 constexpr int P2G_WAVES = 8;         // waves per P2G workgroup
-constexpr int P2G_SEG_MAX = 2048;    // largest P2G chunk in particles (sizes the LDS run lists)
+constexpr int P2G_SEG_MAX = 3072;    // largest P2G chunk in particles (sizes the LDS run lists)
+constexpr int P2G_SEG_MEAN = 2048;   // largest mean chunk of a scene (its particles / its chunks)
+constexpr uint32_t P2G_RUN_COST = 96; // planning cost of a run (tile) in particles: its barriers,
+                                      // partial write and short batches (measured: ~2.6 us per
+                                      // run against ~0.03 us per particle)
 
 struct SceneDev {
   int D[3];
@@ -412,15 +416,15 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const StepCtl* __restrict__ ctl, uint32_t* __restrict__ list_ctr, uint32_t* __restrict__ tile_list,
     uint32_t list_cap, const float4* __restrict__ rec, const float* __restrict__ fa, uint32_t np,
     float* __restrict__ tpart, float* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene,
-    uint32_t* __restrict__ pbound, uint32_t* __restrict__ chunk_lo, int skip_heavy) {
+    uint32_t* __restrict__ pbound, uint32_t* __restrict__ chunk_lo, const uint32_t* __restrict__ chunk_w,
+    uint32_t* __restrict__ chunk_runs, int skip_heavy) {
   constexpr int NT = 64 * P2G_WAVES;
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
   __shared__ double s_acc[NQ * HALO_NODES];  // integer-valued fixed-point sums
   // per-wave staging of P2G_CHUNK particles: for each axis the interleaved (w_o, q_o), o = 0..3
   // (24 floats), then m, m*v (4), then A row-major (9), 3 pad
   __shared__ __attribute__((aligned(16))) float s_stage[P2G_WAVES][P2G_CHUNK * P2G_STAGE];
-  __shared__ uint32_t s_val[P2G_SEG_MAX];        // slots of the chunk's particles
-  __shared__ uint8_t s_cell[P2G_SEG_MAX];        // their local cells (P2G_DEAD: out of grid)
+  __shared__ uint8_t s_cell[P2G_SEG_MAX];        // local cells of the chunk's particles (P2G_DEAD: out of grid)
   __shared__ uint16_t s_run[P2G_SEG_MAX + 1];    // run starts (chunk positions), then the end
   __shared__ uint32_t s_flag[P2G_SEG_MAX / 32];  // run-start bits
   __shared__ float s_bound[3];
@@ -439,18 +443,20 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   const int sid = chunk_scene[chunk];
   const SceneDev& S = scenes[sid];
   const uint32_t end = uint32_t(S.p_off + S.p_cnt);
-  const uint32_t w0 = uint32_t(S.p_off) + (chunk - S.chunk_off) * S.seg;  // < end
-  const uint32_t w1 = min(end, w0 + S.seg);
-  uint32_t lo = w0, hi = w1;
+  // the planned window [w0, w1) of the chunk (k_plan_chunks): w1 - w0 <= P2G_SEG_MAX - align_w
+  const uint32_t nch = uint32_t((S.p_cnt + S.seg - 1) / S.seg);
+  const uint32_t w0 = min(end, chunk_w[chunk]);
+  const uint32_t w1 = chunk + 1u < S.chunk_off + nch ? min(end, chunk_w[chunk + 1]) : end;
+  uint32_t lo = w0, hi = max(w0, w1);
   if (S.align_w) {  // uniform: both bounds to the first tile start in their window, if any
     if (t < 2) s_bnd[t] = 0xffffffffu;
     __syncthreads();
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const uint32_t w = side ? w1 : w0;
-      // the scene's start and end stay; a searched window starts at w >= p_off + seg, so
-      // keys[q - 1] is the scene's
-      const uint32_t qend = (side ? w1 < end : w0 > uint32_t(S.p_off)) ? min(end, w + S.align_w) : w;
+      // the scene's start and end stay; a searched window starts at w > p_off, so keys[q - 1]
+      // is the scene's
+      const uint32_t qend = (w > uint32_t(S.p_off) && w < end) ? min(end, w + S.align_w) : w;
       for (uint32_t q = w + uint32_t(t); q < qend; q += NT)
         if (keys[q] / TILE_CELLS != keys[q - 1] / TILE_CELLS) atomicMin(&s_bnd[side], q);
     }
@@ -458,7 +464,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     if (s_bnd[0] != 0xffffffffu) lo = s_bnd[0];
     if (s_bnd[1] != 0xffffffffu) hi = s_bnd[1];
   }
-  const uint32_t len = hi - lo;  // <= seg + align_w - 1 <= P2G_SEG_MAX
+  const uint32_t len = min(hi - lo, uint32_t(P2G_SEG_MAX));  // hi - lo <= w1 - w0 + align_w - 1
   if (t == 0) {
     chunk_lo[chunk] = lo;  // for the fold of continuation partials
     // tiles of the particles just before and after the chunk (same scene only)
@@ -476,7 +482,6 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
         const uint32_t key = keys[lo + o];
         const uint32_t tl = key / TILE_CELLS;
         flag = o == 0 || tl != keys[lo + o - 1] / TILE_CELLS;
-        s_val[o] = vals[lo + o];
         s_cell[o] = tl == S.dead_tile ? P2G_DEAD : uint8_t(key % TILE_CELLS);
       }
       const unsigned long long bits = __ballot(flag);
@@ -503,9 +508,12 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     for (int i = t; i < NQ * HALO_NODES; i += NT) s_acc[i] = 0.0;
   }
   __syncthreads();
-  if (wave == 0) {  // ordered run starts from the bit mask, one 32-position word per lane
-    uint32_t word = s_flag[lane];
-    const uint32_t cnt = __popc(word);
+  if (wave == 0) {  // ordered run starts from the bit mask, two 32-position words per lane
+    constexpr int WORDS = P2G_SEG_MAX / 32;
+    uint32_t wd[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) wd[h] = 2 * lane + h < WORDS ? s_flag[2 * lane + h] : 0u;
+    const uint32_t cnt = __popc(wd[0]) + __popc(wd[1]);
     uint32_t incl = cnt;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -513,10 +521,14 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       if (lane >= off) incl += y;
     }
     uint32_t o = incl - cnt;
-    while (word) {
-      const int b = __ffs(word) - 1;
-      word &= word - 1u;
-      s_run[o++] = uint16_t(32 * lane + b);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint32_t word = wd[h];
+      while (word) {
+        const int b = __ffs(word) - 1;
+        word &= word - 1u;
+        s_run[o++] = uint16_t(32 * (2 * lane + h) + b);
+      }
     }
     if (lane == 63) {
       s_nrun = incl;
@@ -529,6 +541,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
   __syncthreads();
   const uint32_t nrun = s_nrun;
+  if (t == 0) chunk_runs[chunk] = nrun;  // for the plan of the next P2G's windows
 
   // ---- the wave's batches over all runs, with the records of the next batch loaded ahead
   // cursor: run nr, batch [nc, min(nc + P2G_CHUNK, ne)) of the wave's share [., ne) of run nr
@@ -551,7 +564,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   uint32_t qcell = 0u;
   auto fetch = [&]() {
     if (nr < nrun && uint32_t(lane) < min(uint32_t(P2G_CHUNK), ne - nc)) {
-      const uint32_t p = s_val[nc + lane];
+      const uint32_t p = vals[lo + nc + lane];
       qcell = s_cell[nc + lane];
       q0 = rec[2 * size_t(p)];
       q1 = rec[2 * size_t(p) + 1];
@@ -697,6 +710,11 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
 // A tile whose particles span several P2G chunks has its first run in tpart[tile] and one
 // continuation partial per later chunk. The block of the tile's last chunk adds the
 // continuations to tpart[tile] in chunk order (double sums of the float partials).
+// Blocks total_chunks + s plan the chunk windows of scene s for the next P2G (plan_chunks).
+__device__ void plan_chunks(const SceneDev& S, const uint32_t* __restrict__ chunk_lo,
+                            const uint32_t* __restrict__ chunk_runs, unsigned long long* __restrict__ pref,
+                            uint32_t* __restrict__ chunk_w);
+
 __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restrict__ scenes,
                                                        const uint16_t* __restrict__ chunk_scene,
                                                        const uint32_t* __restrict__ keys,
@@ -704,25 +722,37 @@ __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restri
                                                        const uint32_t* __restrict__ tile_hi,
                                                        const uint32_t* __restrict__ chunk_lo,
                                                        float* __restrict__ tpart,
-                                                       const float* __restrict__ hpart) {
+                                                       const float* __restrict__ hpart, uint32_t total_chunks,
+                                                       const uint32_t* __restrict__ chunk_runs,
+                                                       unsigned long long* __restrict__ plan_pref,
+                                                       uint32_t* __restrict__ chunk_w) {
   const uint32_t chunk = blockIdx.x;
+  if (chunk >= total_chunks) {
+    plan_chunks(scenes[chunk - total_chunks], chunk_lo, chunk_runs, plan_pref, chunk_w);
+    return;
+  }
   const SceneDev& S = scenes[chunk_scene[chunk]];
   const uint32_t pbase = uint32_t(S.p_off);
   const uint32_t end = uint32_t(S.p_off + S.p_cnt);
-  const uint32_t j = chunk - S.chunk_off;
-  const uint32_t nch = (end - pbase + S.seg - 1) / S.seg;
+  const uint32_t nch = uint32_t((S.p_cnt + S.seg - 1) / S.seg);
+  const uint32_t last = S.chunk_off + nch - 1u;
+  auto chunk_hi = [&](uint32_t c) { return c < last ? chunk_lo[c + 1] : end; };
   const uint32_t lo = chunk_lo[chunk];  // the bounds the P2G used
-  const uint32_t hi = j + 1 < nch ? chunk_lo[chunk + 1] : end;
+  const uint32_t hi = chunk_hi(chunk);
+  if (lo >= hi) return;  // an empty chunk
   const uint32_t tile = keys[lo] / TILE_CELLS;
   if (tile == S.dead_tile) return;
   const uint32_t tlo = tile_lo[tile], thi = tile_hi[tile];
   if (tlo == lo || thi > hi) return;  // no continuation here, or a later chunk folds
-  // the chunk holding tlo: the one of its window, or the one before if the window's bound moved
-  // past tlo
-  uint32_t jt = (tlo - pbase) / S.seg;
-  if (tlo < chunk_lo[S.chunk_off + jt]) --jt;
-  const uint32_t c_first = S.chunk_off + jt + 1u;
+  // the chunk holding tlo: the last of the scene's chunks (up to this one) starting at or before it
+  uint32_t a = S.chunk_off, b = chunk;  // chunk_lo[a] = pbase <= tlo < lo = chunk_lo[b]
+  while (b - a > 1u) {
+    const uint32_t m = (a + b) >> 1;
+    if (chunk_lo[m] <= tlo) a = m; else b = m;
+  }
+  const uint32_t c_first = a + 1u;
   float* dst = tpart + size_t(tile) * (NQ * HALO_NODES);
+  // the chunks after the first that hold particles (each wrote a continuation partial)
   for (int i = threadIdx.x; i < NQ * HALO_NODES; i += blockDim.x) {
     double acc = double(dst[i]);
     // eight loads in flight at a time; the sum stays in chunk order (adding 0.0 is exact)
@@ -730,11 +760,85 @@ __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restri
       double v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        v[u] = c + u <= chunk ? double(hpart[size_t(c + u) * (NQ * HALO_NODES) + i]) : 0.0;
+        v[u] = (c + u <= chunk && chunk_lo[c + u] < chunk_hi(c + u))
+                   ? double(hpart[size_t(c + u) * (NQ * HALO_NODES) + i]) : 0.0;
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc += v[u];
     }
     dst[i] = float(acc);
+  }
+  (void)pbase;
+}
+
+// Chunk windows of scene S for the next P2G, from the runs its chunks had in this one. Chunk c
+// costs its particles plus k runs (k = P2G_RUN_COST, lowered where needed so that no window can
+// exceed P2G_SEG_MAX - align_w - 2 positions); the windows cut the cost, taken as uniform along
+// each chunk, into equal parts: window j starts where the cost prefix reaches j total / nch. A
+// window is at most total / nch + 2 <= that bound long, because a chunk's cost is at least its
+// length. The plan depends on the scene's own chunks only (the same alone or in a batch).
+__device__ void plan_chunks(const SceneDev& S, const uint32_t* __restrict__ chunk_lo,
+                            const uint32_t* __restrict__ chunk_runs, unsigned long long* __restrict__ pref,
+                            uint32_t* __restrict__ chunk_w) {
+  __shared__ unsigned long long s_red[32];
+  __shared__ unsigned long long s_carry;
+  const uint32_t n = uint32_t(S.p_cnt);
+  const uint32_t nch = (n + S.seg - 1) / S.seg;
+  if (nch == 0) return;
+  const uint32_t off = S.chunk_off, pbase = uint32_t(S.p_off), end = pbase + n;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+  auto len_of = [&](uint32_t c) {  // positions of chunk c (scene-relative index)
+    return (c + 1 < nch ? chunk_lo[off + c + 1] : end) - chunk_lo[off + c];
+  };
+  // block sum of 64-bit values (all threads call it)
+  auto block_sum = [&](unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) s_red[wave] = v;
+    __syncthreads();
+    unsigned long long r = 0;
+    for (int w = 0; w < nw; ++w) r += s_red[w];
+    return r;
+  };
+  unsigned long long runs = 0;
+  for (uint32_t c = t; c < nch; c += blockDim.x) runs += chunk_runs[off + c];
+  const unsigned long long R = block_sum(runs);
+  const unsigned long long wmax = uint64_t(P2G_SEG_MAX) - S.align_w - 2u;
+  const unsigned long long room = uint64_t(nch) * wmax > n ? uint64_t(nch) * wmax - n : 0u;
+  const unsigned long long k = R ? (room / R < P2G_RUN_COST ? room / R : P2G_RUN_COST) : 0u;
+  // exclusive prefix of cost(c) = len(c) + k runs(c), 1024 chunks at a time
+  if (t == 0) s_carry = 0;
+  for (uint32_t c0 = 0; c0 < nch; c0 += blockDim.x) {
+    const uint32_t c = c0 + t;
+    const unsigned long long v = c < nch ? len_of(c) + k * chunk_runs[off + c] : 0u;
+    unsigned long long incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned long long y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    __syncthreads();  // s_red and s_carry of the previous pass are read
+    if (lane == 63) s_red[wave] = incl;
+    __syncthreads();
+    unsigned long long before = s_carry;
+    for (int w = 0; w < wave; ++w) before += s_red[w];
+    if (c < nch) pref[off + c] = before + incl - v;
+    __syncthreads();
+    if (t == blockDim.x - 1) s_carry = before + incl;
+  }
+  __syncthreads();
+  const unsigned long long total = s_carry;
+  if (t == 0) chunk_w[off] = pbase;
+  for (uint32_t j = 1 + t; j < nch; j += blockDim.x) {
+    const unsigned long long target = total * j / nch;
+    uint32_t a = 0, b = nch;  // pref[a] <= target < pref[b] (pref[nch] = total)
+    while (b - a > 1u) {
+      const uint32_t m = (a + b) >> 1;
+      if (pref[off + m] <= target) a = m; else b = m;
+    }
+    const unsigned long long cost = (a + 1 < nch ? pref[off + a + 1] : total) - pref[off + a];
+    const uint32_t len = len_of(a);
+    const unsigned long long into = target - pref[off + a];
+    const uint32_t w = chunk_lo[off + a] + (cost ? uint32_t(into * len / cost) : 0u);
+    chunk_w[off + j] = w < end ? w : end;
   }
 }
 
