@@ -448,23 +448,30 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   const uint32_t w0 = min(end, chunk_w[chunk]);
   const uint32_t w1 = chunk + 1u < S.chunk_off + nch ? min(end, chunk_w[chunk + 1]) : end;
   uint32_t lo = w0, hi = max(w0, w1);
-  if (S.align_w) {  // uniform: both bounds to the first tile start in their window, if any
+  if (S.align_w) {  // uniform: both bounds to the nearest tile start within align_w / 2, if any
+    const uint32_t A = S.align_w >> 1, pb = uint32_t(S.p_off) + 1u;
+    // the scene's start and end stay; a searched bound w lies in (p_off, end), and its window
+    // [qlo(w), qhi(w)) in [p_off + 1, end], so keys[q - 1] is the scene's
+    auto qlo = [&](uint32_t w) { return w >= pb + A ? w - A : pb; };
     if (t < 2) s_bnd[t] = 0xffffffffu;
     __syncthreads();
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
       const uint32_t w = side ? w1 : w0;
-      // the scene's start and end stay; a searched window starts at w > p_off, so keys[q - 1]
-      // is the scene's
-      const uint32_t qend = (w > uint32_t(S.p_off) && w < end) ? min(end, w + S.align_w) : w;
-      for (uint32_t q = w + uint32_t(t); q < qend; q += NT)
-        if (keys[q] / TILE_CELLS != keys[q - 1] / TILE_CELLS) atomicMin(&s_bnd[side], q);
+      if (w > uint32_t(S.p_off) && w < end) {
+        const uint32_t q0 = qlo(w), q1 = min(end, w + A);
+        // key: distance to w, then position (ties go to the earlier start)
+        for (uint32_t q = q0 + uint32_t(t); q < q1; q += NT)
+          if (keys[q] / TILE_CELLS != keys[q - 1] / TILE_CELLS)
+            atomicMin(&s_bnd[side], ((q >= w ? q - w : w - q) << 20) | (q - q0));
+      }
     }
     __syncthreads();
-    if (s_bnd[0] != 0xffffffffu) lo = s_bnd[0];
-    if (s_bnd[1] != 0xffffffffu) hi = s_bnd[1];
+    if (s_bnd[0] != 0xffffffffu) lo = qlo(w0) + (s_bnd[0] & 0xfffffu);
+    if (s_bnd[1] != 0xffffffffu) hi = qlo(w1) + (s_bnd[1] & 0xfffffu);
+    hi = max(hi, lo);  // nearest-start rounding keeps the bounds in order; kept explicit
   }
-  const uint32_t len = min(hi - lo, uint32_t(P2G_SEG_MAX));  // hi - lo <= w1 - w0 + align_w - 1
+  const uint32_t len = min(hi - lo, uint32_t(P2G_SEG_MAX));  // hi - lo < w1 - w0 + align_w
   if (t == 0) {
     chunk_lo[chunk] = lo;  // for the fold of continuation partials
     // tiles of the particles just before and after the chunk (same scene only)
@@ -779,13 +786,16 @@ __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restri
 __device__ void plan_chunks(const SceneDev& S, const uint32_t* __restrict__ chunk_lo,
                             const uint32_t* __restrict__ chunk_runs, unsigned long long* __restrict__ pref,
                             uint32_t* __restrict__ chunk_w) {
+  constexpr uint32_t LDS_CHUNKS = 4096;  // cost prefixes kept in LDS up to this many chunks
   __shared__ unsigned long long s_red[32];
   __shared__ unsigned long long s_carry;
+  __shared__ unsigned long long s_pref[LDS_CHUNKS];
   const uint32_t n = uint32_t(S.p_cnt);
   const uint32_t nch = (n + S.seg - 1) / S.seg;
   if (nch == 0) return;
   const uint32_t off = S.chunk_off, pbase = uint32_t(S.p_off), end = pbase + n;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+  unsigned long long* pf = nch <= LDS_CHUNKS ? s_pref : pref + off;  // pf[c]: cost before chunk c
   auto len_of = [&](uint32_t c) {  // positions of chunk c (scene-relative index)
     return (c + 1 < nch ? chunk_lo[off + c + 1] : end) - chunk_lo[off + c];
   };
@@ -820,7 +830,7 @@ __device__ void plan_chunks(const SceneDev& S, const uint32_t* __restrict__ chun
     __syncthreads();
     unsigned long long before = s_carry;
     for (int w = 0; w < wave; ++w) before += s_red[w];
-    if (c < nch) pref[off + c] = before + incl - v;
+    if (c < nch) pf[c] = before + incl - v;
     __syncthreads();
     if (t == blockDim.x - 1) s_carry = before + incl;
   }
@@ -832,11 +842,11 @@ __device__ void plan_chunks(const SceneDev& S, const uint32_t* __restrict__ chun
     uint32_t a = 0, b = nch;  // pref[a] <= target < pref[b] (pref[nch] = total)
     while (b - a > 1u) {
       const uint32_t m = (a + b) >> 1;
-      if (pref[off + m] <= target) a = m; else b = m;
+      if (pf[m] <= target) a = m; else b = m;
     }
-    const unsigned long long cost = (a + 1 < nch ? pref[off + a + 1] : total) - pref[off + a];
+    const unsigned long long cost = (a + 1 < nch ? pf[a + 1] : total) - pf[a];
     const uint32_t len = len_of(a);
-    const unsigned long long into = target - pref[off + a];
+    const unsigned long long into = target - pf[a];
     const uint32_t w = chunk_lo[off + a] + (cost ? uint32_t(into * len / cost) : 0u);
     chunk_w[off + j] = w < end ? w : end;
   }
