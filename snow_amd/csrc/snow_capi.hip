@@ -362,9 +362,7 @@ int run_sort(snow_ctx* c, float dt_rigid) {
   return tm.done();
 }
 
-// plan: also plan the chunk windows of the next P2G from this one's runs (the fused step and
-// init do; the per-stage API keeps the current plan, so its P2G leaves later steps unchanged)
-int run_p2g(snow_ctx* c, int mass_only, bool plan) {
+int run_p2g(snow_ctx* c, int mass_only) {
   if (!c->bound_valid) {  // records no writer published (after the step start: this step's parity)
     const int rc = publish_bounds(c, 0u, mass_only ? 0 : 1);
     if (rc) return rc;
@@ -377,26 +375,29 @@ int run_p2g(snow_ctx* c, int mass_only, bool plan) {
         c->d_hpart, c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs,
         mass_only ? 0 : 1);
     LAUNCHCHK(c);
-    // the fold of continuation partials, and one more block per scene that plans the chunk
-    // windows of the next P2G
-    k_fold_partials<<<c->total_chunks + (plan ? uint32_t(c->nscenes) : 0u), 1024, 0, c->stream>>>(
-        c->d_scenes, c->d_chunk_scene, c->d_keys[1], c->d_tile_lo, c->d_tile_hi, c->d_chunk_lo, c->d_tpart,
-        c->d_hpart, c->total_chunks, c->d_chunk_runs, c->d_plan_pref, c->d_chunk_w);
+    k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
+                                                            c->d_tile_lo, c->d_tile_hi, c->d_chunk_lo, c->d_tpart,
+                                                            c->d_hpart);
     LAUNCHCHK(c);
   }
   c->bound_valid = false;  // consumed; the slots of the other parity are cleared for the writers
   return tm.done();
 }
 
-int run_grid(snow_ctx* c, bool keep, bool update, float dt) {
+// plan: the launch also plans the P2G chunk windows of the next step from the runs of the P2G
+// just done (the fused step and init do; the per-stage API keeps the current plan, so its calls
+// leave later steps unchanged)
+int run_grid(snow_ctx* c, bool keep, bool update, float dt, bool plan) {
   StageTimer tm(c, ST_GRID);
   unsigned long long* cnt = c->d_counts;
   if (c->total_chunks == 0) return tm.done();
-  const unsigned blocks = std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu * 4)));
+  const unsigned nplan = plan ? unsigned(c->nscenes) : 0u;
+  const unsigned blocks = nplan + std::max(1u, std::min<unsigned>(c->total_ctiles, unsigned(c->num_cu * 4)));
 #define GRID_ARGS                                                                                          \
   c->d_scenes, c->nscenes == 1 ? nullptr : c->d_tile_scene, c->d_list_ctr, c->d_tile_list, c->total_ctiles,    \
       c->d_ctl, c->d_tile_stamp,                                                                              \
-      c->d_tpart, c->d_zero_part, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass, c->d_cols, dt, cnt
+      c->d_tpart, c->d_zero_part, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass, c->d_cols, dt, cnt, nplan,  \
+      c->d_chunk_lo, c->d_chunk_runs, c->d_plan_pref, c->d_chunk_w
   if (keep && update)
     k_grid<true, true><<<blocks, GRID_THREADS, 0, c->stream>>>(GRID_ARGS);
   else if (update)
@@ -442,9 +443,9 @@ int step_impl(snow_ctx* c, float dt) {
   int rc;
   if ((rc = run_force_prepass(c))) return rc;
   if ((rc = run_sort(c, dt))) return rc;
-  if ((rc = run_p2g(c, 0, true))) return rc;
+  if ((rc = run_p2g(c, 0))) return rc;
   const bool keep = (c->flags & SNOW_FLAG_KEEP_GRID) != 0;
-  if ((rc = run_grid(c, keep, true, dt))) return rc;
+  if ((rc = run_grid(c, keep, true, dt, true))) return rc;
   if ((rc = run_g2p_advance(c, dt))) return rc;
   c->grid_state = 1;
   c->keep_valid = keep;
@@ -781,8 +782,8 @@ int snow_stage_grid_mass(snow_ctx* c) {
   c->bound_valid = true;
   c->force_valid = false;
   if ((rc = run_sort(c, -1.0f))) return rc;
-  if ((rc = run_p2g(c, 1, true))) return rc;
-  if ((rc = run_grid(c, true, false, 0.0f))) return rc;
+  if ((rc = run_p2g(c, 1))) return rc;
+  if ((rc = run_grid(c, true, false, 0.0f, true))) return rc;
   c->grid_state = 1;
   c->keep_valid = true;
   return 0;
@@ -847,8 +848,8 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
       c->cur = p;
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       rc = run_sort(c, 0.0f);
-      if (!rc) rc = run_p2g(c, 0, true);
-      if (!rc) rc = run_grid(c, (c->flags & SNOW_FLAG_KEEP_GRID) != 0, true, 0.0f);
+      if (!rc) rc = run_p2g(c, 0);
+      if (!rc) rc = run_grid(c, (c->flags & SNOW_FLAG_KEEP_GRID) != 0, true, 0.0f, true);
       if (!rc) rc = run_g2p_advance(c, 0.0f);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(c->stream, &g);
@@ -1021,8 +1022,8 @@ int snow_stage_p2g(snow_ctx* c) {
   int rc;
   if ((rc = run_force_prepass(c))) return rc;
   if ((rc = run_sort(c, -1.0f))) return rc;
-  if ((rc = run_p2g(c, 0, false))) return rc;
-  if ((rc = run_grid(c, true, false, 0.0f))) return rc;
+  if ((rc = run_p2g(c, 0))) return rc;
+  if ((rc = run_grid(c, true, false, 0.0f, false))) return rc;
   c->grid_state = 1;
   c->keep_valid = true;
   return 0;
@@ -1034,7 +1035,7 @@ int snow_stage_grid_update(snow_ctx* c, double dt) {
   if (!(fdt > 0.0f)) return fail(c, SNOW_E_ARG, "dt must be > 0");
   // after snow_stage_p2g: the tile merge + update of snow_step (the dense node fields the P2G stage
   // kept stay valid)
-  if (c->grid_state == 1) return run_grid(c, false, true, fdt);
+  if (c->grid_state == 1) return run_grid(c, false, true, fdt, false);
   if (c->grid_state != 2) return fail(c, SNOW_E_STATE, "grid update needs a P2G result or a written grid");
   StageTimer tm(c, ST_GRID);
   if (c->total_nodes) {

@@ -717,11 +717,6 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
 // A tile whose particles span several P2G chunks has its first run in tpart[tile] and one
 // continuation partial per later chunk. The block of the tile's last chunk adds the
 // continuations to tpart[tile] in chunk order (double sums of the float partials).
-// Blocks total_chunks + s plan the chunk windows of scene s for the next P2G (plan_chunks).
-__device__ void plan_chunks(const SceneDev& S, const uint32_t* __restrict__ chunk_lo,
-                            const uint32_t* __restrict__ chunk_runs, unsigned long long* __restrict__ pref,
-                            uint32_t* __restrict__ chunk_w);
-
 __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restrict__ scenes,
                                                        const uint16_t* __restrict__ chunk_scene,
                                                        const uint32_t* __restrict__ keys,
@@ -729,15 +724,8 @@ __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restri
                                                        const uint32_t* __restrict__ tile_hi,
                                                        const uint32_t* __restrict__ chunk_lo,
                                                        float* __restrict__ tpart,
-                                                       const float* __restrict__ hpart, uint32_t total_chunks,
-                                                       const uint32_t* __restrict__ chunk_runs,
-                                                       unsigned long long* __restrict__ plan_pref,
-                                                       uint32_t* __restrict__ chunk_w) {
+                                                       const float* __restrict__ hpart) {
   const uint32_t chunk = blockIdx.x;
-  if (chunk >= total_chunks) {
-    plan_chunks(scenes[chunk - total_chunks], chunk_lo, chunk_runs, plan_pref, chunk_w);
-    return;
-  }
   const SceneDev& S = scenes[chunk_scene[chunk]];
   const uint32_t pbase = uint32_t(S.p_off);
   const uint32_t end = uint32_t(S.p_off + S.p_cnt);
@@ -880,7 +868,18 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
                                                        float4* __restrict__ g_mmv, float4* __restrict__ g_f,
                                                        float2* __restrict__ g_node, float* __restrict__ g_mass,
                                                        const Collider* __restrict__ cols, float dt_override,
-                                                       unsigned long long* __restrict__ active_count) {
+                                                       unsigned long long* __restrict__ active_count,
+                                                       uint32_t nplan, const uint32_t* __restrict__ chunk_lo,
+                                                       const uint32_t* __restrict__ chunk_runs,
+                                                       unsigned long long* __restrict__ plan_pref,
+                                                       uint32_t* __restrict__ chunk_w) {
+  // blocks 0 .. nplan-1 plan the P2G chunk windows of scene blockIdx.x for the next step (first,
+  // so that they run beside the merge blocks rather than after them)
+  if (blockIdx.x < nplan) {
+    plan_chunks(scenes[blockIdx.x], chunk_lo, chunk_runs, plan_pref, chunk_w);
+    return;
+  }
+  const uint32_t bid = blockIdx.x - nplan, nblk = gridDim.x - nplan;
   const uint32_t step = ctl->step;
   const int t = threadIdx.x;
   const int a = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -901,11 +900,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
     const uint32_t first = uint32_t(__builtin_amdgcn_readlane(int(incl - cnt), sub));
     return tile_list[size_t(sub) * list_cap + (e - first)];
   };
-  uint32_t k_next = blockIdx.x < total ? entry(blockIdx.x) : 0u;
-  for (uint32_t e = blockIdx.x; e < total; e += gridDim.x) {
+  uint32_t k_next = bid < total ? entry(bid) : 0u;
+  for (uint32_t e = bid; e < total; e += nblk) {
     const uint32_t k = k_next;
     // the next entry's list slot loads while this tile is merged
-    if (e + gridDim.x < total) k_next = entry(e + gridDim.x);
+    if (e + nblk < total) k_next = entry(e + nblk);
     const int sid = tile_scene ? tile_scene[k] : 0;  // no table: one scene, whose record loads at once
     const SceneDev& S = scenes[sid];
     const uint32_t kl = k - S.ct_off;
