@@ -573,7 +573,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
       while (seg < uint32_t(P2G_SEG_MEAN) && uint64_t(seg) * 2 * 64 <= uint64_t(d.num_particles)) seg *= 2;
       S.seg = seg;
       // bounds move to the first tile start within align_w positions of the planned window
-      S.align_w = std::min(seg, 512u);
+      S.align_w = std::min(seg, 256u);
       S.chunk_off = uint32_t(chunk_off);
       chunk_off += (uint64_t(d.num_particles) + seg - 1) / seg;
     }
