@@ -110,10 +110,16 @@ def valu_floor(kernel: str, measured_us: float):
     rows = {}
     try:
         with open(INSTR_SUMMARY) as f:
+            col = None  # the INSTS_VALU column while inside the table that has one
             for line in f:
+                if not line.startswith("|"):
+                    col = None
+                    continue
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cells) >= 3:
-                    rows[cells[0]] = cells[2]
+                if "INSTS_VALU" in cells:
+                    col = cells.index("INSTS_VALU")
+                elif col is not None and len(cells) > col and not cells[0].startswith("-"):
+                    rows[cells[0]] = cells[col]
         names = [PLAIN_STEP_KERNEL.get(k) for k in STAGE_LAUNCHES.get(kernel, (kernel,))]
         if any(nm not in rows for nm in names):
             return None
