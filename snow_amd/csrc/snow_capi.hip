@@ -87,7 +87,7 @@ struct snow_ctx {
   uint32_t* d_slot = nullptr;  // slot of the particle at each sorted position (the sort's permutation)
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
-  uint32_t *d_tile_lo = nullptr, *d_tile_hi = nullptr;
+  uint32_t *d_tile_c0 = nullptr, *d_tile_c1 = nullptr;  // first and last P2G chunk of each tile's particles
   float* d_tpart = nullptr;
   float* d_hpart = nullptr;
   float* d_zero_part = nullptr;  // NQ * HALO_NODES zeros
@@ -370,13 +370,13 @@ int run_p2g(snow_ctx* c, int mass_only) {
   StageTimer tm(c, ST_P2G);
   if (c->total_chunks) {
     k_p2g_chunk<<<c->total_chunks, 64 * P2G_WAVES, 0, c->stream>>>(
-        c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_slot, c->d_tile_lo, c->d_tile_hi, c->d_tile_stamp,
+        c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_slot, c->d_tile_c0, c->d_tile_c1, c->d_tile_stamp,
         c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_fa, c->N, c->d_tpart,
         c->d_hpart, c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs,
         mass_only ? 0 : 1);
     LAUNCHCHK(c);
     k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
-                                                            c->d_tile_lo, c->d_tile_hi, c->d_chunk_lo, c->d_tpart,
+                                                            c->d_tile_c0, c->d_tile_c1, c->d_chunk_lo, c->d_tpart,
                                                             c->d_hpart);
     LAUNCHCHK(c);
   }
@@ -461,7 +461,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_fa, c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp,
-                  c->d_tile_lo, c->d_tile_hi, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
+                  c->d_tile_c0, c->d_tile_c1, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -642,8 +642,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
     c->sort_tmp_bytes = tb;
     CREATECHK(hipMalloc(&c->d_sort_tmp, std::max<size_t>(tb, 16)));
   }
-  CREATECHK(dalloc(&c->d_tile_lo, c->total_ctiles));
-  CREATECHK(dalloc(&c->d_tile_hi, c->total_ctiles));
+  CREATECHK(dalloc(&c->d_tile_c0, c->total_ctiles));
+  CREATECHK(dalloc(&c->d_tile_c1, c->total_ctiles));
   CREATECHK(dalloc(&c->d_tpart, size_t(c->total_ctiles) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_hpart, size_t(std::max<uint32_t>(1, c->total_chunks)) * NQ * HALO_NODES));
   CREATECHK(dalloc(&c->d_chunk_scene, std::max<uint32_t>(1, c->total_chunks)));
@@ -736,8 +736,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
       CREATECHK(hipMemset(c->d_fa, 0, sizeof(float) * 9 * size_t(N)));
     }
   }
-  CREATECHK(hipMemset(c->d_tile_lo, 0, sizeof(uint32_t) * c->total_ctiles));
-  CREATECHK(hipMemset(c->d_tile_hi, 0, sizeof(uint32_t) * c->total_ctiles));
+  CREATECHK(hipMemset(c->d_tile_c0, 0, sizeof(uint32_t) * c->total_ctiles));
+  CREATECHK(hipMemset(c->d_tile_c1, 0, sizeof(uint32_t) * c->total_ctiles));
   CREATECHK(hipMemset(c->d_ctl, 0, sizeof(StepCtl)));
   CREATECHK(hipMemset(c->d_pbound, 0, sizeof(uint32_t) * c->nscenes * 2 * PB_SLOTS * PB_STRIDE));
   CREATECHK(hipMemset(c->d_tile_stamp, 0, sizeof(uint32_t) * c->total_ctiles));

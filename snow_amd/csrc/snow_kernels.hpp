@@ -412,7 +412,7 @@ constexpr uint8_t P2G_DEAD = 0xff;   // local-cell mark of out-of-grid particles
 __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     const SceneDev* __restrict__ scenes, const uint16_t* __restrict__ tile_scene,
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-    uint32_t* __restrict__ tile_lo, uint32_t* __restrict__ tile_hi, uint32_t* __restrict__ tile_stamp,
+    uint32_t* __restrict__ tile_c0, uint32_t* __restrict__ tile_c1, uint32_t* __restrict__ tile_stamp,
     const StepCtl* __restrict__ ctl, uint32_t* __restrict__ list_ctr, uint32_t* __restrict__ tile_list,
     uint32_t list_cap, const float4* __restrict__ rec, const float* __restrict__ fa, uint32_t np,
     float* __restrict__ tpart, float* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene,
@@ -682,8 +682,8 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   }
 
   // ---- the tiles of the chunk's runs, once for the whole chunk, so that no run waits on the list
-  // atomic: the tiles a run starts get their first position and this step's stamp and join a tile
-  // sublist (one atomic per 64 runs), the tiles a run ends their end position
+  // atomic: the tiles a run starts get this chunk as their first and this step's stamp and join a
+  // tile sublist (one atomic per 64 runs), the tiles a run ends this chunk as their last
   if (wave == 0) {
     const uint32_t sub = chunk % TILE_LISTS;
     for (uint32_t r0 = 0; r0 < nrun; r0 += 64) {
@@ -691,15 +691,15 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
       bool starts = false;
       uint32_t tile = 0u;
       if (r < nrun) {
-        const uint32_t r_lo = s_run[r], r_hi = s_run[r + 1];
+        const uint32_t r_lo = s_run[r];
         if (s_cell[r_lo] != P2G_DEAD) {
           tile = keys[lo + r_lo] / TILE_CELLS;
           starts = r > 0 || s_edge[0] != tile;
           if (starts) {
-            tile_lo[tile] = lo + r_lo;
+            tile_c0[tile] = chunk;  // the chunks of the tile's first and last runs, for the fold
             tile_stamp[tile] = step;
           }
-          if (r + 1 < nrun || s_edge[1] != tile) tile_hi[tile] = lo + r_hi;
+          if (r + 1 < nrun || s_edge[1] != tile) tile_c1[tile] = chunk;
         }
       }
       const uint64_t mask = __ballot(starts);
@@ -720,38 +720,29 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
 __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restrict__ scenes,
                                                        const uint16_t* __restrict__ chunk_scene,
                                                        const uint32_t* __restrict__ keys,
-                                                       const uint32_t* __restrict__ tile_lo,
-                                                       const uint32_t* __restrict__ tile_hi,
+                                                       const uint32_t* __restrict__ tile_c0,
+                                                       const uint32_t* __restrict__ tile_c1,
                                                        const uint32_t* __restrict__ chunk_lo,
                                                        float* __restrict__ tpart,
                                                        const float* __restrict__ hpart) {
   const uint32_t chunk = blockIdx.x;
   const SceneDev& S = scenes[chunk_scene[chunk]];
-  const uint32_t pbase = uint32_t(S.p_off);
   const uint32_t end = uint32_t(S.p_off + S.p_cnt);
-  const uint32_t nch = uint32_t((S.p_cnt + S.seg - 1) / S.seg);
-  const uint32_t last = S.chunk_off + nch - 1u;
+  const uint32_t last = S.chunk_off + uint32_t((S.p_cnt + S.seg - 1) / S.seg) - 1u;
   auto chunk_hi = [&](uint32_t c) { return c < last ? chunk_lo[c + 1] : end; };
   const uint32_t lo = chunk_lo[chunk];  // the bounds the P2G used
-  const uint32_t hi = chunk_hi(chunk);
-  if (lo >= hi) return;  // an empty chunk
+  if (lo >= chunk_hi(chunk)) return;   // an empty chunk
   const uint32_t tile = keys[lo] / TILE_CELLS;
   if (tile == S.dead_tile) return;
-  const uint32_t tlo = tile_lo[tile], thi = tile_hi[tile];
-  if (tlo == lo || thi > hi) return;  // no continuation here, or a later chunk folds
-  // the chunk holding tlo: the last of the scene's chunks (up to this one) starting at or before it
-  uint32_t a = S.chunk_off, b = chunk;  // chunk_lo[a] = pbase <= tlo < lo = chunk_lo[b]
-  while (b - a > 1u) {
-    const uint32_t m = (a + b) >> 1;
-    if (chunk_lo[m] <= tlo) a = m; else b = m;
-  }
-  const uint32_t c_first = a + 1u;
+  const uint32_t c0 = tile_c0[tile];
+  if (c0 == chunk || tile_c1[tile] != chunk) return;  // no continuation here, or a later chunk folds
   float* dst = tpart + size_t(tile) * (NQ * HALO_NODES);
-  // the chunks after the first that hold particles (each wrote a continuation partial)
+  // the chunks after the first that hold particles (each wrote a continuation partial; a chunk
+  // whose planned window was empty wrote nothing)
   for (int i = threadIdx.x; i < NQ * HALO_NODES; i += blockDim.x) {
     double acc = double(dst[i]);
     // eight loads in flight at a time; the sum stays in chunk order (adding 0.0 is exact)
-    for (uint32_t c = c_first; c <= chunk; c += 8) {
+    for (uint32_t c = c0 + 1u; c <= chunk; c += 8) {
       double v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
@@ -762,7 +753,6 @@ __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restri
     }
     dst[i] = float(acc);
   }
-  (void)pbase;
 }
 
 // Chunk windows of scene S for the next P2G, from the runs its chunks had in this one. Chunk c
