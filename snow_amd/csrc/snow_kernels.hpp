@@ -46,7 +46,7 @@ struct SceneDev {
   int nt[3];          // node tiles per axis
   uint32_t nt_off;    // global index of the scene's first node tile
   uint32_t dead_tile; // global index of the scene's out-of-grid tile
-  uint32_t seg;       // P2G chunk of the scene, in particles
+  uint32_t seg;       // mean P2G chunk of the scene, in particles: ceil(p_cnt / seg) chunks
   int64_t node_off;   // offset of node 0 in the dense node arrays
   int64_t p_off;      // first particle
   int64_t p_cnt;      // particle count
@@ -54,8 +54,8 @@ struct SceneDev {
   float inv_h;        // 1/h
   uint32_t chunk_off; // global index of the scene's first P2G chunk
   float uo[3];        // -origin/h: grid coordinate u = x/h + uo
-  uint32_t align_w;   // P2G chunk bounds move to the first cell-tile start within this many
-                      // positions of p_off + c*seg (0: they stay there)
+  uint32_t align_w;   // a P2G chunk bound moves to the nearest cell-tile start within
+                      // align_w / 2 positions of its planned position (0: it stays there)
 };
 
 // Element i of an SoA plane addressed by the byte offset 4*i in 32 bits (a batch holds fewer than
@@ -211,13 +211,14 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
 // ---------------------------------------------------------------------------------------
 // P2G (p2gTransfer.glsl:44-151 / gMassCompute.glsl:33-84) over chunks of the sorted particles.
 // A scene's particles are contiguous in sorted order; its chunk c (global index chunk_off + c)
-// is the sorted range [b(c), b(c+1)) with b(c) = p_off + c*seg, so every workgroup gets about the
-// same number of particles however unevenly they fill the cell tiles, and a scene's chunks are
-// the same alone or in a batch. Where seg + align_w <= P2G_SEG_MAX (scenes with small chunks),
-// b(c) moves forward to the first cell-tile start within align_w positions, so that a tile
-// is split between chunks only when it is longer than that: no continuation partial, no fold.
-// The workgroup first reads the chunk's keys and slots once, all loads in flight together: it
-// keeps the slots and local cells in LDS and marks where the cell tile changes (the chunk falls
+// is a sorted range of about seg particles whose window the chunk plan (plan_chunks) sets from
+// the previous fused step, so that every workgroup gets about the same cost (particles plus a
+// fixed cost per run of one tile) however unevenly the particles fill the cell tiles; a scene's
+// chunks are the same alone or in a batch. Both window bounds move to the nearest cell-tile
+// start within align_w / 2 positions, so that a tile is split between chunks only when it is
+// longer than that: no continuation partial, no fold.
+// The workgroup first reads the chunk's keys once, all loads in flight together: it keeps the
+// local cells in LDS and marks where the cell tile changes (the chunk falls
 // apart into runs of one tile, contiguous and cell by cell); the fixed-point bound is the
 // scene's, published by the writers of the records (below). Then, run by run, the 8 waves take
 // equal contiguous shares and stage P2G_CHUNK particles at a time into their LDS slot: one lane
@@ -232,8 +233,8 @@ __global__ void __launch_bounds__(256) k_rigid_advance(const SceneDev* __restric
 // sum can reach 2^FIX_BITS and the resolution is ~2^-39 of the scene's largest term for a
 // 2048-particle chunk.
 // A run that starts its tile writes the tile partial tpart[tile]; a run that continues a tile
-// begun in an earlier chunk is the chunk's first run and writes hpart[chunk]. The grid merge
-// adds tpart[tile] and the hpart of the tile's later chunks in chunk order.
+// begun in an earlier chunk is the chunk's first run and writes hpart[chunk]. The fold adds the
+// hpart of the tile's later chunks to tpart[tile] in chunk order, before the grid merge.
 struct P2GBound { float m, p, f; };
 
 // The largest per-particle terms of the P2G sums: |m|, |m v_k| and the absolute row sums of A
