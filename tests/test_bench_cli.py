@@ -41,12 +41,17 @@ def test_stage_figures_cover_every_launch_of_the_stage():
     if p2g is not None and fold is not None:
         assert bench.stage_traffic("k_p2g_chunk") == p2g + fold
     assert bench.stage_traffic("k_g2p_advance") == bench.measured_traffic("k_g2p_advance")
-    rows = {}
+    rows, col = {}, None  # the INSTS_VALU column of the instruction-counter table
     with open(bench.INSTR_SUMMARY) as f:
         for line in f:
+            if not line.startswith("|"):
+                col = None
+                continue
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) >= 3 and cells[0].startswith("snow::"):
-                rows[cells[0]] = int(cells[2])
+            if "INSTS_VALU" in cells:
+                col = cells.index("INSTS_VALU")
+            elif col is not None and cells[0].startswith("snow::"):
+                rows[cells[0]] = int(cells[col])
     vf = bench.valu_floor("k_p2g_chunk", 100.0)
     assert vf["instructions"] == rows["snow::k_p2g_chunk"] + rows["snow::k_fold_partials"]
     assert bench.valu_floor("k_grid", 100.0)["instructions"] == rows["snow::k_grid<false, true>"]
