@@ -20,7 +20,10 @@ FP32 form's particle-steps/s. The same line also reports
 Prints ONE JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-errors] [--no-sweep]
-                    [--no-second-form] [--no-stress] [--large]
+                    [--no-second-form] [--no-stress] [--large] [--no-autotune]
+
+Before its W warm-up steps the headline context runs snow_autotune (launch shapes that leave the
+results unchanged, timed on a snapshot it restores); config.launch_shape reports its choice.
 
 Kernel times come from HIP events around the reference's stages. The stage named k_p2g_chunk
 ("Transfer to Grid") spans k_p2g_chunk and k_fold_partials; `launches` lists what each measured
@@ -397,6 +400,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of snow_autotune's choice")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-second-form", action="store_true", help="skip the mixed (or plain) second form")
     ap.add_argument("--no-stress", action="store_true", help="skip the 2^24-particle slab stress scene")
@@ -432,6 +437,9 @@ def main():
     ctx = SnowContext(scene, flags)
     ctx.init()
     dt = scene.dt
+    # the launch shapes that leave the results unchanged (G2P block, sort digit), timed by
+    # snow_autotune on a snapshot of the state that it restores afterwards
+    shape = None if args.no_autotune else ctx.autotune(steps=5)
     for _ in range(args.warmup):
         ctx.step(dt)
     ctx.sync()
@@ -489,7 +497,7 @@ def main():
         "dtype": "fp32" if args.form == "plain" else "fp32+fp64-svd",
         "data": "synthetic (seeded SplitMix64 particle sampling)",
         "config": {"workload": "two_snowballs_128^3", "particles": n, "grid": [128, 128, 128], "dt": dt,
-                   "form": args.form, "active_nodes": active},
+                   "form": args.form, "active_nodes": active, "launch_shape": shape},
         "roofline": {"bound": "hbm", "kernel": dom, "launches": kernels[dom]["launches"], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": stage_traffic(dom), "valu_floor": kernels[dom].get("valu_floor")},
