@@ -373,7 +373,7 @@ int run_p2g(snow_ctx* c, int mass_only) {
         c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_slot, c->d_tile_c0, c->d_tile_c1, c->d_tile_stamp,
         c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_fa, c->N, c->d_tpart,
         c->d_hpart, c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs,
-        mass_only ? 0 : 1);
+        mass_only ? 0 : 1, c->total_chunks <= uint32_t(P2G_BLOCKS_PER_CU * c->num_cu) ? 1 : 0);
     LAUNCHCHK(c);
     k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
                                                             c->d_tile_c0, c->d_tile_c1, c->d_chunk_lo, c->d_tpart,

@@ -27,6 +27,7 @@ constexpr int HALO_NODES = 343;      // 7^3
 constexpr int NQ = 7;                // m, p.xyz, f.xyz
 constexpr int P2G_WAVES = 8;         // waves per P2G workgroup
 constexpr int P2G_SEG_MAX = 3072;    // largest P2G chunk in particles (sizes the LDS run lists)
+constexpr int P2G_BLOCKS_PER_CU = 2; // P2G workgroups resident per CU (about 70 KB of LDS each)
 constexpr int P2G_SEG_MEAN = 2048;   // largest mean chunk of a scene (its particles / its chunks)
 constexpr uint32_t P2G_RUN_COST = 96; // planning cost of a run (tile) in particles: its barriers,
                                       // partial write and short batches (measured: ~2.6 us per
@@ -418,7 +419,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     uint32_t list_cap, const float4* __restrict__ rec, const float* __restrict__ fa, uint32_t np,
     float* __restrict__ tpart, float* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene,
     uint32_t* __restrict__ pbound, uint32_t* __restrict__ chunk_lo, const uint32_t* __restrict__ chunk_w,
-    uint32_t* __restrict__ chunk_runs, int skip_heavy) {
+    uint32_t* __restrict__ chunk_runs, int skip_heavy, int by_progress) {
   constexpr int NT = 64 * P2G_WAVES;
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
   __shared__ double s_acc[NQ * HALO_NODES];  // integer-valued fixed-point sums
@@ -581,6 +582,19 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   };
   seek(0);
   fetch();
+  // Issue priority by progress, when every workgroup of the launch is resident at once (by_progress):
+  // a CU then holds two of them to the end, and its arbiter prefers older waves, so without this
+  // the first workgroup runs ahead and the second finishes alone at half the occupancy. A wave's
+  // priority falls by one at each quarter of its share, so the waves of a CU stay within a quarter
+  // of each other. With more workgroups than slots the oldest-first order frees slots soonest and
+  // is kept.
+  const uint32_t share = len / P2G_WAVES + 1u;
+  uint32_t done = 0u;
+  int prio = 0;
+  if (by_progress) {
+    prio = 3;
+    __builtin_amdgcn_s_setprio(3);
+  }
 
   for (uint32_t r = 0; r < nrun; ++r) {
     const uint32_t r_lo = s_run[r], r_hi = s_run[r + 1];
@@ -608,6 +622,16 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     };
     while (nr == r) {  // the wave's batches of this run
       const uint32_t cnt = min(uint32_t(P2G_CHUNK), ne - nc);
+      if (by_progress) {
+        const int want = 3 - int(min(3u, (4u * done) / share));
+        if (want != prio) {
+          prio = want;
+          if (want == 2) __builtin_amdgcn_s_setprio(2);
+          else if (want == 1) __builtin_amdgcn_s_setprio(1);
+          else __builtin_amdgcn_s_setprio(0);
+        }
+      }
+      done += cnt;
       // stage: lane k < cnt evaluates the weights of its prefetched particle once
       const uint32_t cellv = uint32_t(lane) < cnt ? qcell : 0xffffffffu;
       if (uint32_t(lane) < cnt) {
