@@ -140,3 +140,58 @@ int check(const C* c) {
 
 ORACLE_API(double, oracle_scene64, oracle64)
 ORACLE_API(float, oracle_scene32, oracle32)
+
+// ---- semi-implicit update (FP64 only) ---------------------------------------------------------
+namespace {
+oracle::Mat3<double> mat_in(const double* a) { oracle::Mat3<double> m; std::memcpy(m.m, a, sizeof(m.m)); return m; }
+}
+extern "C" void oracle64_fc_stress(const double* F, double mu, double la, double* P) {
+  const auto r = oracle::fc_stress(mat_in(F), mu, la);
+  std::memcpy(P, r.m, sizeof(r.m));
+}
+extern "C" void oracle64_fc_hessian(const double* F, double mu, double la, const double* dF, double* A) {
+  const auto r = oracle::fc_hessian_action(mat_in(F), mu, la, mat_in(dF));
+  std::memcpy(A, r.m, sizeof(r.m));
+}
+extern "C" void oracle64_polar_exact(const double* F, double* R) {
+  const auto r = oracle::polar_newton(mat_in(F));
+  std::memcpy(R, r.m, sizeof(r.m));
+}
+extern "C" int oracle64_implicit_df(oracle_scene64* c, const double* du, double* out) {
+  if (check(c) || !du || !out) return -1;
+  auto s = wrap<double>(c);
+  oracle::implicit_df(s, du, out);
+  return 0;
+}
+extern "C" int oracle64_implicit_force(oracle_scene64* c, const double* du, double* out) {
+  if (check(c) || !du || !out) return -1;
+  auto s = wrap<double>(c);
+  oracle::implicit_force(s, du, out);
+  return 0;
+}
+extern "C" int oracle64_implicit_apply(oracle_scene64* c, double dt, double beta, const double* v, double* out) {
+  if (check(c) || !v || !out) return -1;
+  auto s = wrap<double>(c);
+  oracle::implicit_apply(s, dt, beta, v, out);
+  return 0;
+}
+extern "C" int oracle64_implicit_solve(oracle_scene64* c, double dt, double beta, double tol, int32_t max_iter,
+                                       int32_t* iters, double* relres) {
+  if (check(c)) return -1;
+  auto s = wrap<double>(c);
+  int it = 0;
+  oracle::implicit_solve(s, dt, beta, tol, int(max_iter), &it, relres);
+  if (iters) *iters = it;
+  return 0;
+}
+extern "C" int oracle64_step_semi_implicit(oracle_scene64* c, double dt, double beta, double tol, int32_t max_iter,
+                                           int32_t nsteps, int32_t* iters, double* relres) {
+  if (check(c)) return -1;
+  auto s = wrap<double>(c);
+  for (int32_t i = 0; i < nsteps; ++i) {
+    int it = 0;
+    oracle::step_semi_implicit(s, dt, beta, tol, int(max_iter), &it, relres);
+    if (iters) *iters = it;
+  }
+  return 0;
+}

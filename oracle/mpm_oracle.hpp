@@ -20,6 +20,8 @@
 //   particle advance ................... shader/compute/mpm/particleAdvance.glsl:56-217
 //   rigid advance ...................... shader/compute/mpm/rigidAdvance.glsl:14-18
 //   stage order ........................ src/snow/simulation/timeUpdate/explicittimeupdate.cpp:4-63
+//   semi-implicit step ................. semiimplicittimeupdate.cpp:3-5 is empty; restated from the
+//                                        published method (Stomakhin et al. 2013, eq. 9), see below
 //
 // Deliberate differences from the reference, all recorded as quirk decisions:
 //  * accumulation is in Real (not int32 fixed point x1e8 / x1e6, p2gTransfer.glsl:98-146);
@@ -622,6 +624,252 @@ template <class R> void step(Scene<R>& s, R dt) {
   rigid_advance(s, dt);
   p2g(s, /*mass_only=*/false);  // gReset + p2gTransfer
   grid_update(s, dt);
+  g2p_advance(s, dt);
+}
+
+// ---- Semi-implicit velocity update ----------------------------------------------------------
+// The reference declares SemiImplicitTimeUpdate (src/snow/simulation/timeUpdate/
+// semiimplicittimeupdate.hpp:6-13) with empty init/update bodies (semiimplicittimeupdate.cpp:3-5).
+// This restates the published method the reference's explicit step follows: Stomakhin et al.
+// 2013, "A material point method for snow simulation" (SIGGRAPH), eq. (9) and the Hessian action
+// of its implicit-update section, for the energy whose gradient is the explicit force above
+// (fixed corotated with hardening, the same grad w with its grad_scale):
+//   K v = M v - beta dt df(dt v) = M v*,     df_i(du) = -sum_p V_p0 A_p F_Ep^T grad w_ip,
+//   A_p = d2Psi/dF2 (F_Ep) : dF_p,           dF_p = sum_j du_j grad w_jp^T F_Ep,
+//   d2Psi/dF2 : dF = 2 mu (dF - dR) + lambda (cof F : dF) cof F + lambda (J - 1) d(cof F),
+// with mu, lambda hardened by J_P as in the explicit force and V_p0 = m_p / rho_p0 (0 where the
+// explicit force skips the particle: rho0 <= 0 or |m| > 100, p2gTransfer.glsl:64-66/126).
+// K is symmetric, so it is solved by conjugate residuals (the paper's solver) from v*.
+// This oracle takes the exact polar factor (scaled Newton iteration) and dR = R [omega]x with
+// (tr S I - S) omega = axial(R^T dF - dF^T R), S = R^T F; the device takes the factors of the
+// SVD the explicit force uses.
+
+// cof F = J F^-T: column k is the cross product of the other two columns (cyclic order).
+template <class R> inline Mat3<R> cofactor(const Mat3<R>& F) {
+  auto cr = [](const R* a, const R* b, R* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  Mat3<R> C;
+  cr(F.m[1], F.m[2], C.m[0]);
+  cr(F.m[2], F.m[0], C.m[1]);
+  cr(F.m[0], F.m[1], C.m[2]);
+  return C;
+}
+// d(cof F)[dF]: each cross product differentiated by the product rule.
+template <class R> inline Mat3<R> cofactor_diff(const Mat3<R>& F, const Mat3<R>& dF) {
+  auto cr = [](const R* a, const R* b, R* o) {
+    o[0] += a[1] * b[2] - a[2] * b[1]; o[1] += a[2] * b[0] - a[0] * b[2]; o[2] += a[0] * b[1] - a[1] * b[0];
+  };
+  Mat3<R> C = Mat3<R>::zero();
+  const int nx[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+  for (int k = 0; k < 3; ++k) {
+    cr(dF.m[nx[k][0]], F.m[nx[k][1]], C.m[k]);
+    cr(F.m[nx[k][0]], dF.m[nx[k][1]], C.m[k]);
+  }
+  return C;
+}
+template <class R> inline R ddot(const Mat3<R>& A, const Mat3<R>& B) {
+  R s = 0;
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) s += A.m[c][r] * B.m[c][r];
+  return s;
+}
+template <class R> inline Mat3<R> inverse(const Mat3<R>& A) { return transpose(cofactor(A)) * (R(1) / determinant(A)); }
+
+// Rotation factor of F = R S (det F > 0) by the scaled Newton iteration R <- (g R + R^-T / g) / 2.
+template <class R> inline Mat3<R> polar_newton(const Mat3<R>& F) {
+  Mat3<R> X = F;
+  for (int it = 0; it < 100; ++it) {
+    const Mat3<R> Xit = transpose(inverse(X));
+    const R g = std::sqrt(std::sqrt(ddot(Xit, Xit) / ddot(X, X)));
+    const Mat3<R> Y = (X * g + Xit * (R(1) / g)) * R(0.5);
+    const Mat3<R> D = Y - X;
+    X = Y;
+    if (ddot(D, D) < R(1e-30) * ddot(X, X)) break;
+  }
+  return X;
+}
+
+// First Piola-Kirchhoff stress of the energy: P = 2 mu (F - R) + lambda (J - 1) cof F. The explicit
+// force matrix is V0 P F^T (p2g_force_matrix, with the reference's rounded R).
+template <class R> inline Mat3<R> fc_stress(const Mat3<R>& F, R mu_, R la) {
+  const Mat3<R> Rot = polar_newton(F);
+  return R(2) * mu_ * (F - Rot) + la * (determinant(F) - R(1)) * cofactor(F);
+}
+
+// d2Psi/dF2 (F) : dF.
+template <class R> inline Mat3<R> fc_hessian_action(const Mat3<R>& F, R mu_, R la, const Mat3<R>& dF) {
+  const Mat3<R> Rot = polar_newton(F);
+  const Mat3<R> S = transpose(Rot) * F;
+  const Mat3<R> Ss = (S + transpose(S)) * R(0.5);
+  const R tr = Ss.m[0][0] + Ss.m[1][1] + Ss.m[2][2];
+  const Mat3<R> T = Mat3<R>::ident(tr) - Ss;
+  const Mat3<R> D = transpose(Rot) * dF;
+  // axial vector of the skew D - D^T (m[col][row]): a0 = K(2,1), a1 = K(0,2), a2 = K(1,0)
+  const Vec3<R> a = v3(D.m[1][2] - D.m[2][1], D.m[2][0] - D.m[0][2], D.m[0][1] - D.m[1][0]);
+  const Vec3<R> w = inverse(T) * a;
+  Mat3<R> Om = Mat3<R>::zero();  // [w]x: (1,0) = w.z, (0,2) = w.y, (2,1) = w.x, skew
+  Om.m[0][1] = w.z; Om.m[1][0] = -w.z;
+  Om.m[2][0] = w.y; Om.m[0][2] = -w.y;
+  Om.m[1][2] = w.x; Om.m[2][1] = -w.x;
+  const Mat3<R> dR = Rot * Om;
+  const Mat3<R> C = cofactor(F);
+  const R J = determinant(F);
+  return R(2) * mu_ * (dF - dR) + la * ddot(C, dF) * C + la * (J - R(1)) * cofactor_diff(F, dF);
+}
+
+// mu, lambda (hardened) and V0 of particle p as the explicit force takes them.
+template <class R> inline void implicit_coefficients(const Scene<R>& s, int64_t p, R& mu_, R& la, R& V0) {
+  const R JP = determinant(s.FP(p));
+  mu_ = mu(s.young, s.poisson, s.hardening, JP);
+  la = lambda(s.young, s.poisson, s.hardening, JP);
+  const R mp = s.m[p], r0 = s.rho0[p];
+  V0 = (r0 > R(0) && std::fabs(mp) <= R(100)) ? mp / r0 : R(0);
+}
+
+// sum_j du_j grad w_jp^T over the in-grid stencil nodes (grad w with grad_scale).
+template <class R> inline Mat3<R> implicit_gather(const Scene<R>& s, int64_t p, const R* du) {
+  const int Dx = s.D[0], Dy = s.D[1], Dz = s.D[2];
+  Vec3<R> u; int base[3];
+  particle_in_grid(s, s.xp(p), u, base);
+  const StencilWeights<R> sw(u, base);
+  Mat3<R> L = Mat3<R>::zero();
+  for (int sidx = 0; sidx < 64; ++sidx) {
+    int o[3]; getIJK(sidx, o);
+    int gi[3] = {base[0] + o[0], base[1] + o[1], base[2] + o[2]};
+    if (!(gi[0] >= 0 && gi[1] >= 0 && gi[2] >= 0 && gi[0] < Dx && gi[1] < Dy && gi[2] < Dz)) continue;
+    const Vec3<R> g = sw.gradient(o) * s.gscale;
+    const int64_t gI = gi[0] + int64_t(gi[1]) * Dx + int64_t(gi[2]) * Dx * Dy;
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r) L.m[c][r] += du[3 * gI + r] * g[c];
+  }
+  return L;
+}
+
+// out_i -= sum_p G_p grad w_ip over the nodes with z in [k0, k1), in the bucket order of p2g.
+template <class R> void implicit_scatter(const Scene<R>& s, const std::vector<Mat3<R>>& G,
+                                         const std::vector<std::vector<int64_t>>& zb, int k0, int k1, R* out) {
+  const int Dx = s.D[0], Dy = s.D[1], Dz = s.D[2];
+  for (int bz = k0 - 2; bz <= k1 + 1; ++bz) {
+    if (bz < -2 || bz > Dz) continue;
+    for (int64_t p : zb[bz + 2]) {
+      Vec3<R> u; int base[3];
+      particle_in_grid(s, s.xp(p), u, base);
+      const StencilWeights<R> sw(u, base);
+      for (int sidx = 0; sidx < 64; ++sidx) {
+        int o[3]; getIJK(sidx, o);
+        int gi[3] = {base[0] + o[0], base[1] + o[1], base[2] + o[2]};
+        if (!(gi[0] >= 0 && gi[1] >= 0 && gi[2] >= 0 && gi[0] < Dx && gi[1] < Dy && gi[2] < Dz)) continue;
+        if (gi[2] < k0 || gi[2] >= k1) continue;
+        const Vec3<R> f = G[p] * (sw.gradient(o) * s.gscale);
+        const int64_t gI = gi[0] + int64_t(gi[1]) * Dx + int64_t(gi[2]) * Dx * Dy;
+        out[3 * gI] -= f.x; out[3 * gI + 1] -= f.y; out[3 * gI + 2] -= f.z;
+      }
+    }
+  }
+}
+
+template <class R> void implicit_scatter_all(const Scene<R>& s, const std::vector<Mat3<R>>& G, R* out) {
+  std::fill(out, out + 3 * s.nodes(), R(0));
+  auto zb = z_buckets(s);
+  const int T = std::max(1, std::min(s.nthreads, s.D[2]));
+  parallel_for(T, s.D[2], [&](int, int64_t kb, int64_t ke) { implicit_scatter(s, G, zb, int(kb), int(ke), out); });
+}
+
+// df(du) at every node (the Hessian action with the sign of a force, no dt).
+template <class R> void implicit_df(const Scene<R>& s, const R* du, R* out) {
+  std::vector<Mat3<R>> G(s.n);
+  parallel_for(s.nthreads, s.n, [&](int, int64_t b, int64_t e) {
+    for (int64_t p = b; p < e; ++p) {
+      R mu_, la, V0;
+      implicit_coefficients(s, p, mu_, la, V0);
+      if (V0 == R(0)) { G[p] = Mat3<R>::zero(); continue; }
+      const Mat3<R> F = s.FE(p);
+      const Mat3<R> dF = implicit_gather(s, p, du) * F;
+      G[p] = V0 * (fc_hessian_action(F, mu_, la, dF) * transpose(F));
+    }
+  });
+  implicit_scatter_all(s, G, out);
+}
+
+// The node force of the energy at displaced nodes x_j + du_j (F_p -> (I + sum_j du_j grad w^T) F_p),
+// f_i = -sum_p V0 P(F_p') F_p^T grad w_ip; implicit_df is its derivative at du = 0.
+template <class R> void implicit_force(const Scene<R>& s, const R* du, R* out) {
+  std::vector<Mat3<R>> G(s.n);
+  parallel_for(s.nthreads, s.n, [&](int, int64_t b, int64_t e) {
+    for (int64_t p = b; p < e; ++p) {
+      R mu_, la, V0;
+      implicit_coefficients(s, p, mu_, la, V0);
+      if (V0 == R(0)) { G[p] = Mat3<R>::zero(); continue; }
+      const Mat3<R> F = s.FE(p);
+      const Mat3<R> Fd = (Mat3<R>::ident() + implicit_gather(s, p, du)) * F;
+      G[p] = V0 * (fc_stress(Fd, mu_, la) * transpose(F));
+    }
+  });
+  implicit_scatter_all(s, G, out);
+}
+
+// Kv = M v - beta dt df(dt v) on the nodes with m > 0 (0 elsewhere); v is read only there.
+template <class R> void implicit_apply(const Scene<R>& s, R dt, R beta, const R* v, R* out) {
+  const int64_t nn = s.nodes();
+  std::vector<R> du(3 * nn, R(0)), df(3 * nn);
+  for (int64_t i = 0; i < nn; ++i)
+    if (s.gm[i] > R(0))
+      for (int k = 0; k < 3; ++k) du[3 * i + k] = dt * v[3 * i + k];
+  implicit_df(s, du.data(), df.data());
+  for (int64_t i = 0; i < nn; ++i)
+    for (int k = 0; k < 3; ++k)
+      out[3 * i + k] = s.gm[i] > R(0) ? s.gm[i] * v[3 * i + k] - beta * dt * df[3 * i + k] : R(0);
+}
+
+// Conjugate residuals for K v = M v*, from v = v*, on the nodes with m > 0; v* (s.gvstar) is
+// replaced by the solution. Stops when |r| <= tol |M v*| or after max_iter updates.
+template <class R> void implicit_solve(Scene<R>& s, R dt, R beta, R tol, int max_iter, int* iters, R* relres) {
+  const int64_t n3 = 3 * s.nodes();
+  std::vector<R> x(s.gvstar, s.gvstar + n3), b(n3), r(n3), p(n3), Kp(n3), Kr(n3);
+  for (int64_t i = 0; i < s.nodes(); ++i)
+    for (int k = 0; k < 3; ++k) {
+      x[3 * i + k] = s.gm[i] > R(0) ? x[3 * i + k] : R(0);
+      b[3 * i + k] = s.gm[i] > R(0) ? s.gm[i] * x[3 * i + k] : R(0);
+    }
+  auto dotv = [&](const std::vector<R>& a, const std::vector<R>& c) {
+    R acc = 0;
+    for (int64_t i = 0; i < n3; ++i) acc += a[i] * c[i];
+    return acc;
+  };
+  implicit_apply(s, dt, beta, x.data(), Kr.data());
+  for (int64_t i = 0; i < n3; ++i) r[i] = b[i] - Kr[i];
+  const R bb = dotv(b, b);
+  R rr = dotv(r, r);
+  int it = 0;
+  R rho = 0;
+  std::fill(p.begin(), p.end(), R(0));
+  std::fill(Kp.begin(), Kp.end(), R(0));
+  while (rr > tol * tol * bb && it < max_iter) {
+    implicit_apply(s, dt, beta, r.data(), Kr.data());
+    const R rho_new = dotv(r, Kr);
+    const R bc = it == 0 ? R(0) : rho_new / rho;
+    rho = rho_new;
+    for (int64_t i = 0; i < n3; ++i) { p[i] = r[i] + bc * p[i]; Kp[i] = Kr[i] + bc * Kp[i]; }
+    const R pp = dotv(Kp, Kp);
+    if (!(pp > R(0))) break;
+    const R alpha = rho / pp;
+    for (int64_t i = 0; i < n3; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * Kp[i]; }
+    rr = dotv(r, r);
+    ++it;
+  }
+  std::copy(x.begin(), x.end(), s.gvstar);
+  if (iters) *iters = it;
+  if (relres) *relres = bb > R(0) ? std::sqrt(rr / bb) : R(0);
+}
+
+// One semi-implicit step: the explicit step with the grid velocities v* replaced by the solution
+// of K v = M v* before the transfer back to the particles (the paper's steps 1-6, 7, 8-10).
+template <class R> void step_semi_implicit(Scene<R>& s, R dt, R beta, R tol, int max_iter, int* iters, R* relres) {
+  rigid_advance(s, dt);
+  p2g(s, /*mass_only=*/false);
+  grid_update(s, dt);
+  implicit_solve(s, dt, beta, tol, max_iter, iters, relres);
   g2p_advance(s, dt);
 }
 

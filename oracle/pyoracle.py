@@ -81,6 +81,24 @@ def load() -> C.CDLL:
         getattr(lib, f"{p}_getijk").restype = None
         getattr(lib, f"{p}_collide").argtypes = [TP, TP, TP, C.c_int32, TP]
         getattr(lib, f"{p}_collide").restype = None
+    DP = C.POINTER(C.c_double)
+    SP = C.POINTER(Scene64)
+    lib.oracle64_fc_stress.argtypes = [DP, C.c_double, C.c_double, DP]
+    lib.oracle64_fc_hessian.argtypes = [DP, C.c_double, C.c_double, DP, DP]
+    lib.oracle64_polar_exact.argtypes = [DP, DP]
+    for fn in ("fc_stress", "fc_hessian", "polar_exact"):
+        getattr(lib, f"oracle64_{fn}").restype = None
+    for fn in ("implicit_df", "implicit_force"):
+        getattr(lib, f"oracle64_{fn}").argtypes = [SP, DP, DP]
+        getattr(lib, f"oracle64_{fn}").restype = C.c_int
+    lib.oracle64_implicit_apply.argtypes = [SP, C.c_double, C.c_double, DP, DP]
+    lib.oracle64_implicit_apply.restype = C.c_int
+    lib.oracle64_implicit_solve.argtypes = [SP, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                            C.POINTER(C.c_int32), DP]
+    lib.oracle64_implicit_solve.restype = C.c_int
+    lib.oracle64_step_semi_implicit.argtypes = [SP, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                                C.POINTER(C.c_int32), DP]
+    lib.oracle64_step_semi_implicit.restype = C.c_int
     _lib = lib
     return lib
 
@@ -163,6 +181,50 @@ class Oracle:
     def step(self, dt, nsteps: int = 1):
         self._call("step", float(np.float32(dt)), int(nsteps))
 
+    # ---- semi-implicit update (FP64 oracle only) -------------------------------------------
+    def _need64(self):
+        if self.pre != "oracle64":
+            raise RuntimeError("the semi-implicit update is in the FP64 oracle only")
+
+    def implicit_df(self, du):
+        """df(du) per node (the Hessian action of the node force, no dt), from the current particles."""
+        self._need64()
+        du = np.ascontiguousarray(du, np.float64).reshape(-1, 3)
+        out = np.empty_like(du)
+        self._call("implicit_df", _p(du, C.c_double), _p(out, C.c_double))
+        return out
+
+    def implicit_force(self, du):
+        """The node force of the energy with the nodes displaced by du (exact polar factor)."""
+        self._need64()
+        du = np.ascontiguousarray(du, np.float64).reshape(-1, 3)
+        out = np.empty_like(du)
+        self._call("implicit_force", _p(du, C.c_double), _p(out, C.c_double))
+        return out
+
+    def implicit_apply(self, dt, beta, v):
+        """K v = M v - beta dt df(dt v) on the nodes with m > 0 (uses the current grid mass)."""
+        self._need64()
+        v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        out = np.empty_like(v)
+        self._call("implicit_apply", float(dt), float(beta), _p(v, C.c_double), _p(out, C.c_double))
+        return out
+
+    def implicit_solve(self, dt, beta, tol=1e-6, max_iter=50):
+        """Replaces gvstar by the solution of K v = M v*; returns (iterations, relative residual)."""
+        self._need64()
+        it, rr = C.c_int32(0), C.c_double(0.0)
+        self._call("implicit_solve", float(np.float32(dt)), float(beta), float(tol), int(max_iter), C.byref(it),
+                   C.byref(rr))
+        return it.value, rr.value
+
+    def step_semi_implicit(self, dt, beta, tol=1e-6, max_iter=50, nsteps: int = 1):
+        self._need64()
+        it, rr = C.c_int32(0), C.c_double(0.0)
+        self._call("step_semi_implicit", float(np.float32(dt)), float(beta), float(tol), int(max_iter),
+                   int(nsteps), C.byref(it), C.byref(rr))
+        return it.value, rr.value
+
 
 def svd(A, precision=64):
     lib = load()
@@ -205,3 +267,28 @@ def collide(x, v, colliders):
     out = np.empty(3, np.float64)
     lib.oracle64_collide(_p(x, C.c_double), _p(v, C.c_double), _p(cs, C.c_double), len(cs), _p(out, C.c_double))
     return out
+
+
+def _cm(A):
+    return np.ascontiguousarray(np.asarray(A, np.float64).T).reshape(9)
+
+
+def fc_stress(F, mu, la):
+    """First Piola-Kirchhoff stress 2 mu (F - R) + lambda (J - 1) cof F, exact polar factor."""
+    out = np.empty(9)
+    load().oracle64_fc_stress(_p(_cm(F), C.c_double), float(mu), float(la), _p(out, C.c_double))
+    return out.reshape(3, 3).T
+
+
+def fc_hessian(F, mu, la, dF):
+    """d2Psi/dF2 (F) : dF of the same energy."""
+    out = np.empty(9)
+    load().oracle64_fc_hessian(_p(_cm(F), C.c_double), float(mu), float(la), _p(_cm(dF), C.c_double),
+                               _p(out, C.c_double))
+    return out.reshape(3, 3).T
+
+
+def polar_exact(F):
+    out = np.empty(9)
+    load().oracle64_polar_exact(_p(_cm(F), C.c_double), _p(out, C.c_double))
+    return out.reshape(3, 3).T

@@ -332,6 +332,17 @@ class SnowContext:
     def run_graph(self, steps: int):
         self._chk(self.lib.snow_run_graph(self.h, steps))
 
+    def step_semi_implicit(self, dt: float | None = None, beta: float = 1.0, tol: float = 1e-5,
+                           max_iter: int = 100):
+        """One semi-implicit step (snow_step_semi_implicit): dt None takes each scene's own dt."""
+        self._chk(self.lib.snow_step_semi_implicit(self.h, 0.0 if dt is None else dt, beta, tol, max_iter))
+
+    def implicit_stats(self, scene: int = 0) -> tuple:
+        """(iterations, relative residual) of the scene's last semi-implicit solve."""
+        it, rr = C.c_int32(), C.c_double()
+        self._chk(self.lib.snow_implicit_stats(self.h, scene, C.byref(it), C.byref(rr)))
+        return int(it.value), float(rr.value)
+
     def autotune(self, steps: int = 5) -> dict:
         g, p, q, b = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         self._chk(self.lib.snow_autotune(self.h, steps, C.byref(g), C.byref(p), C.byref(q), C.byref(b)))
@@ -460,6 +471,23 @@ class ExplicitTimeUpdate:
 
     def update(self, dt: float):
         self.ctx.step(dt)
+
+
+class SemiImplicitTimeUpdate:
+    """SemiImplicitTimeUpdate (semiimplicittimeupdate.hpp:6-13, empty in the reference): ``init()`` and
+    ``update(dt)`` with the semi-implicit grid velocity solve of snow_step_semi_implicit."""
+
+    def __init__(self, scene: PhysicalScene, flags: int = 0, beta: float = 1.0, tol: float = 1e-5,
+                 max_iter: int = 100):
+        self.scene = scene
+        self.ctx = SnowContext([scene], flags)
+        self.beta, self.tol, self.max_iter = beta, tol, max_iter
+
+    def init(self):
+        self.ctx.init()
+
+    def update(self, dt: float):
+        self.ctx.step_semi_implicit(dt, self.beta, self.tol, self.max_iter)
 
 
 class MPMPhysicEngine:

@@ -72,6 +72,8 @@ SIGNATURES = [
     ("snow_step", C.c_int, [C.c_void_p, C.c_double]),
     ("snow_step_batch", C.c_int, [C.c_void_p]),
     ("snow_run_graph", C.c_int, [C.c_void_p, C.c_int32]),
+    ("snow_step_semi_implicit", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    ("snow_implicit_stats", C.c_int, [C.c_void_p, C.c_int32, i32p, f64p]),
     ("snow_autotune", C.c_int, [C.c_void_p, C.c_int32, i32p, i32p, i32p, i32p]),
     ("snow_stage_grid_mass", C.c_int, [C.c_void_p]),
     ("snow_stage_particle_volume", C.c_int, [C.c_void_p]),

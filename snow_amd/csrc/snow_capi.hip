@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/snow.h"
+#include "snow_implicit.hpp"
 #include "snow_kernels.hpp"
 #include "snow_sort.hpp"
 
@@ -142,6 +143,19 @@ struct snow_ctx {
   // per wave; only for batches of several scenes (one scene: wave w starts at 64 w)
   uint2* d_waves = nullptr;
   uint32_t nwaves = 0;
+
+  // semi-implicit solve (allocated on the first snow_step_semi_implicit)
+  float4* d_imp_vec = nullptr;           // IMP_VECS dense node vectors
+  float* d_imp_mass = nullptr;           // node masses of the step (0 where unused)
+  long long* d_imp_acc = nullptr;        // fixed-point df, 3 per node
+  float* d_imp_pd = nullptr;             // IMP_PLANES planes of prepared particle data
+  float* d_imp_G = nullptr;              // 9 planes of G_p
+  double* d_imp_part = nullptr;          // 3 partial sums per node block
+  ImpBlock* d_imp_blocks = nullptr;      // node blocks (scene, first node)
+  uint32_t* d_imp_scene_blocks = nullptr;  // first node block of every scene, then the total
+  ImpScal* d_imp_scal = nullptr;         // per-scene solver state
+  ImpScal* h_imp_scal = nullptr;         // pinned host copy for the convergence polls
+  uint32_t imp_nblocks = 0;
 };
 
 namespace {
@@ -438,6 +452,136 @@ int run_g2p_advance(snow_ctx* c, float dt) {
 }
 
 // One fused step of every scene. dt > 0 overrides the per-scene dt.
+// ---- semi-implicit solve ------------------------------------------------------------------
+int ensure_implicit(snow_ctx* c) {
+  if (c->d_imp_scal) return 0;
+  const size_t nn = size_t(std::max<int64_t>(1, c->total_nodes)), np = std::max<size_t>(1, c->N);
+  std::vector<ImpBlock> blocks;
+  std::vector<uint32_t> first(size_t(c->nscenes) + 1, 0u);
+  for (int s = 0; s < c->nscenes; ++s) {
+    const SceneDev& S = c->scenes[s];
+    first[s] = uint32_t(blocks.size());
+    const int64_t nodes = int64_t(S.D[0]) * S.D[1] * S.D[2];
+    for (int64_t q = 0; q < nodes; q += IMP_NODES_PER_BLOCK) blocks.push_back(ImpBlock{uint32_t(s), 0u, S.node_off + q});
+  }
+  first[c->nscenes] = uint32_t(blocks.size());
+  c->imp_nblocks = uint32_t(blocks.size());
+  auto bad = [&](hipError_t e) {
+    if (e == hipSuccess) return false;
+    void* ptrs[] = {c->d_imp_vec, c->d_imp_mass, c->d_imp_acc, c->d_imp_pd, c->d_imp_G, c->d_imp_part,
+                    c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    if (c->h_imp_scal) (void)hipHostFree(c->h_imp_scal);
+    c->d_imp_vec = nullptr; c->d_imp_mass = nullptr; c->d_imp_acc = nullptr; c->d_imp_pd = nullptr;
+    c->d_imp_G = nullptr; c->d_imp_part = nullptr; c->d_imp_blocks = nullptr; c->d_imp_scene_blocks = nullptr;
+    c->d_imp_scal = nullptr; c->h_imp_scal = nullptr;
+    return true;
+  };
+  if (bad(dalloc(&c->d_imp_vec, nn * IMP_VECS)) || bad(dalloc(&c->d_imp_mass, nn)) ||
+      bad(dalloc(&c->d_imp_acc, 3 * nn)) || bad(dalloc(&c->d_imp_pd, np * IMP_PLANES)) ||
+      bad(dalloc(&c->d_imp_G, np * 9)) || bad(dalloc(&c->d_imp_part, 3 * std::max<size_t>(1, blocks.size()))) ||
+      bad(dalloc(&c->d_imp_blocks, std::max<size_t>(1, blocks.size()))) ||
+      bad(dalloc(&c->d_imp_scene_blocks, first.size())) || bad(dalloc(&c->d_imp_scal, size_t(c->nscenes))) ||
+      bad(hipHostMalloc(reinterpret_cast<void**>(&c->h_imp_scal), sizeof(ImpScal) * c->nscenes)))
+    return fail(c, SNOW_E_CAPACITY, "semi-implicit solve: no device memory for its node vectors");
+  HIPCHK(c, hipMemsetAsync(c->d_imp_acc, 0, sizeof(long long) * 3 * nn, c->stream));
+  if (!blocks.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_imp_blocks, blocks.data(), sizeof(ImpBlock) * blocks.size(),
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_imp_scene_blocks, first.data(), sizeof(uint32_t) * first.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Conjugate residuals for K v = M v* on the grid the grid update just wrote; the node records then
+// hold v instead of v*. Convergence is polled every IMP_POLL iterations (one small copy to the host).
+int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
+  int rc;
+  if ((rc = ensure_implicit(c))) return rc;
+  constexpr int IMP_POLL = 4;
+  const uint32_t nb = c->imp_nblocks;
+  const int64_t nn = c->total_nodes;
+  const uint2* waves = c->nscenes > 1 ? c->d_waves : nullptr;
+  const unsigned pg = grid_for(c->nwaves, 4u);
+  auto scalars = [&](int phase) -> int {
+    k_imp_scalars<<<c->nscenes, IMP_THREADS, 0, c->stream>>>(phase, c->d_imp_scene_blocks, nb, c->d_imp_part,
+                                                             c->d_imp_scal, c->d_scenes, dt, beta, tol, max_iter);
+    LAUNCHCHK(c);
+    return 0;
+  };
+  // K applied to vector `v` (0 = x, 1 = r) and the node phase that consumes it
+  auto apply = [&](int v) -> int {
+    if (c->N) {
+      k_imp_apply<<<pg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_imp_pd, c->N,
+                                             c->d_imp_vec + size_t(v) * nn, c->d_imp_G, c->d_imp_scal);
+      LAUNCHCHK(c);
+      k_imp_scatter<<<pg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_imp_pd, c->d_imp_G, c->N,
+                                               reinterpret_cast<unsigned long long*>(c->d_imp_acc), c->d_imp_scal);
+      LAUNCHCHK(c);
+    }
+    return 0;
+  };
+#define NODE_ARGS c->d_scenes, c->d_imp_blocks, nb, c->d_imp_mass, c->d_imp_vec, c->d_imp_acc, c->d_imp_scal, nn, c->d_imp_part
+  if ((rc = scalars(-1))) return rc;
+  if (c->N) {
+    k_imp_prep<<<pg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_slot, c->d_rec[c->cur],
+                                          c->d_fe[c->cur], c->d_fp[c->cur], c->N, c->d_imp_pd, 1);
+    LAUNCHCHK(c);
+  }
+  if (nb == 0) return 0;
+  k_imp_setup<<<nb, IMP_THREADS, 0, c->stream>>>(c->d_scenes, c->d_imp_blocks, c->d_ctl, c->d_tile_stamp,
+                                                 c->d_g_node, c->d_g_mass, c->d_imp_mass, c->d_imp_vec,
+                                                 c->d_imp_acc, nn);
+  LAUNCHCHK(c);
+  if ((rc = apply(0))) return rc;
+  k_imp_node<IMP_INIT><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
+  LAUNCHCHK(c);
+  if ((rc = scalars(IMP_INIT))) return rc;
+  for (int it = 0; it < max_iter; ++it) {
+    if (it % IMP_POLL == 0) {
+      HIPCHK(c, hipMemcpyAsync(c->h_imp_scal, c->d_imp_scal, sizeof(ImpScal) * c->nscenes, hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      bool all = true;
+      for (int s = 0; s < c->nscenes; ++s) all = all && c->h_imp_scal[s].done;
+      if (all) break;
+    }
+    if ((rc = apply(1))) return rc;
+    k_imp_node<IMP_KR><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
+    LAUNCHCHK(c);
+    if ((rc = scalars(IMP_KR))) return rc;
+    k_imp_node<IMP_P><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
+    LAUNCHCHK(c);
+    if ((rc = scalars(IMP_P))) return rc;
+    k_imp_node<IMP_X><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
+    LAUNCHCHK(c);
+    if ((rc = scalars(IMP_X))) return rc;
+  }
+#undef NODE_ARGS
+  k_imp_finish<<<nb, IMP_THREADS, 0, c->stream>>>(c->d_scenes, c->d_imp_blocks, c->d_imp_mass, c->d_imp_vec,
+                                                  c->d_g_node, c->d_g_mass);
+  LAUNCHCHK(c);
+  return 0;
+}
+
+// The explicit step with the semi-implicit grid velocity solve between the grid update and G2P.
+int step_semi_implicit_impl(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
+  if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before stepping");
+  int rc;
+  if ((rc = run_force_prepass(c))) return rc;
+  if ((rc = run_sort(c, dt))) return rc;
+  if ((rc = run_p2g(c, 0))) return rc;
+  const bool keep = (c->flags & SNOW_FLAG_KEEP_GRID) != 0;
+  if ((rc = run_grid(c, keep, true, dt, true))) return rc;
+  if ((rc = run_implicit(c, dt, beta, tol, max_iter))) return rc;
+  if ((rc = run_g2p_advance(c, dt))) return rc;
+  c->grid_state = 1;
+  c->keep_valid = keep;
+  return 0;
+}
+
 int step_impl(snow_ctx* c, float dt) {
   if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before stepping");
   int rc;
@@ -462,9 +606,12 @@ void release(snow_ctx* c) {
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_fa, c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp,
                   c->d_tile_c0, c->d_tile_c1, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
-                  c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage};
+                  c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage,
+                  c->d_imp_vec, c->d_imp_mass, c->d_imp_acc, c->d_imp_pd, c->d_imp_G, c->d_imp_part,
+                  c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->h_imp_scal) (void)hipHostFree(c->h_imp_scal);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& p : c->pending) {
@@ -817,6 +964,33 @@ int snow_step(snow_ctx* c, double dt) {
   const float fdt = float(dt);  // technique.cpp:122-125 narrows the uniform to float
   if (!(fdt > 0.0f) || !std::isfinite(fdt)) return fail(c, SNOW_E_ARG, "dt must be > 0");
   return step_impl(c, fdt);
+}
+
+int snow_step_semi_implicit(snow_ctx* c, double dt, double beta, double tol, int32_t max_iter) {
+  if (!c) return SNOW_E_ARG;
+  const float fdt = float(dt);
+  if (!(fdt >= 0.0f) || !std::isfinite(fdt)) return fail(c, SNOW_E_ARG, "dt must be >= 0");
+  if (fdt == 0.0f)
+    for (const auto& S : c->scenes)
+      if (!(S.dt > 0.0f)) return fail(c, SNOW_E_ARG, "dt = 0 takes each scene's dt, which must be > 0");
+  if (!(beta >= 0.0) || !(beta <= 1.0)) return fail(c, SNOW_E_ARG, "beta must lie in [0, 1]");
+  if (!(tol > 0.0) || !std::isfinite(tol)) return fail(c, SNOW_E_ARG, "tol must be > 0");
+  if (max_iter < 0) return fail(c, SNOW_E_ARG, "max_iter must be >= 0");
+  for (const auto& S : c->scenes)
+    if (S.D[0] > IMP_MAX_DIM || S.D[1] > IMP_MAX_DIM || S.D[2] > IMP_MAX_DIM)
+      return fail(c, SNOW_E_ARG, "the semi-implicit step takes grids of at most 1021 nodes per axis");
+  return step_semi_implicit_impl(c, fdt, float(beta), float(tol), int(max_iter));
+}
+
+int snow_implicit_stats(snow_ctx* c, int32_t scene, int32_t* iterations, double* rel_residual) {
+  if (!c || scene < 0 || scene >= c->nscenes) return SNOW_E_ARG;
+  if (!c->d_imp_scal) return fail(c, SNOW_E_STATE, "no semi-implicit step has run on this context");
+  ImpScal z;
+  HIPCHK(c, hipMemcpyAsync(&z, c->d_imp_scal + scene, sizeof(ImpScal), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (iterations) *iterations = z.iters;
+  if (rel_residual) *rel_residual = z.bb > 0.0 ? std::sqrt(z.rr / z.bb) : 0.0;
+  return 0;
 }
 
 int snow_step_batch(snow_ctx* c) {

@@ -1,0 +1,536 @@
+// snow_implicit.hpp - semi-implicit grid velocity solve of the snow MPM step for gfx950 (CDNA4).
+//
+// The reference declares SemiImplicitTimeUpdate (src/snow/simulation/timeUpdate/
+// semiimplicittimeupdate.hpp:6-13) with empty init/update bodies (semiimplicittimeupdate.cpp:3-5).
+// This is the published method the reference's explicit step follows: Stomakhin et al. 2013,
+// "A material point method for snow simulation", eq. (9) with its Hessian action, for the energy
+// whose gradient is the explicit force of this code (fixed corotated with hardening, the same
+// grad w and grad_scale):
+//   K v = M v - beta dt df(dt v) = M v*,        df_i(du) = -sum_p V_p0 A_p F_Ep^T grad w_ip,
+//   A_p = d2Psi/dF2 (F_Ep) : dF_p,              dF_p = sum_j du_j grad w_jp^T F_Ep,
+//   d2Psi/dF2 : dF = 2 mu (dF - dR) + lambda (cof F : dF) cof F + lambda (J - 1) d(cof F).
+// R is the rotation the explicit force takes (W V^T of the reference SVD, rounded to 1e-9);
+// R^T dR = [w]x with (tr S I - S) w = axial(R^T dF - dF^T R), S the symmetric part of R^T F.
+// K is symmetric; conjugate residuals (the paper's solver) run on the grid from v = v*, then the
+// transfer back to the particles takes v instead of v*.
+//
+// Layout: dense node vectors over the context's nodes (float4 per node: x, r, p, Kp, Kr), the node
+// masses of this step (0 outside the node tiles the grid update wrote or where m <= 0), a dense
+// fixed-point accumulator (3 x int64 per node) for df, and per sorted particle 40 planes of data
+// prepared once per step plus 9 planes of G_p = gs V_p0 A_p F_Ep^T per application.
+// Sums of df are exact: each wave reduces its particles per cell in registers and adds the
+// rounded fixed-point values with integer atomics, so the result does not depend on their order.
+// Dot products are block partials summed in a fixed order, so a solve is bit-reproducible.
+#pragma once
+#include "snow_kernels.hpp"
+
+namespace snow {
+
+// Per-scene solver state in device memory.
+struct ImpScal {
+  double rho;    // r . K r of the current update
+  double alpha;  // step length of the current update
+  double beta;   // weight of the previous direction in the current update
+  double bb;     // |M v*|^2
+  double rr;     // |r|^2
+  double tol2;   // squared relative residual at which the solve stops
+  uint32_t gmax; // float bits of the largest row sum of |G_p| of the current application
+  int32_t iters; // direction updates done
+  int32_t done;  // converged, broke down or out of iterations
+  int32_t max_iter;
+  float beta_dt; // beta * dt
+  float dt;
+  uint32_t pad[2];
+};
+
+enum ImpPhase { IMP_INIT = 0, IMP_KR = 1, IMP_P = 2, IMP_X = 3 };
+
+constexpr int IMP_THREADS = 256;
+constexpr int IMP_NODES_PER_THREAD = 4;
+constexpr int IMP_NODES_PER_BLOCK = IMP_THREADS * IMP_NODES_PER_THREAD;
+constexpr int IMP_VECS = 5;       // x, r, p, Kp, Kr
+constexpr int IMP_PLANES = 40;    // prepared particle data, below
+constexpr int IMP_STAGE = 36;     // LDS floats per staged particle in the scatter
+constexpr int IMP_MAX_DIM = 1021; // largest grid dimension: base cells + 2 fit 10 bits per axis
+// plane offsets of the prepared data
+constexpr int IP_F = 0, IP_R = 9, IP_MI = 18, IP_COF = 24, IP_C = 33, IP_X = 37;
+
+// A node-vector block: scene and first node (the blocks of a scene are consecutive).
+struct ImpBlock {
+  uint32_t scene;
+  uint32_t pad;
+  int64_t first;
+};
+
+__device__ __forceinline__ void cross_add(const float* a, const float* b, float* o) {
+  o[0] += a[1] * b[2] - a[2] * b[1];
+  o[1] += a[2] * b[0] - a[0] * b[2];
+  o[2] += a[0] * b[1] - a[1] * b[0];
+}
+
+// Fixed-point exponent of a scene's df sums: every node sums at most N_s terms of magnitude at most
+// gmax * 8/27 (|N| <= 2/3, |N'| <= 2/3, so each weight-gradient component is at most 8/27), so with
+// 2^e * N_s * gmax * 8/27 < 2^62 no sum leaves the int64 range.
+__device__ __forceinline__ int imp_exponent(uint32_t gmax_bits, int64_t count) {
+  const double bound = double(__uint_as_float(gmax_bits)) * (8.0 / 27.0) * double(count > 0 ? count : 1);
+  if (!(bound > 0.0)) return 0;
+  int ex;
+  (void)frexp(bound, &ex);  // bound < 2^ex
+  return 62 - ex;
+}
+
+// ---------------------------------------------------------------------------------------
+// Once per step, per sorted position: F_E, R, (tr S I - S)^-1 (symmetric), cof F_E, 2 mu, lambda,
+// lambda (J - 1), gs V_p0 and x. Waves hold particles of one scene (the G2P wave table).
+__global__ void __launch_bounds__(256) k_imp_prep(const SceneDev* __restrict__ scenes, const uint2* __restrict__ waves,
+                                                  uint32_t nwaves, const uint32_t* __restrict__ vals,
+                                                  const float4* __restrict__ rec, const float* __restrict__ fe,
+                                                  const float* __restrict__ fp, uint32_t n,
+                                                  float* __restrict__ pd, int skip_heavy) {
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nwaves) return;
+  const uint2 ws = waves ? waves[w] : make_uint2(64u * w, 0u);
+  const SceneDev& S = scenes[ws.y];
+  const uint32_t i = ws.x + (threadIdx.x & 63);
+  if (int64_t(i) >= S.p_off + S.p_cnt) return;
+  const uint32_t p = vals[i];
+  const float4 xm = rec[2 * size_t(p)], vr = rec[2 * size_t(p) + 1];
+  float F[9], FP[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) { F[e] = ld_plane(fe + size_t(e) * n, p); FP[e] = ld_plane(fp + size_t(e) * n, p); }
+  const float hf = hardening_factor(S.hardening, det3(FP));
+  const float mu = S.mu0 * hf, la = S.lambda0 * hf;
+  float W[9], Sg[9], V[9], Rot[9], Sm[9];
+  svd3(F, W, Sg, V);
+  mul_bt(W, V, Rot);
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Rot[e] = round_to(Rot[e], 1e9f, 1e-9f);
+  mul_at(Rot, F, Sm);  // R^T F
+  const float s00 = Sm[0], s11 = Sm[4], s22 = Sm[8];
+  const float s01 = 0.5f * (Sm[1] + Sm[3]), s02 = 0.5f * (Sm[2] + Sm[6]), s12 = 0.5f * (Sm[5] + Sm[7]);
+  // T = tr(S) I - S and its inverse from the adjugate (T is symmetric, about 2 I for F near I)
+  const float t00 = s11 + s22, t11 = s00 + s22, t22 = s00 + s11, t01 = -s01, t02 = -s02, t12 = -s12;
+  const float a00 = t11 * t22 - t12 * t12, a11 = t00 * t22 - t02 * t02, a22 = t00 * t11 - t01 * t01;
+  const float a01 = t02 * t12 - t01 * t22, a02 = t01 * t12 - t02 * t11, a12 = t01 * t02 - t00 * t12;
+  const float det = t00 * a00 + t01 * a01 + t02 * a02;
+  const float id = fabsf(det) > 1e-20f ? 1.0f / det : 0.0f;
+  float cof[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  cross_add(F + 3, F + 6, cof);
+  cross_add(F + 6, F, cof + 3);
+  cross_add(F, F + 3, cof + 6);
+  const float J = det3(F);
+  const float m = xm.w, r0 = vr.w;
+  const bool use = r0 > 0.0f && !(skip_heavy && fabsf(m) > 100.0f);  // as the explicit force
+  const float v0 = use ? S.gscale * (m / r0) : 0.0f;
+  const float out[IMP_PLANES] = {
+      F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8],
+      Rot[0], Rot[1], Rot[2], Rot[3], Rot[4], Rot[5], Rot[6], Rot[7], Rot[8],
+      a00 * id, a11 * id, a22 * id, a01 * id, a02 * id, a12 * id,
+      cof[0], cof[1], cof[2], cof[3], cof[4], cof[5], cof[6], cof[7], cof[8],
+      2.0f * mu, la, la * (J - 1.0f), v0,
+      xm.x, xm.y, xm.z};
+#pragma unroll
+  for (int e = 0; e < IMP_PLANES; ++e) st_plane(pd + size_t(e) * n, i, out[e]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Node setup: the mass of every node of a node tile the grid update wrote this step (0 elsewhere),
+// x = v* where m > 0 (0 elsewhere), cleared accumulator.
+__global__ void __launch_bounds__(IMP_THREADS) k_imp_setup(const SceneDev* __restrict__ scenes,
+                                                           const ImpBlock* __restrict__ blocks,
+                                                           const StepCtl* __restrict__ ctl,
+                                                           const uint32_t* __restrict__ tile_stamp,
+                                                           const float2* __restrict__ g_node,
+                                                           const float* __restrict__ g_mass,
+                                                           float* __restrict__ mass, float4* __restrict__ vec,
+                                                           long long* __restrict__ acc, int64_t total_nodes) {
+  const ImpBlock B = blocks[blockIdx.x];
+  const SceneDev& S = scenes[B.scene];
+  const int64_t end = S.node_off + int64_t(S.D[0]) * S.D[1] * S.D[2];
+  const uint32_t step = ctl->step;
+#pragma unroll
+  for (int j = 0; j < IMP_NODES_PER_THREAD; ++j) {
+    const int64_t n = B.first + j * IMP_THREADS + threadIdx.x;
+    if (n >= end) break;
+    const int64_t l = n - S.node_off;
+    const int nx = int(l % S.D[0]), ny = int((l / S.D[0]) % S.D[1]), nz = int(l / (int64_t(S.D[0]) * S.D[1]));
+    const int Kx = nx / TILE, Ky = ny / TILE, Kz = nz / TILE;
+    bool live = false;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) live |= tile_live(S, tile_stamp, step, Kx + (b & 1), Ky + ((b >> 1) & 1), Kz + (b >> 2));
+    float m = live ? g_mass[n] : 0.0f;
+    m = m > 0.0f ? m : 0.0f;
+    float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (m > 0.0f) {
+      const float2 a = g_node[3 * n], b = g_node[3 * n + 1];
+      x = make_float4(a.x, a.y, b.x, 0.f);
+    }
+    mass[n] = m;
+    vec[n] = x;  // vector 0: x
+    acc[3 * n] = 0;
+    acc[3 * n + 1] = 0;
+    acc[3 * n + 2] = 0;
+  }
+  (void)total_nodes;
+}
+
+// ---------------------------------------------------------------------------------------
+// First half of K applied to the node vector vin: per particle dF = dt (sum_j vin_j grad w_jp^T) F,
+// A = d2Psi/dF2 : dF and G = gs V0 A F^T (9 planes, column-major), and the scene's largest row sum
+// of |G| for the fixed-point scale.
+__global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ scenes, const uint2* __restrict__ waves,
+                                                   uint32_t nwaves, const float* __restrict__ pd, uint32_t n,
+                                                   const float4* __restrict__ vin, float* __restrict__ G,
+                                                   ImpScal* __restrict__ scal) {
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nwaves) return;
+  const uint2 ws = waves ? waves[w] : make_uint2(64u * w, 0u);
+  const int sid = int(ws.y);
+  if (scal[sid].done) return;
+  const SceneDev& S = scenes[sid];
+  const uint32_t i = ws.x + (threadIdx.x & 63);
+  const bool valid = int64_t(i) < S.p_off + S.p_cnt;
+  float Gm[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (valid) {
+    const float v0 = ld_plane(pd + size_t(IP_C + 3) * n, i);
+    const float x = ld_plane(pd + size_t(IP_X) * n, i), y = ld_plane(pd + size_t(IP_X + 1) * n, i),
+                z = ld_plane(pd + size_t(IP_X + 2) * n, i);
+    const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
+    if (v0 != 0.0f && in_stencil_range(S, ux, uy, uz)) {
+      float wx[4], wy[4], wz[4], qx[4], qy[4], qz[4];
+      const int bx = axis_weights(ux, wx, qx), by = axis_weights(uy, wy, qy), bz = axis_weights(uz, wz, qz);
+      float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // column c = sum v dw/dx_c
+      for (int oz = 0; oz < 4; ++oz) {
+        const int iz = bz + oz - 1;
+        if (iz < 0 || iz >= S.D[2]) continue;
+        for (int oy = 0; oy < 4; ++oy) {
+          const int iy = by + oy - 1;
+          if (iy < 0 || iy >= S.D[1]) continue;
+          const float wyz = wy[oy] * wz[oz], qyz = qy[oy] * wz[oz], wqz = wy[oy] * qz[oz];
+          const int64_t row = S.node_off + int64_t(S.D[0]) * (iy + int64_t(S.D[1]) * iz);
+#pragma unroll
+          for (int ox = 0; ox < 4; ++ox) {
+            const int ix = bx + ox - 1;
+            if (ix < 0 || ix >= S.D[0]) continue;
+            const float4 v = vin[row + ix];
+            const float g0 = qx[ox] * wyz, g1 = wx[ox] * qyz, g2 = wx[ox] * wqz;
+            L[0] = fmaf(v.x, g0, L[0]); L[1] = fmaf(v.y, g0, L[1]); L[2] = fmaf(v.z, g0, L[2]);
+            L[3] = fmaf(v.x, g1, L[3]); L[4] = fmaf(v.y, g1, L[4]); L[5] = fmaf(v.z, g1, L[5]);
+            L[6] = fmaf(v.x, g2, L[6]); L[7] = fmaf(v.y, g2, L[7]); L[8] = fmaf(v.z, g2, L[8]);
+          }
+        }
+      }
+      const float k = scal[sid].dt * S.gscale;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) L[e] *= k;
+      float F[9], Rot[9], C[9], dF[9], D[9];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        F[e] = ld_plane(pd + size_t(IP_F + e) * n, i);
+        Rot[e] = ld_plane(pd + size_t(IP_R + e) * n, i);
+        C[e] = ld_plane(pd + size_t(IP_COF + e) * n, i);
+      }
+      mul(L, F, dF);  // dF = dt L F
+      mul_at(Rot, dF, D);  // R^T dF
+      // axial vector of D - D^T: (D(2,1) - D(1,2), D(0,2) - D(2,0), D(1,0) - D(0,1)), D(r,c) = D[c*3+r]
+      const float a0 = D[5] - D[7], a1 = D[6] - D[2], a2 = D[1] - D[3];
+      const float m00 = ld_plane(pd + size_t(IP_MI) * n, i), m11 = ld_plane(pd + size_t(IP_MI + 1) * n, i),
+                  m22 = ld_plane(pd + size_t(IP_MI + 2) * n, i), m01 = ld_plane(pd + size_t(IP_MI + 3) * n, i),
+                  m02 = ld_plane(pd + size_t(IP_MI + 4) * n, i), m12 = ld_plane(pd + size_t(IP_MI + 5) * n, i);
+      const float w0 = m00 * a0 + m01 * a1 + m02 * a2;
+      const float w1 = m01 * a0 + m11 * a1 + m12 * a2;
+      const float w2 = m02 * a0 + m12 * a1 + m22 * a2;
+      // [w]x column-major: (1,0) = w2, (2,0) = -w1, (0,1) = -w2, (2,1) = w0, (0,2) = w1, (1,2) = -w0
+      const float Om[9] = {0.f, w2, -w1, -w2, 0.f, w0, w1, -w0, 0.f};
+      float dR[9];
+      mul(Rot, Om, dR);
+      float dJ = 0.f;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) dJ = fmaf(C[e], dF[e], dJ);
+      float dC[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      cross_add(dF + 3, F + 6, dC);
+      cross_add(F + 3, dF + 6, dC);
+      cross_add(dF + 6, F, dC + 3);
+      cross_add(F + 6, dF, dC + 3);
+      cross_add(dF, F + 3, dC + 6);
+      cross_add(F, dF + 3, dC + 6);
+      const float mu2 = ld_plane(pd + size_t(IP_C) * n, i), la = ld_plane(pd + size_t(IP_C + 1) * n, i),
+                  laj = ld_plane(pd + size_t(IP_C + 2) * n, i);
+      float A[9];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) A[e] = mu2 * (dF[e] - dR[e]) + (la * dJ) * C[e] + laj * dC[e];
+      mul_bt(A, F, Gm);  // A F^T
+#pragma unroll
+      for (int e = 0; e < 9; ++e) Gm[e] *= v0;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) st_plane(G + size_t(e) * n, i, Gm[e]);
+  }
+  // the scene's largest row sum of |G| (non-negative floats compare as their bits)
+  float rs = 0.f;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) rs = fmaxf(rs, fabsf(Gm[r]) + fabsf(Gm[3 + r]) + fabsf(Gm[6 + r]));
+  const int mx = wave_max_i(int(__float_as_uint(rs)));
+  if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&scal[sid].gmax, uint32_t(mx));
+}
+
+// ---------------------------------------------------------------------------------------
+// Second half: acc_i += sum_p G_p grad w_ip (unscaled grad w: G holds gs) in fixed point. A wave
+// takes 64 sorted positions of one scene: lane k stages particle k's weights and G in LDS, then
+// lane l is the stencil node getIJK(l) and sums the staged particles in registers while the base
+// cell stays the same, adding the rounded sum to the node with integer atomics at each change.
+__global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict__ scenes,
+                                                     const uint2* __restrict__ waves, uint32_t nwaves,
+                                                     const float* __restrict__ pd, const float* __restrict__ G,
+                                                     uint32_t n, unsigned long long* __restrict__ acc,
+                                                     const ImpScal* __restrict__ scal) {
+  __shared__ __attribute__((aligned(16))) float s_stage[4][64 * IMP_STAGE];
+  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= nwaves) return;
+  const uint2 ws = waves ? waves[w] : make_uint2(64u * w, 0u);
+  const int sid = int(ws.y);
+  if (scal[sid].done) return;
+  const SceneDev& S = scenes[sid];
+  const int lane = threadIdx.x & 63;
+  const int64_t end = S.p_off + S.p_cnt;
+  const uint32_t cnt = uint32_t(min<int64_t>(64, end - int64_t(ws.x)));
+  float* st = s_stage[threadIdx.x >> 6];
+  const double scale = ldexp(1.0, imp_exponent(scal[sid].gmax, S.p_cnt));
+  if (uint32_t(lane) < cnt) {
+    const uint32_t i = ws.x + uint32_t(lane);
+    const float x = ld_plane(pd + size_t(IP_X) * n, i), y = ld_plane(pd + size_t(IP_X + 1) * n, i),
+                z = ld_plane(pd + size_t(IP_X + 2) * n, i);
+    const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
+    float* d = st + lane * IMP_STAGE;
+    int cell = -1;
+    if (in_stencil_range(S, ux, uy, uz)) {
+      const float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
+      const float f[3] = {ux - fx, uy - fy, uz - fz};
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+          d[8 * k + 2 * o] = spline_w(o, f[k]);
+          d[8 * k + 2 * o + 1] = spline_q(o, f[k]);
+        }
+      // base cell in [-2, D] per axis, packed 10 bits per axis (IMP_MAX_DIM), non-negative
+      cell = (int(fx) + 2) | ((int(fy) + 2) << 10) | ((int(fz) + 2) << 20);
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) d[24 + e] = ld_plane(G + size_t(e) * n, i);
+    d[33] = __int_as_float(cell);
+  }
+  __builtin_amdgcn_wave_barrier();  // the wave's own LDS writes complete before its reads
+  const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
+  int cur = -1;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  auto commit = [&]() {
+    const int nx = (cur & 1023) - 3 + ox, ny = ((cur >> 10) & 1023) - 3 + oy, nz = (cur >> 20) - 3 + oz;
+    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < S.D[0] && ny < S.D[1] && nz < S.D[2]) {
+      const int64_t node = S.node_off + nx + int64_t(S.D[0]) * (ny + int64_t(S.D[1]) * nz);
+      const long long q0 = llrint(double(s0) * scale), q1 = llrint(double(s1) * scale), q2 = llrint(double(s2) * scale);
+      if (q0) atomicAdd(acc + 3 * node, (unsigned long long)q0);
+      if (q1) atomicAdd(acc + 3 * node + 1, (unsigned long long)q1);
+      if (q2) atomicAdd(acc + 3 * node + 2, (unsigned long long)q2);
+    }
+    s0 = s1 = s2 = 0.f;
+  };
+  for (uint32_t k = 0; k < cnt; ++k) {
+    const float* d = st + k * IMP_STAGE;
+    const int cell = __float_as_int(d[33]);
+    if (cell != cur) {
+      if (cur >= 0) commit();
+      cur = cell;
+    }
+    if (cell < 0) continue;
+    const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
+    const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
+    const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
+    const float wyz = wqy.x * wqz.x;
+    const float g0 = wqx.y * wyz, g1 = wqy.y * (wqx.x * wqz.x), g2 = wqz.y * (wqx.x * wqy.x);
+    // (G grad w)_r = G(r,0) g0 + G(r,1) g1 + G(r,2) g2, G(r,c) = d[24 + c*3 + r]
+    s0 = fmaf(d[24], g0, fmaf(d[27], g1, fmaf(d[30], g2, s0)));
+    s1 = fmaf(d[25], g0, fmaf(d[28], g1, fmaf(d[31], g2, s1)));
+    s2 = fmaf(d[26], g0, fmaf(d[29], g1, fmaf(d[32], g2, s2)));
+  }
+  if (cur >= 0) commit();
+}
+
+// ---------------------------------------------------------------------------------------
+// Block sum of up to three doubles in a fixed order (waves by shuffles, then wave 0 over the waves).
+__device__ __forceinline__ void imp_block_sum(double v[3], int nq, double* __restrict__ part, uint32_t nblocks) {
+  __shared__ double s_red[3][IMP_THREADS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = 0; q < nq; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) s_red[q][wv] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < nq; ++q) {
+      double s = 0.0;
+      for (int k = 0; k < IMP_THREADS / 64; ++k) s += s_red[q][k];
+      part[size_t(q) * nblocks + blockIdx.x] = s;
+    }
+}
+
+// Node phases of the solve (K = M v - beta dt df, df = -acc 2^-e):
+//   IMP_INIT: vin = x = v*: r = M v* - K x, p = Kp = 0; partials |M v*|^2, |r|^2
+//   IMP_KR:   vin = r:      Kr = K r; partial r . Kr
+//   IMP_P:    p = r + beta p, Kp = Kr + beta Kp; partial Kp . Kp
+//   IMP_X:    x += alpha p, r -= alpha Kp; partial |r|^2
+template <int PHASE>
+__global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __restrict__ scenes,
+                                                          const ImpBlock* __restrict__ blocks, uint32_t nblocks,
+                                                          const float* __restrict__ mass, float4* __restrict__ vec,
+                                                          long long* __restrict__ acc,
+                                                          const ImpScal* __restrict__ scal, int64_t total_nodes,
+                                                          double* __restrict__ part) {
+  const ImpBlock B = blocks[blockIdx.x];
+  const ImpScal& Z = scal[B.scene];
+  if (PHASE != IMP_INIT && Z.done) return;  // the scene's partials are not read again
+  const SceneDev& S = scenes[B.scene];
+  const int64_t end = S.node_off + int64_t(S.D[0]) * S.D[1] * S.D[2];
+  float4* X = vec;
+  float4* Rv = vec + total_nodes;
+  float4* P = vec + 2 * total_nodes;
+  float4* KP = vec + 3 * total_nodes;
+  float4* KR = vec + 4 * total_nodes;
+  const double inv = (PHASE == IMP_INIT || PHASE == IMP_KR) ? ldexp(1.0, -imp_exponent(Z.gmax, S.p_cnt)) : 0.0;
+  const float bdt = Z.beta_dt;
+  const float al = float(Z.alpha), be = float(Z.beta);
+  double sum[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < IMP_NODES_PER_THREAD; ++j) {
+    const int64_t nd = B.first + j * IMP_THREADS + threadIdx.x;
+    if (nd >= end) break;
+    const float m = mass[nd];
+    if (PHASE == IMP_INIT || PHASE == IMP_KR) {
+      // df = -acc 2^-e; K v = m v - beta dt df = m v + beta dt acc 2^-e
+      const float a0 = float(double(acc[3 * nd]) * inv), a1 = float(double(acc[3 * nd + 1]) * inv),
+                  a2 = float(double(acc[3 * nd + 2]) * inv);
+      acc[3 * nd] = 0;
+      acc[3 * nd + 1] = 0;
+      acc[3 * nd + 2] = 0;
+      if (PHASE == IMP_INIT) {
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f), b = r;
+        if (m > 0.0f) {
+          const float4 x = X[nd];
+          b = make_float4(m * x.x, m * x.y, m * x.z, 0.f);
+          const float kx = fmaf(bdt, a0, b.x), ky = fmaf(bdt, a1, b.y), kz = fmaf(bdt, a2, b.z);
+          r = make_float4(b.x - kx, b.y - ky, b.z - kz, 0.f);
+        }
+        Rv[nd] = r;
+        P[nd] = make_float4(0.f, 0.f, 0.f, 0.f);
+        KP[nd] = make_float4(0.f, 0.f, 0.f, 0.f);
+        KR[nd] = make_float4(0.f, 0.f, 0.f, 0.f);
+        sum[0] += double(b.x) * b.x + double(b.y) * b.y + double(b.z) * b.z;
+        sum[1] += double(r.x) * r.x + double(r.y) * r.y + double(r.z) * r.z;
+      } else {
+        float4 kr = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 r = Rv[nd];
+        if (m > 0.0f) kr = make_float4(fmaf(bdt, a0, m * r.x), fmaf(bdt, a1, m * r.y), fmaf(bdt, a2, m * r.z), 0.f);
+        KR[nd] = kr;
+        sum[0] += double(r.x) * kr.x + double(r.y) * kr.y + double(r.z) * kr.z;
+      }
+    } else if (PHASE == IMP_P) {
+      const float4 r = Rv[nd], kr = KR[nd], p = P[nd], kp = KP[nd];
+      const float4 pn = make_float4(fmaf(be, p.x, r.x), fmaf(be, p.y, r.y), fmaf(be, p.z, r.z), 0.f);
+      const float4 kpn = make_float4(fmaf(be, kp.x, kr.x), fmaf(be, kp.y, kr.y), fmaf(be, kp.z, kr.z), 0.f);
+      P[nd] = pn;
+      KP[nd] = kpn;
+      sum[0] += double(kpn.x) * kpn.x + double(kpn.y) * kpn.y + double(kpn.z) * kpn.z;
+    } else {
+      const float4 p = P[nd], kp = KP[nd], x = X[nd], r = Rv[nd];
+      X[nd] = make_float4(fmaf(al, p.x, x.x), fmaf(al, p.y, x.y), fmaf(al, p.z, x.z), 0.f);
+      const float4 rn = make_float4(fmaf(-al, kp.x, r.x), fmaf(-al, kp.y, r.y), fmaf(-al, kp.z, r.z), 0.f);
+      Rv[nd] = rn;
+      sum[0] += double(rn.x) * rn.x + double(rn.y) * rn.y + double(rn.z) * rn.z;
+    }
+  }
+  imp_block_sum(sum, PHASE == IMP_INIT ? 2 : 1, part, nblocks);
+}
+
+// One workgroup per scene: the scene's block partials summed in a fixed order, then the scalars
+// of the phase. START (phase < 0) resets the state for a new solve.
+__global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const uint32_t* __restrict__ scene_blocks,
+                                                             uint32_t nblocks, const double* __restrict__ part,
+                                                             ImpScal* __restrict__ scal, const SceneDev* __restrict__ scenes,
+                                                             float dt_override, float beta, float tol, int max_iter) {
+  const int sid = blockIdx.x;
+  ImpScal& Z = scal[sid];
+  if (phase < 0) {
+    if (threadIdx.x == 0) {
+      const float dt = dt_override > 0.0f ? dt_override : scenes[sid].dt;
+      Z.rho = Z.alpha = Z.beta = Z.bb = Z.rr = 0.0;
+      Z.tol2 = double(tol) * double(tol);
+      Z.gmax = 0u;
+      Z.iters = 0;
+      Z.done = 0;
+      Z.max_iter = max_iter;
+      Z.dt = dt;
+      Z.beta_dt = beta * dt;
+    }
+    return;
+  }
+  if (phase != IMP_INIT && Z.done) return;
+  const uint32_t b0 = scene_blocks[sid], b1 = scene_blocks[sid + 1];
+  const int nq = phase == IMP_INIT ? 2 : 1;
+  __shared__ double s_v[3][IMP_THREADS];
+  for (int q = 0; q < nq; ++q) {
+    double s = 0.0;
+    for (uint32_t b = b0 + threadIdx.x; b < b1; b += IMP_THREADS) s += part[size_t(q) * nblocks + b];
+    s_v[q][threadIdx.x] = s;
+  }
+  __syncthreads();
+  for (int h = IMP_THREADS / 2; h >= 1; h >>= 1) {
+    if (int(threadIdx.x) < h)
+      for (int q = 0; q < nq; ++q) s_v[q][threadIdx.x] += s_v[q][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double t0 = s_v[0][0], t1 = s_v[1][0];
+  if (phase == IMP_INIT) {
+    Z.bb = t0;
+    Z.rr = t1;
+    Z.gmax = 0u;
+    Z.done = (t1 <= Z.tol2 * t0 || Z.max_iter <= 0) ? 1 : 0;
+  } else if (phase == IMP_KR) {
+    Z.beta = Z.iters == 0 ? 0.0 : t0 / Z.rho;
+    Z.rho = t0;
+    Z.gmax = 0u;
+  } else if (phase == IMP_P) {
+    if (t0 > 0.0) Z.alpha = Z.rho / t0;
+    else Z.done = 1;  // K p = 0: no further progress
+  } else {
+    Z.rr = t0;
+    Z.iters += 1;
+    Z.done = (t0 <= Z.tol2 * Z.bb || Z.iters >= Z.max_iter) ? 1 : 0;
+  }
+}
+
+// v = x where m > 0: the grid update's record becomes (v, u + v - v*), u = v* - alpha p/m.
+__global__ void __launch_bounds__(IMP_THREADS) k_imp_finish(const SceneDev* __restrict__ scenes,
+                                                            const ImpBlock* __restrict__ blocks,
+                                                            const float* __restrict__ mass,
+                                                            const float4* __restrict__ vec,
+                                                            float2* __restrict__ g_node, float* __restrict__ g_mass) {
+  const ImpBlock B = blocks[blockIdx.x];
+  const SceneDev& S = scenes[B.scene];
+  const int64_t end = S.node_off + int64_t(S.D[0]) * S.D[1] * S.D[2];
+#pragma unroll
+  for (int j = 0; j < IMP_NODES_PER_THREAD; ++j) {
+    const int64_t nd = B.first + j * IMP_THREADS + threadIdx.x;
+    if (nd >= end) break;
+    const float m = mass[nd];
+    if (!(m > 0.0f)) continue;
+    const float4 x = vec[nd];
+    const float2 a = g_node[3 * nd], b = g_node[3 * nd + 1], c = g_node[3 * nd + 2];
+    if (x.x == a.x && x.y == a.y && x.z == b.x) continue;  // unchanged (beta = 0, no iteration): keep the bits
+    store_node(g_node, g_mass, nd, x.x, x.y, x.z, b.y + (x.x - a.x), c.x + (x.y - a.y), c.y + (x.z - b.x), m);
+  }
+}
+
+}  // namespace snow

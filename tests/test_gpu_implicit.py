@@ -1,0 +1,140 @@
+"""The semi-implicit step (snow_step_semi_implicit) against the FP64 oracle's restatement of the
+published method (Stomakhin et al. 2013, eq. 9; the reference's SemiImplicitTimeUpdate is empty,
+semiimplicittimeupdate.cpp:3-5), plus its invariants: beta = 0 is the explicit step bit for bit, runs
+repeat bit for bit, a scene gives the same bits alone or in a batch.
+
+Tolerances: the device solves in FP32 with the rotation of the reference SVD (V rounded to 1e-5),
+the oracle in FP64 with the exact polar factor; both stop at the same relative residual. Differences
+are therefore of the order of the FP32 rounding of the explicit step plus 1e-5 of the implicit
+correction.
+"""
+import numpy as np
+import pytest
+
+from snow_amd import SnowContext, SnowError
+from snow_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def deformed(scene, amp=0.03, seed=5):
+    rng = np.random.default_rng(seed)
+    n = len(scene.particles)
+    fe = np.eye(3)[None] + amp * rng.uniform(-1, 1, (n, 3, 3))
+    fp = np.eye(3)[None] + 0.3 * amp * rng.uniform(-1, 1, (n, 3, 3))
+    scene.particles.fe = np.ascontiguousarray(fe.transpose(0, 2, 1).reshape(n, 9), np.float32)
+    scene.particles.fp = np.ascontiguousarray(fp.transpose(0, 2, 1).reshape(n, 9), np.float32)
+    return scene
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def test_beta_zero_is_the_explicit_step():
+    sc = deformed(scenes.snowball_drop())
+    with SnowContext(sc) as a, SnowContext(sc) as b:
+        a.init()
+        b.init()
+        for _ in range(4):
+            a.step(sc.dt)
+            b.step_semi_implicit(sc.dt, beta=0.0)
+            assert b.implicit_stats()[0] == 0
+        pa, pb = a.read_particles(), b.read_particles()
+    for k in ("x", "v", "fe", "fp"):
+        assert np.array_equal(pa[k], pb[k]), k
+
+
+@pytest.mark.parametrize("beta", [1.0, 0.5])
+def test_one_step_teacher_forced(oracle_mod, beta):
+    """From the same deformed state, one step at ten times the scene's dt: the GPU against the oracle,
+    and the implicit correction (semi-implicit minus explicit) well above that difference."""
+    sc = deformed(scenes.snowball_drop())
+    sc.particles.v[:] = np.random.default_rng(3).normal(0, 0.5, sc.particles.v.shape).astype(np.float32)
+    dt, tol = 1e-3, 1e-6
+    o = oracle_mod.Oracle(sc)
+    o.init()
+    o.rho0[:] = o.rho0.astype(np.float32)
+    print(f"\nscene: snowball drop, {len(sc.particles)} particles, dt {dt}, beta {beta}", flush=True)
+    with SnowContext(sc) as ctx, SnowContext(sc) as ex:
+        for c in (ctx, ex):
+            c.init()
+            c.write_particles(rho0=o.rho0.astype(np.float32))
+        ctx.step_semi_implicit(dt, beta=beta, tol=tol, max_iter=200)
+        it_gpu, rr_gpu = ctx.implicit_stats()
+        g = ctx.read_particles()
+        ex.step(dt)
+        e = ex.read_particles()
+    it_o, rr_o = o.step_semi_implicit(dt, beta, tol, 200)
+    print(f"iterations GPU {it_gpu} (residual {rr_gpu:.2e}), oracle {it_o} (residual {rr_o:.2e})")
+    dx, dv = np.abs(g["x"] - o.x).max(), rel_err(g["v"], o.v)
+    dfe, dfp = np.abs(g["fe"] - o.fe).max(), np.abs(g["fp"] - o.fp).max()
+    corr = rel_err(e["v"], g["v"])
+    print(f"x {dx:.2e}  v rel {dv:.2e}  F_E {dfe:.2e}  F_P {dfp:.2e}  implicit correction of v (rel) {corr:.2e}")
+    assert rr_gpu <= tol and rr_o <= tol
+    assert 0 < it_gpu < 200 and 0 < it_o < 200
+    assert dx < 1e-6
+    assert dv < 1e-4
+    assert dfe < 1e-4 and dfp < 1e-4
+    assert corr > 20 * dv
+
+
+def test_free_running_against_oracle(oracle_mod):
+    """20 semi-implicit steps of the deformed snowball at 5x its dt, free running."""
+    sc = deformed(scenes.snowball_drop(), amp=0.02)
+    dt, steps = 5e-4, 20
+    o = oracle_mod.Oracle(sc, nthreads=8)
+    o.init()
+    o.rho0[:] = o.rho0.astype(np.float32)
+    with SnowContext(sc) as ctx:
+        ctx.init()
+        ctx.write_particles(rho0=o.rho0.astype(np.float32))
+        for _ in range(steps):
+            ctx.step_semi_implicit(dt, beta=1.0, tol=1e-6, max_iter=200)
+        ctx.check_finite()
+        g = ctx.read_particles()
+    for _ in range(steps):
+        o.step_semi_implicit(dt, 1.0, 1e-6, 200)
+    dx, dv = np.abs(g["x"] - o.x).max(), rel_err(g["v"], o.v)
+    dfe = np.abs(g["fe"] - o.fe).max()
+    print(f"\n{steps} steps: x {dx:.2e}  v rel {dv:.2e}  F_E {dfe:.2e}")
+    assert dx < 1e-5
+    assert dv < 1e-3
+    assert dfe < 1e-3
+
+
+def test_deterministic_and_batch_equals_single():
+    a_sc = deformed(scenes.snowball_drop(), seed=1)
+    b_sc = deformed(scenes.tiny_block(n_side=16, dims=32, velocity=(0.3, -0.5, 0.2)), seed=2)
+    dt = 5e-4
+    out = []
+    for scs in ([a_sc], [a_sc], [a_sc, b_sc]):
+        with SnowContext(scs) as ctx:
+            ctx.init()
+            for _ in range(3):
+                ctx.step_semi_implicit(dt, beta=1.0, tol=1e-6, max_iter=100)
+            out.append([ctx.read_particles(s) for s in range(len(scs))])
+    with SnowContext([b_sc]) as ctx:
+        ctx.init()
+        for _ in range(3):
+            ctx.step_semi_implicit(dt, beta=1.0, tol=1e-6, max_iter=100)
+        single_b = ctx.read_particles()
+    for k in ("x", "v", "fe", "fp"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k  # run to run
+        assert np.array_equal(out[0][0][k], out[2][0][k]), k  # scene a alone / in the batch
+        assert np.array_equal(single_b[k], out[2][1][k]), k  # scene b alone / in the batch
+
+
+def test_argument_checks():
+    sc = scenes.tiny_block(n_side=4, dims=8)
+    with SnowContext(sc) as ctx:
+        with pytest.raises(SnowError):
+            ctx.implicit_stats()  # no solve yet
+        ctx.init()
+        for kw in ({"beta": -0.1}, {"beta": 1.5}, {"tol": 0.0}, {"max_iter": -1}):
+            with pytest.raises(SnowError):
+                ctx.step_semi_implicit(1e-4, **kw)
+        ctx.step_semi_implicit(1e-4)
+        it, rr = ctx.implicit_stats()
+        assert it >= 0 and rr >= 0.0

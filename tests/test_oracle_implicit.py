@@ -1,0 +1,143 @@
+"""Pins the oracle's semi-implicit update (oracle/mpm_oracle.hpp, "Semi-implicit velocity update").
+
+The reference's SemiImplicitTimeUpdate is empty (semiimplicittimeupdate.cpp:3-5), so the oracle
+restates the published method (Stomakhin et al. 2013, eq. 9). Each part is pinned by what the
+mathematics fixes: the Hessian action is the derivative of the stress (central differences, and the
+closed form of linear elasticity at F = I), it is rotation-equivariant, the node operator is the
+derivative of the node force and conserves momentum, K is symmetric, the solve meets its residual,
+and beta = 0 is the explicit step bit for bit.
+"""
+import numpy as np
+import pytest
+
+from snow_amd import scenes
+
+MU, LA = 5.8e4, 3.9e4
+
+
+def rand_F(rng, scale):
+    F = np.eye(3) + scale * rng.standard_normal((3, 3))
+    if np.linalg.det(F) < 0:
+        F[:, 0] *= -1
+    return F
+
+
+def rotation(rng):
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def test_polar_exact_is_the_rotation_factor(oracle_mod):
+    rng = np.random.default_rng(11)
+    for scale in (0.01, 0.1, 0.4):
+        F = rand_F(rng, scale)
+        R = oracle_mod.polar_exact(F)
+        np.testing.assert_allclose(R.T @ R, np.eye(3), atol=1e-13)
+        assert np.linalg.det(R) == pytest.approx(1.0, abs=1e-13)
+        S = R.T @ F
+        np.testing.assert_allclose(S, S.T, atol=1e-13)
+        assert np.all(np.linalg.eigvalsh((S + S.T) / 2) > 0)
+        U, _, Vt = np.linalg.svd(F)
+        np.testing.assert_allclose(R, U @ Vt, atol=1e-12)
+
+
+@pytest.mark.parametrize("scale", [0.02, 0.1, 0.3])
+def test_hessian_is_the_stress_derivative(oracle_mod, scale):
+    rng = np.random.default_rng(int(scale * 1000))
+    for _ in range(5):
+        F = rand_F(rng, scale)
+        dF = rng.standard_normal((3, 3))
+        A = oracle_mod.fc_hessian(F, MU, LA, dF)
+        eps = 1e-4
+        fd = (oracle_mod.fc_stress(F + eps * dF, MU, LA) - oracle_mod.fc_stress(F - eps * dF, MU, LA)) / (2 * eps)
+        assert np.abs(fd - A).max() <= 1e-7 * np.abs(A).max()
+
+
+def test_hessian_at_identity_is_linear_elasticity(oracle_mod):
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        dF = rng.standard_normal((3, 3))
+        A = oracle_mod.fc_hessian(np.eye(3), MU, LA, dF)
+        lin = MU * (dF + dF.T) + LA * np.trace(dF) * np.eye(3)
+        np.testing.assert_allclose(A, lin, rtol=0, atol=1e-9 * np.abs(lin).max())
+        # a rotation rate (skew dF) costs no energy at F = I
+        W = dF - dF.T
+        np.testing.assert_allclose(oracle_mod.fc_hessian(np.eye(3), MU, LA, W), 0.0, atol=1e-9 * MU)
+
+
+def test_hessian_rotation_equivariant(oracle_mod):
+    rng = np.random.default_rng(8)
+    for _ in range(5):
+        F, dF, Q = rand_F(rng, 0.2), rng.standard_normal((3, 3)), rotation(rng)
+        A = oracle_mod.fc_hessian(F, MU, LA, dF)
+        AQ = oracle_mod.fc_hessian(Q @ F, MU, LA, Q @ dF)
+        np.testing.assert_allclose(AQ, Q @ A, atol=1e-9 * np.abs(A).max())
+
+
+def test_hessian_is_symmetric_bilinear(oracle_mod):
+    rng = np.random.default_rng(9)
+    for _ in range(5):
+        F, X, Y = rand_F(rng, 0.2), rng.standard_normal((3, 3)), rng.standard_normal((3, 3))
+        a = np.sum(X * oracle_mod.fc_hessian(F, MU, LA, Y))
+        b = np.sum(Y * oracle_mod.fc_hessian(F, MU, LA, X))
+        assert a == pytest.approx(b, rel=1e-10)
+
+
+def deformed_oracle(oracle_mod, seed=3, n_side=8, dims=16, fe_scale=0.05):
+    sc = scenes.tiny_block(n_side=n_side, dims=dims, velocity=(0.5, -1.0, 0.25))
+    o = oracle_mod.Oracle(sc)
+    o.init()
+    rng = np.random.default_rng(seed)
+    n = len(sc.particles)
+    o.fe[:] = np.stack([rand_F(rng, fe_scale).T.reshape(9) for _ in range(n)])
+    o.fp[:] = np.stack([rand_F(rng, 0.01).T.reshape(9) for _ in range(n)])
+    return sc, o
+
+
+def test_node_operator_is_the_force_derivative(oracle_mod):
+    _, o = deformed_oracle(oracle_mod)
+    rng = np.random.default_rng(4)
+    du = rng.standard_normal((o.gm.shape[0], 3)) * 1e-3
+    df = o.implicit_df(du)
+    eps = 1e-2
+    fd = (o.implicit_force(eps * du) - o.implicit_force(-eps * du)) / (2 * eps)
+    assert np.abs(df).max() > 0
+    assert np.abs(fd - df).max() <= 1e-8 * np.abs(df).max()
+
+
+def test_node_operator_conserves_momentum(oracle_mod):
+    # every stencil inside the grid: sum_i grad w_ip = 0, so sum_i df_i = 0
+    _, o = deformed_oracle(oracle_mod)
+    du = np.random.default_rng(6).standard_normal((o.gm.shape[0], 3))
+    df = o.implicit_df(du)
+    assert np.abs(df.sum(0)).max() <= 1e-12 * np.abs(df).max()
+
+
+def test_system_symmetric_and_solved(oracle_mod):
+    sc, o = deformed_oracle(oracle_mod)
+    o.p2g()
+    o.grid_update(sc.dt)
+    dt, beta = 1e-3, 1.0
+    rng = np.random.default_rng(7)
+    a, b = rng.standard_normal(o.gvstar.shape), rng.standard_normal(o.gvstar.shape)
+    Ka, Kb = o.implicit_apply(dt, beta, a), o.implicit_apply(dt, beta, b)
+    m = o.gm > 0
+    assert (b[m] * Ka[m]).sum() == pytest.approx((a[m] * Kb[m]).sum(), rel=1e-10)
+    vstar = o.gvstar.copy()
+    it, rel = o.implicit_solve(dt, beta, tol=1e-9, max_iter=500)
+    assert 0 < it < 500 and rel <= 1e-9
+    rhs = o.gm[:, None] * vstar
+    res = o.implicit_apply(dt, beta, o.gvstar) - rhs
+    assert np.linalg.norm(res[m]) <= 1.01e-9 * np.linalg.norm(rhs[m])
+    assert np.abs(o.gvstar - vstar).max() > 1e-6  # the solve changes the velocities
+
+
+def test_beta_zero_is_the_explicit_step(oracle_mod):
+    sc, o = deformed_oracle(oracle_mod)
+    _, e = deformed_oracle(oracle_mod)
+    for _ in range(3):
+        it, _ = o.step_semi_implicit(sc.dt, 0.0, 1e-6, 50)
+        e.step(sc.dt)
+        assert it == 0
+    for name in ("x", "v", "fe", "fp"):
+        assert np.array_equal(getattr(o, name), getattr(e, name)), name
