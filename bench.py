@@ -14,6 +14,9 @@ FP32 form's particle-steps/s. The same line also reports
   * forms.stress_slab - the 2^24-particle slab impact on 256^3 (plastic clamp active), plain and
                      mixed throughput, and both forms' errors against the FP64 oracle on a 2^20-
                      particle version on the same grid,
+  * forms.semi_implicit - the semi-implicit step (beta = 1) on the same scene in contact at 4x the dt:
+                     ms per step and per solver iteration, and its error against the FP64 oracle on the
+                     64^3 snowball,
   * forms.large_batch - (--large) 512 small scenes plus one 2^25-particle 512^3 scene in one graph,
   * cpu_baseline   - the FP32 CPU restatement of the step on the host cores this process may use
                      (median of 3 runs), with the 64^3 scene and the sweep scene by scene beside it.
@@ -333,6 +336,72 @@ def stress_slab(steps: int, error_steps: int) -> dict:
     return out
 
 
+def semi_implicit(steps: int) -> dict:
+    """The semi-implicit step (snow_step_semi_implicit, beta = 1) on the headline geometry with the
+    physical weight gradient (1/h; with the reference's gradient the stiffness term is h^2 smaller and
+    the solve stops at its first residual), once the balls are in contact (600 explicit steps of 5e-5 s
+    first), at four times the explicit dt: ms per step, solver iterations, particle-steps/s; and its
+    error against the FP64 oracle on the 64^3 snowball with a deformed F_E after 10 steps at five times
+    that scene's dt."""
+    import numpy as np
+    from oracle import pyoracle
+    from snow_amd import SnowContext, scenes
+    sc = scenes.two_snowballs(grad_scale=0.0)
+    n = len(sc.particles)
+    dt, tol, max_iter = 4 * sc.dt, 1e-5, 100
+    out = {"workload": "two_snowballs_128^3 in contact, gradient 1/h", "particles": n, "dt": dt, "beta": 1.0,
+           "tol": tol,
+           "steps": steps, "unit": "particle-steps/s"}
+    with SnowContext(sc) as ctx:
+        ctx.init()
+        for _ in range(600):
+            ctx.step(sc.dt)
+        ctx.step_semi_implicit(dt, 1.0, tol, max_iter)
+        ctx.sync()
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.step_semi_implicit(dt, 1.0, tol, max_iter)
+            its.append(ctx.implicit_stats()[0])
+        ctx.sync()
+        el = time.perf_counter() - t0
+        ctx.check_finite()
+        t0 = time.perf_counter()
+        for _ in range(4 * steps):
+            ctx.step(sc.dt)
+        ctx.sync()
+        el_ex = time.perf_counter() - t0
+    out.update({"value": n * steps / el, "ms_per_step": el / steps * 1e3, "iterations": its,
+                "ms_per_iteration": (el - steps * el_ex / (4 * steps)) / max(1, sum(its)) * 1e3,
+                "explicit_ms_per_step": el_ex / (4 * steps) * 1e3})
+    # error against the oracle (deformed F_E so that the solve iterates)
+    small = scenes.snowball_drop()
+    rng = np.random.default_rng(5)
+    m = len(small.particles)
+    fe = np.eye(3)[None] + 0.02 * rng.uniform(-1, 1, (m, 3, 3))
+    small.particles.fe = np.ascontiguousarray(fe.transpose(0, 2, 1).reshape(m, 9), np.float32)
+    sdt, ssteps = 5 * small.dt, 10
+    o = pyoracle.Oracle(small, nthreads=min(16, host_cpu()["usable_cpus"]))
+    o.init()
+    o.rho0[:] = o.rho0.astype(np.float32)
+    with SnowContext(small) as ctx:
+        ctx.init()
+        ctx.write_particles(rho0=o.rho0.astype(np.float32))
+        for _ in range(ssteps):
+            ctx.step_semi_implicit(sdt, 1.0, 1e-6, 200)
+        g_it = ctx.implicit_stats()[0]
+        p = ctx.read_particles()
+    for _ in range(ssteps):
+        o_it, _ = o.step_semi_implicit(sdt, 1.0, 1e-6, 200)
+    err = {"workload": "snowball_drop_64^3, deformed F_E", "particles": m, "dt": sdt, "steps": ssteps,
+           "iterations_last_step": {"gpu": g_it, "oracle": o_it}}
+    for name, a, b in (("x", p["x"], o.x), ("v", p["v"], o.v), ("F_E", p["fe"], o.fe), ("F_P", p["fp"], o.fp)):
+        d = np.sqrt(((np.asarray(a, np.float64) - b) ** 2).sum(-1))
+        err[name] = {"max": float(d.max()), "rms": float(np.sqrt((d ** 2).mean()))}
+    out["errors"] = err
+    return out
+
+
 def large_batch(steps: int) -> dict:
     """512 small scenes (1k-64k particles, grids 16^3-64^3) plus one 2^25-particle 512^3 scene in one
     step graph; throughput of the whole batch and of its two parts run alone."""
@@ -405,12 +474,14 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-second-form", action="store_true", help="skip the mixed (or plain) second form")
     ap.add_argument("--no-stress", action="store_true", help="skip the 2^24-particle slab stress scene")
+    ap.add_argument("--no-semi-implicit", action="store_true", help="skip the semi-implicit step")
     ap.add_argument("--stress-steps", type=int, default=10)
     ap.add_argument("--large", action="store_true", help="also run the 512+1-scene batch (2^25 + ~8M particles)")
     ap.add_argument("--large-steps", type=int, default=10)
     ap.add_argument("--reference-scenes", action="store_true",
                     help="also time the reference's own scene programs (src/exe/*.cpp) on the 201^3 grid")
     ap.add_argument("--only-sweep", action="store_true", help="run only the batched 64-scene sweep")
+    ap.add_argument("--only-semi-implicit", action="store_true", help="run only the semi-implicit step")
     ap.add_argument("--error-steps", type=int, default=100)
     ap.add_argument("--stress-error-steps", type=int, default=20)
     ap.add_argument("--sweep-steps", type=int, default=100)
@@ -422,6 +493,9 @@ def main():
         return
     if args.only_sweep:
         print(json.dumps({"forms": {"batched": batched_sweep(args.sweep_steps)}}), flush=True)
+        return
+    if args.only_semi_implicit:
+        print(json.dumps({"forms": {"semi_implicit": semi_implicit(5)}}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding of the "
@@ -536,6 +610,9 @@ def main():
     if not args.no_stress:
         out["forms"]["stress_slab"] = stress_slab(args.stress_steps, args.stress_error_steps)
         print("[bench] stress slab done", file=sys.stderr, flush=True)
+    if not args.no_semi_implicit:
+        out["forms"]["semi_implicit"] = semi_implicit(5)
+        print("[bench] semi-implicit step done", file=sys.stderr, flush=True)
     if args.large:
         out["forms"]["large_batch"] = large_batch(args.large_steps)
     if args.reference_scenes:

@@ -154,6 +154,7 @@ struct snow_ctx {
   ImpBlock* d_imp_blocks = nullptr;      // node blocks (scene, first node)
   uint32_t* d_imp_scene_blocks = nullptr;  // first node block of every scene, then the total
   ImpScal* d_imp_scal = nullptr;         // per-scene solver state
+  uint32_t* d_imp_gslots = nullptr;      // IMP_GSLOTS bound slots per scene
   ImpScal* h_imp_scal = nullptr;         // pinned host copy for the convergence polls
   uint32_t imp_nblocks = 0;
 };
@@ -469,13 +470,13 @@ int ensure_implicit(snow_ctx* c) {
   auto bad = [&](hipError_t e) {
     if (e == hipSuccess) return false;
     void* ptrs[] = {c->d_imp_vec, c->d_imp_mass, c->d_imp_acc, c->d_imp_pd, c->d_imp_G, c->d_imp_part,
-                    c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal};
+                    c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal, c->d_imp_gslots};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (c->h_imp_scal) (void)hipHostFree(c->h_imp_scal);
     c->d_imp_vec = nullptr; c->d_imp_mass = nullptr; c->d_imp_acc = nullptr; c->d_imp_pd = nullptr;
     c->d_imp_G = nullptr; c->d_imp_part = nullptr; c->d_imp_blocks = nullptr; c->d_imp_scene_blocks = nullptr;
-    c->d_imp_scal = nullptr; c->h_imp_scal = nullptr;
+    c->d_imp_scal = nullptr; c->h_imp_scal = nullptr; c->d_imp_gslots = nullptr;
     return true;
   };
   if (bad(dalloc(&c->d_imp_vec, nn * IMP_VECS)) || bad(dalloc(&c->d_imp_mass, nn)) ||
@@ -483,6 +484,7 @@ int ensure_implicit(snow_ctx* c) {
       bad(dalloc(&c->d_imp_G, np * 9)) || bad(dalloc(&c->d_imp_part, 3 * std::max<size_t>(1, blocks.size()))) ||
       bad(dalloc(&c->d_imp_blocks, std::max<size_t>(1, blocks.size()))) ||
       bad(dalloc(&c->d_imp_scene_blocks, first.size())) || bad(dalloc(&c->d_imp_scal, size_t(c->nscenes))) ||
+      bad(dalloc(&c->d_imp_gslots, size_t(c->nscenes) * IMP_GSLOTS)) ||
       bad(hipHostMalloc(reinterpret_cast<void**>(&c->h_imp_scal), sizeof(ImpScal) * c->nscenes)))
     return fail(c, SNOW_E_CAPACITY, "semi-implicit solve: no device memory for its node vectors");
   HIPCHK(c, hipMemsetAsync(c->d_imp_acc, 0, sizeof(long long) * 3 * nn, c->stream));
@@ -507,7 +509,8 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
   const unsigned pg = grid_for(c->nwaves, 4u);
   auto scalars = [&](int phase) -> int {
     k_imp_scalars<<<c->nscenes, IMP_THREADS, 0, c->stream>>>(phase, c->d_imp_scene_blocks, nb, c->d_imp_part,
-                                                             c->d_imp_scal, c->d_scenes, dt, beta, tol, max_iter);
+                                                             c->d_imp_scal, c->d_scenes, dt, beta, tol, max_iter,
+                                                             c->d_imp_gslots);
     LAUNCHCHK(c);
     return 0;
   };
@@ -515,15 +518,18 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
   auto apply = [&](int v) -> int {
     if (c->N) {
       k_imp_apply<<<pg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_imp_pd, c->N,
-                                             c->d_imp_vec + size_t(v) * nn, c->d_imp_G, c->d_imp_scal);
+                                             c->d_imp_vec + size_t(v) * nn, c->d_imp_G, c->d_imp_scal,
+                                             c->d_imp_gslots);
       LAUNCHCHK(c);
       k_imp_scatter<<<pg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_imp_pd, c->d_imp_G, c->N,
-                                               reinterpret_cast<unsigned long long*>(c->d_imp_acc), c->d_imp_scal);
+                                               reinterpret_cast<unsigned long long*>(c->d_imp_acc), c->d_imp_scal,
+                                               c->d_imp_gslots);
       LAUNCHCHK(c);
     }
     return 0;
   };
-#define NODE_ARGS c->d_scenes, c->d_imp_blocks, nb, c->d_imp_mass, c->d_imp_vec, c->d_imp_acc, c->d_imp_scal, nn, c->d_imp_part
+#define NODE_ARGS c->d_scenes, c->d_imp_blocks, nb, c->d_imp_mass, c->d_imp_vec, c->d_imp_acc, c->d_imp_scal, nn, \
+                  c->d_imp_part, c->d_imp_gslots
   if ((rc = scalars(-1))) return rc;
   if (c->N) {
     k_imp_prep<<<pg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_slot, c->d_rec[c->cur],
@@ -608,7 +614,7 @@ void release(snow_ctx* c) {
                   c->d_tile_c0, c->d_tile_c1, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage,
                   c->d_imp_vec, c->d_imp_mass, c->d_imp_acc, c->d_imp_pd, c->d_imp_G, c->d_imp_part,
-                  c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal};
+                  c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal, c->d_imp_gslots};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_imp_scal) (void)hipHostFree(c->h_imp_scal);

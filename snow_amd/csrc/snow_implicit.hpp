@@ -34,7 +34,7 @@ struct ImpScal {
   double bb;     // |M v*|^2
   double rr;     // |r|^2
   double tol2;   // squared relative residual at which the solve stops
-  uint32_t gmax; // float bits of the largest row sum of |G_p| of the current application
+  uint32_t spare;
   int32_t iters; // direction updates done
   int32_t done;  // converged, broke down or out of iterations
   int32_t max_iter;
@@ -175,94 +175,150 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_setup(const SceneDev* __res
 }
 
 // ---------------------------------------------------------------------------------------
+// The largest row sum of |G| of an application is published as atomic maxima into IMP_GSLOTS slots
+// per scene (wave w into slot w mod IMP_GSLOTS), so that the maxima of many waves do not serialise
+// on one address; readers take the maximum of the slots.
+constexpr int IMP_GSLOTS = 64;
+__device__ __forceinline__ uint32_t imp_gmax(const uint32_t* __restrict__ gslots, int sid) {
+  return uint32_t(wave_max_i(int(gslots[sid * IMP_GSLOTS + (threadIdx.x & 63)])));
+}
+
+constexpr int IMP_BOX = 384;       // nodes of a wave's LDS gather box (as the G2P box)
+constexpr int IMP_ACC_BOX = 768;   // nodes of a workgroup's LDS accumulation box
+
 // First half of K applied to the node vector vin: per particle dF = dt (sum_j vin_j grad w_jp^T) F,
 // A = d2Psi/dF2 : dF and G = gs V0 A F^T (9 planes, column-major), and the scene's largest row sum
-// of |G| for the fixed-point scale.
+// of |G| for the fixed-point scale. A full wave of interior particles gathers from an LDS copy of the
+// node box its stencils cover (sorted particles span a few cells), else from global memory.
 __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ scenes, const uint2* __restrict__ waves,
                                                    uint32_t nwaves, const float* __restrict__ pd, uint32_t n,
                                                    const float4* __restrict__ vin, float* __restrict__ G,
-                                                   ImpScal* __restrict__ scal) {
+                                                   const ImpScal* __restrict__ scal, uint32_t* __restrict__ gslots) {
+  __shared__ float4 s_box[4][IMP_BOX];
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= nwaves) return;
   const uint2 ws = waves ? waves[w] : make_uint2(64u * w, 0u);
   const int sid = int(ws.y);
   if (scal[sid].done) return;
   const SceneDev& S = scenes[sid];
-  const uint32_t i = ws.x + (threadIdx.x & 63);
+  const int lane = threadIdx.x & 63;
+  const uint32_t i = ws.x + uint32_t(lane);
   const bool valid = int64_t(i) < S.p_off + S.p_cnt;
-  float Gm[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float x = 0.f, y = 0.f, z = 0.f, v0 = 0.f;
   if (valid) {
-    const float v0 = ld_plane(pd + size_t(IP_C + 3) * n, i);
-    const float x = ld_plane(pd + size_t(IP_X) * n, i), y = ld_plane(pd + size_t(IP_X + 1) * n, i),
-                z = ld_plane(pd + size_t(IP_X + 2) * n, i);
-    const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
-    if (v0 != 0.0f && in_stencil_range(S, ux, uy, uz)) {
-      float wx[4], wy[4], wz[4], qx[4], qy[4], qz[4];
-      const int bx = axis_weights(ux, wx, qx), by = axis_weights(uy, wy, qy), bz = axis_weights(uz, wz, qz);
-      float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // column c = sum v dw/dx_c
-      for (int oz = 0; oz < 4; ++oz) {
-        const int iz = bz + oz - 1;
-        if (iz < 0 || iz >= S.D[2]) continue;
-        for (int oy = 0; oy < 4; ++oy) {
-          const int iy = by + oy - 1;
-          if (iy < 0 || iy >= S.D[1]) continue;
-          const float wyz = wy[oy] * wz[oz], qyz = qy[oy] * wz[oz], wqz = wy[oy] * qz[oz];
-          const int64_t row = S.node_off + int64_t(S.D[0]) * (iy + int64_t(S.D[1]) * iz);
+    v0 = ld_plane(pd + size_t(IP_C + 3) * n, i);
+    x = ld_plane(pd + size_t(IP_X) * n, i);
+    y = ld_plane(pd + size_t(IP_X + 1) * n, i);
+    z = ld_plane(pd + size_t(IP_X + 2) * n, i);
+  }
+  const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
+  const bool inr = valid && in_stencil_range(S, ux, uy, uz);
+  const bool work = inr && v0 != 0.0f;
+  const int D0 = S.D[0], D1 = S.D[1], D2 = S.D[2];
+  float wx[4], wy[4], wz[4], qx[4], qy[4], qz[4];
+  const int bx = axis_weights(ux, wx, qx), by = axis_weights(uy, wy, qy), bz = axis_weights(uz, wz, qz);
+  const bool interior = inr && bx >= 1 && by >= 1 && bz >= 1 && bx + 2 < D0 && by + 2 < D1 && bz + 2 < D2;
+  float L[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // column c = sum v dw/dx_c
+  bool boxed = false;
+  if (__ballot(interior) == ~0ull) {
+    const int xmin = wave_min_i(bx) - 1, ymin = wave_min_i(by) - 1, zmin = wave_min_i(bz) - 1;
+    const int ex = wave_max_i(bx) + 3 - xmin, ey = wave_max_i(by) + 3 - ymin, ez = wave_max_i(bz) + 3 - zmin;
+    const int vol = ex * ey * ez;
+    if (vol <= IMP_BOX) {
+      boxed = true;
+      float4* box = s_box[threadIdx.x >> 6];
+      for (int idx = lane; idx < vol; idx += 64) {
+        const int ix = idx % ex, iy = (idx / ex) % ey, iz = idx / (ex * ey);
+        box[idx] = vin[S.node_off + (xmin + ix) + int64_t(D0) * ((ymin + iy) + int64_t(D1) * (zmin + iz))];
+      }
+      __builtin_amdgcn_wave_barrier();  // the wave's own LDS writes complete before its reads
+      if (work) {
+        const float4* p0 = box + ((bx - 1 - xmin) + ex * ((by - 1 - ymin) + ey * (bz - 1 - zmin)));
+        for (int oz = 0; oz < 4; ++oz)
 #pragma unroll
-          for (int ox = 0; ox < 4; ++ox) {
-            const int ix = bx + ox - 1;
-            if (ix < 0 || ix >= S.D[0]) continue;
-            const float4 v = vin[row + ix];
-            const float g0 = qx[ox] * wyz, g1 = wx[ox] * qyz, g2 = wx[ox] * wqz;
-            L[0] = fmaf(v.x, g0, L[0]); L[1] = fmaf(v.y, g0, L[1]); L[2] = fmaf(v.z, g0, L[2]);
-            L[3] = fmaf(v.x, g1, L[3]); L[4] = fmaf(v.y, g1, L[4]); L[5] = fmaf(v.z, g1, L[5]);
-            L[6] = fmaf(v.x, g2, L[6]); L[7] = fmaf(v.y, g2, L[7]); L[8] = fmaf(v.z, g2, L[8]);
+          for (int oy = 0; oy < 4; ++oy) {
+            const float wyz = wy[oy] * wz[oz], qyz = qy[oy] * wz[oz], wqz = wy[oy] * qz[oz];
+            const float4* row = p0 + ex * (oy + ey * oz);
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox) {
+              const float4 v = row[ox];
+              const float g0 = qx[ox] * wyz, g1 = wx[ox] * qyz, g2 = wx[ox] * wqz;
+              L[0] = fmaf(v.x, g0, L[0]); L[1] = fmaf(v.y, g0, L[1]); L[2] = fmaf(v.z, g0, L[2]);
+              L[3] = fmaf(v.x, g1, L[3]); L[4] = fmaf(v.y, g1, L[4]); L[5] = fmaf(v.z, g1, L[5]);
+              L[6] = fmaf(v.x, g2, L[6]); L[7] = fmaf(v.y, g2, L[7]); L[8] = fmaf(v.z, g2, L[8]);
+            }
           }
+      }
+    }
+  }
+  if (!boxed && work) {
+    for (int oz = 0; oz < 4; ++oz) {
+      const int iz = bz + oz - 1;
+      if (iz < 0 || iz >= D2) continue;
+      for (int oy = 0; oy < 4; ++oy) {
+        const int iy = by + oy - 1;
+        if (iy < 0 || iy >= D1) continue;
+        const float wyz = wy[oy] * wz[oz], qyz = qy[oy] * wz[oz], wqz = wy[oy] * qz[oz];
+        const int64_t row = S.node_off + int64_t(D0) * (iy + int64_t(D1) * iz);
+#pragma unroll
+        for (int ox = 0; ox < 4; ++ox) {
+          const int ix = bx + ox - 1;
+          if (ix < 0 || ix >= D0) continue;
+          const float4 v = vin[row + ix];
+          const float g0 = qx[ox] * wyz, g1 = wx[ox] * qyz, g2 = wx[ox] * wqz;
+          L[0] = fmaf(v.x, g0, L[0]); L[1] = fmaf(v.y, g0, L[1]); L[2] = fmaf(v.z, g0, L[2]);
+          L[3] = fmaf(v.x, g1, L[3]); L[4] = fmaf(v.y, g1, L[4]); L[5] = fmaf(v.z, g1, L[5]);
+          L[6] = fmaf(v.x, g2, L[6]); L[7] = fmaf(v.y, g2, L[7]); L[8] = fmaf(v.z, g2, L[8]);
         }
       }
-      const float k = scal[sid].dt * S.gscale;
-#pragma unroll
-      for (int e = 0; e < 9; ++e) L[e] *= k;
-      float F[9], Rot[9], C[9], dF[9], D[9];
-#pragma unroll
-      for (int e = 0; e < 9; ++e) {
-        F[e] = ld_plane(pd + size_t(IP_F + e) * n, i);
-        Rot[e] = ld_plane(pd + size_t(IP_R + e) * n, i);
-        C[e] = ld_plane(pd + size_t(IP_COF + e) * n, i);
-      }
-      mul(L, F, dF);  // dF = dt L F
-      mul_at(Rot, dF, D);  // R^T dF
-      // axial vector of D - D^T: (D(2,1) - D(1,2), D(0,2) - D(2,0), D(1,0) - D(0,1)), D(r,c) = D[c*3+r]
-      const float a0 = D[5] - D[7], a1 = D[6] - D[2], a2 = D[1] - D[3];
-      const float m00 = ld_plane(pd + size_t(IP_MI) * n, i), m11 = ld_plane(pd + size_t(IP_MI + 1) * n, i),
-                  m22 = ld_plane(pd + size_t(IP_MI + 2) * n, i), m01 = ld_plane(pd + size_t(IP_MI + 3) * n, i),
-                  m02 = ld_plane(pd + size_t(IP_MI + 4) * n, i), m12 = ld_plane(pd + size_t(IP_MI + 5) * n, i);
-      const float w0 = m00 * a0 + m01 * a1 + m02 * a2;
-      const float w1 = m01 * a0 + m11 * a1 + m12 * a2;
-      const float w2 = m02 * a0 + m12 * a1 + m22 * a2;
-      // [w]x column-major: (1,0) = w2, (2,0) = -w1, (0,1) = -w2, (2,1) = w0, (0,2) = w1, (1,2) = -w0
-      const float Om[9] = {0.f, w2, -w1, -w2, 0.f, w0, w1, -w0, 0.f};
-      float dR[9];
-      mul(Rot, Om, dR);
-      float dJ = 0.f;
-#pragma unroll
-      for (int e = 0; e < 9; ++e) dJ = fmaf(C[e], dF[e], dJ);
-      float dC[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      cross_add(dF + 3, F + 6, dC);
-      cross_add(F + 3, dF + 6, dC);
-      cross_add(dF + 6, F, dC + 3);
-      cross_add(F + 6, dF, dC + 3);
-      cross_add(dF, F + 3, dC + 6);
-      cross_add(F, dF + 3, dC + 6);
-      const float mu2 = ld_plane(pd + size_t(IP_C) * n, i), la = ld_plane(pd + size_t(IP_C + 1) * n, i),
-                  laj = ld_plane(pd + size_t(IP_C + 2) * n, i);
-      float A[9];
-#pragma unroll
-      for (int e = 0; e < 9; ++e) A[e] = mu2 * (dF[e] - dR[e]) + (la * dJ) * C[e] + laj * dC[e];
-      mul_bt(A, F, Gm);  // A F^T
-#pragma unroll
-      for (int e = 0; e < 9; ++e) Gm[e] *= v0;
     }
+  }
+  float Gm[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (work) {
+    const float k = scal[sid].dt * S.gscale;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) L[e] *= k;
+    float F[9], Rot[9], C[9], dF[9], D[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      F[e] = ld_plane(pd + size_t(IP_F + e) * n, i);
+      Rot[e] = ld_plane(pd + size_t(IP_R + e) * n, i);
+      C[e] = ld_plane(pd + size_t(IP_COF + e) * n, i);
+    }
+    mul(L, F, dF);       // dF = dt L F
+    mul_at(Rot, dF, D);  // R^T dF
+    // axial vector of D - D^T: (D(2,1) - D(1,2), D(0,2) - D(2,0), D(1,0) - D(0,1)), D(r,c) = D[c*3+r]
+    const float a0 = D[5] - D[7], a1 = D[6] - D[2], a2 = D[1] - D[3];
+    const float m00 = ld_plane(pd + size_t(IP_MI) * n, i), m11 = ld_plane(pd + size_t(IP_MI + 1) * n, i),
+                m22 = ld_plane(pd + size_t(IP_MI + 2) * n, i), m01 = ld_plane(pd + size_t(IP_MI + 3) * n, i),
+                m02 = ld_plane(pd + size_t(IP_MI + 4) * n, i), m12 = ld_plane(pd + size_t(IP_MI + 5) * n, i);
+    const float w0 = m00 * a0 + m01 * a1 + m02 * a2;
+    const float w1 = m01 * a0 + m11 * a1 + m12 * a2;
+    const float w2 = m02 * a0 + m12 * a1 + m22 * a2;
+    // [w]x column-major: (1,0) = w2, (2,0) = -w1, (0,1) = -w2, (2,1) = w0, (0,2) = w1, (1,2) = -w0
+    const float Om[9] = {0.f, w2, -w1, -w2, 0.f, w0, w1, -w0, 0.f};
+    float dR[9];
+    mul(Rot, Om, dR);
+    float dJ = 0.f;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) dJ = fmaf(C[e], dF[e], dJ);
+    float dC[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    cross_add(dF + 3, F + 6, dC);
+    cross_add(F + 3, dF + 6, dC);
+    cross_add(dF + 6, F, dC + 3);
+    cross_add(F + 6, dF, dC + 3);
+    cross_add(dF, F + 3, dC + 6);
+    cross_add(F, dF + 3, dC + 6);
+    const float mu2 = ld_plane(pd + size_t(IP_C) * n, i), la = ld_plane(pd + size_t(IP_C + 1) * n, i),
+                laj = ld_plane(pd + size_t(IP_C + 2) * n, i);
+    float A[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) A[e] = mu2 * (dF[e] - dR[e]) + (la * dJ) * C[e] + laj * dC[e];
+    mul_bt(A, F, Gm);  // A F^T
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Gm[e] *= v0;
+  }
+  if (valid) {
 #pragma unroll
     for (int e = 0; e < 9; ++e) st_plane(G + size_t(e) * n, i, Gm[e]);
   }
@@ -271,89 +327,149 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
 #pragma unroll
   for (int r = 0; r < 3; ++r) rs = fmaxf(rs, fabsf(Gm[r]) + fabsf(Gm[3 + r]) + fabsf(Gm[6 + r]));
   const int mx = wave_max_i(int(__float_as_uint(rs)));
-  if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&scal[sid].gmax, uint32_t(mx));
+  if (lane == 0 && mx > 0) atomicMax(&gslots[sid * IMP_GSLOTS + int(w % IMP_GSLOTS)], uint32_t(mx));
 }
 
 // ---------------------------------------------------------------------------------------
 // Second half: acc_i += sum_p G_p grad w_ip (unscaled grad w: G holds gs) in fixed point. A wave
-// takes 64 sorted positions of one scene: lane k stages particle k's weights and G in LDS, then
-// lane l is the stencil node getIJK(l) and sums the staged particles in registers while the base
-// cell stays the same, adding the rounded sum to the node with integer atomics at each change.
+// takes 64 sorted positions of one scene in two batches of 32: lane k stages particle k's weights
+// and G in LDS, then lane l is the stencil node getIJK(l) and sums the staged particles in
+// registers while the base cell stays the same, adding the rounded sum at each change. When the
+// workgroup's waves are of one scene and their stencils cover at most IMP_ACC_BOX nodes, the sums
+// go into an LDS box first (integer LDS atomics) and the box is added to acc once; otherwise
+// straight to acc.
 __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict__ scenes,
                                                      const uint2* __restrict__ waves, uint32_t nwaves,
                                                      const float* __restrict__ pd, const float* __restrict__ G,
                                                      uint32_t n, unsigned long long* __restrict__ acc,
-                                                     const ImpScal* __restrict__ scal) {
-  __shared__ __attribute__((aligned(16))) float s_stage[4][64 * IMP_STAGE];
-  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w >= nwaves) return;
-  const uint2 ws = waves ? waves[w] : make_uint2(64u * w, 0u);
+                                                     const ImpScal* __restrict__ scal,
+                                                     const uint32_t* __restrict__ gslots) {
+  __shared__ __attribute__((aligned(16))) float s_stage[4][32 * IMP_STAGE];
+  __shared__ unsigned long long s_acc[3 * IMP_ACC_BOX];
+  __shared__ int s_ext[6];  // box min (x, y, z), max (x, y, z) of the base cells + 2
+  const int t = threadIdx.x, lane = t & 63;
+  const uint32_t w0 = blockIdx.x * 4u, w = w0 + uint32_t(t >> 6);
+  const uint32_t wlast = min(w0 + 3u, nwaves - 1u);
+  const int sid0 = waves ? int(waves[w0].y) : 0;
+  // one scene for the whole workgroup (the scene of a wave is constant along the table)
+  const bool one_scene = !waves || int(waves[wlast].y) == sid0;
+  const bool live_wave = w < nwaves;
+  const uint2 ws = live_wave ? (waves ? waves[w] : make_uint2(64u * w, 0u)) : make_uint2(0u, uint32_t(sid0));
   const int sid = int(ws.y);
-  if (scal[sid].done) return;
   const SceneDev& S = scenes[sid];
-  const int lane = threadIdx.x & 63;
-  const int64_t end = S.p_off + S.p_cnt;
-  const uint32_t cnt = uint32_t(min<int64_t>(64, end - int64_t(ws.x)));
-  float* st = s_stage[threadIdx.x >> 6];
-  const double scale = ldexp(1.0, imp_exponent(scal[sid].gmax, S.p_cnt));
+  const bool run = live_wave && !scal[sid].done;
+  if (t < 3) s_ext[t] = 0x7fffffff;
+  if (t >= 3 && t < 6) s_ext[t] = -1;
+  const uint32_t cnt = run ? uint32_t(min<int64_t>(64, S.p_off + S.p_cnt - int64_t(ws.x))) : 0u;
+  // this lane's particle (lane < cnt): grid coordinate and base cell + 2 (-1: no in-grid node)
+  float ux = 0.f, uy = 0.f, uz = 0.f;
+  int cell = -1;
   if (uint32_t(lane) < cnt) {
     const uint32_t i = ws.x + uint32_t(lane);
-    const float x = ld_plane(pd + size_t(IP_X) * n, i), y = ld_plane(pd + size_t(IP_X + 1) * n, i),
-                z = ld_plane(pd + size_t(IP_X + 2) * n, i);
-    const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
-    float* d = st + lane * IMP_STAGE;
-    int cell = -1;
-    if (in_stencil_range(S, ux, uy, uz)) {
-      const float fx = floorf(ux), fy = floorf(uy), fz = floorf(uz);
-      const float f[3] = {ux - fx, uy - fy, uz - fz};
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int o = 0; o < 4; ++o) {
-          d[8 * k + 2 * o] = spline_w(o, f[k]);
-          d[8 * k + 2 * o + 1] = spline_q(o, f[k]);
-        }
-      // base cell in [-2, D] per axis, packed 10 bits per axis (IMP_MAX_DIM), non-negative
-      cell = (int(fx) + 2) | ((int(fy) + 2) << 10) | ((int(fz) + 2) << 20);
-    }
-#pragma unroll
-    for (int e = 0; e < 9; ++e) d[24 + e] = ld_plane(G + size_t(e) * n, i);
-    d[33] = __int_as_float(cell);
+    ux = grid_u(S, ld_plane(pd + size_t(IP_X) * n, i), 0);
+    uy = grid_u(S, ld_plane(pd + size_t(IP_X + 1) * n, i), 1);
+    uz = grid_u(S, ld_plane(pd + size_t(IP_X + 2) * n, i), 2);
+    if (in_stencil_range(S, ux, uy, uz))  // base cell in [-2, D] per axis, + 2, 10 bits per axis
+      cell = (int(floorf(ux)) + 2) | ((int(floorf(uy)) + 2) << 10) | ((int(floorf(uz)) + 2) << 20);
   }
-  __builtin_amdgcn_wave_barrier();  // the wave's own LDS writes complete before its reads
+  __syncthreads();
+  if (one_scene && cell >= 0) {
+    const int cx = cell & 1023, cy = (cell >> 10) & 1023, cz = cell >> 20;
+    atomicMin(&s_ext[0], cx); atomicMin(&s_ext[1], cy); atomicMin(&s_ext[2], cz);
+    atomicMax(&s_ext[3], cx); atomicMax(&s_ext[4], cy); atomicMax(&s_ext[5], cz);
+  }
+  __syncthreads();
+  // the accumulation box: nodes (c - 2) - 1 .. (c - 2) + 2 of the base cells, clipped to the grid
+  const SceneDev& S0 = scenes[sid0];
+  int bx0 = 0, by0 = 0, bz0 = 0, ex = 0, ey = 0, ez = 0;
+  bool boxed = false;
+  if (one_scene && s_ext[3] >= 0) {
+    bx0 = max(0, s_ext[0] - 3); by0 = max(0, s_ext[1] - 3); bz0 = max(0, s_ext[2] - 3);
+    ex = min(S0.D[0], s_ext[3] + 1) - bx0; ey = min(S0.D[1], s_ext[4] + 1) - by0; ez = min(S0.D[2], s_ext[5] + 1) - bz0;
+    boxed = ex > 0 && ey > 0 && ez > 0 && ex * ey * ez <= IMP_ACC_BOX;
+  }
+  if (boxed)
+    for (int k = t; k < 3 * ex * ey * ez; k += 256) s_acc[k] = 0ull;
+  __syncthreads();
+  const double scale = run ? ldexp(1.0, imp_exponent(imp_gmax(gslots, sid), S.p_cnt)) : 0.0;
   const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
   int cur = -1;
   float s0 = 0.f, s1 = 0.f, s2 = 0.f;
   auto commit = [&]() {
     const int nx = (cur & 1023) - 3 + ox, ny = ((cur >> 10) & 1023) - 3 + oy, nz = (cur >> 20) - 3 + oz;
     if (nx >= 0 && ny >= 0 && nz >= 0 && nx < S.D[0] && ny < S.D[1] && nz < S.D[2]) {
-      const int64_t node = S.node_off + nx + int64_t(S.D[0]) * (ny + int64_t(S.D[1]) * nz);
       const long long q0 = llrint(double(s0) * scale), q1 = llrint(double(s1) * scale), q2 = llrint(double(s2) * scale);
-      if (q0) atomicAdd(acc + 3 * node, (unsigned long long)q0);
-      if (q1) atomicAdd(acc + 3 * node + 1, (unsigned long long)q1);
-      if (q2) atomicAdd(acc + 3 * node + 2, (unsigned long long)q2);
+      if (boxed) {
+        unsigned long long* a = s_acc + 3 * ((nx - bx0) + ex * ((ny - by0) + ey * (nz - bz0)));
+        if (q0) atomicAdd(a, (unsigned long long)q0);
+        if (q1) atomicAdd(a + 1, (unsigned long long)q1);
+        if (q2) atomicAdd(a + 2, (unsigned long long)q2);
+      } else {
+        const int64_t node = S.node_off + nx + int64_t(S.D[0]) * (ny + int64_t(S.D[1]) * nz);
+        if (q0) atomicAdd(acc + 3 * node, (unsigned long long)q0);
+        if (q1) atomicAdd(acc + 3 * node + 1, (unsigned long long)q1);
+        if (q2) atomicAdd(acc + 3 * node + 2, (unsigned long long)q2);
+      }
     }
     s0 = s1 = s2 = 0.f;
   };
-  for (uint32_t k = 0; k < cnt; ++k) {
-    const float* d = st + k * IMP_STAGE;
-    const int cell = __float_as_int(d[33]);
-    if (cell != cur) {
-      if (cur >= 0) commit();
-      cur = cell;
+  if (run) {
+    float* st = s_stage[t >> 6];
+    for (uint32_t b0 = 0; b0 < cnt; b0 += 32u) {
+      const uint32_t bc = min(32u, cnt - b0);
+      // lanes b0 .. b0 + bc - 1 stage their particle
+      if (uint32_t(lane) >= b0 && uint32_t(lane) < b0 + bc) {
+        float* d = st + (lane - int(b0)) * IMP_STAGE;
+        if (cell >= 0) {
+          const float f[3] = {ux - floorf(ux), uy - floorf(uy), uz - floorf(uz)};
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+              d[8 * k + 2 * o] = spline_w(o, f[k]);
+              d[8 * k + 2 * o + 1] = spline_q(o, f[k]);
+            }
+        }
+        const uint32_t i = ws.x + uint32_t(lane);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) d[24 + e] = ld_plane(G + size_t(e) * n, i);
+        d[33] = __int_as_float(cell);
+      }
+      __builtin_amdgcn_wave_barrier();  // the wave's own staging writes complete before its reads
+      for (uint32_t k = 0; k < bc; ++k) {
+        const float* d = st + k * IMP_STAGE;
+        const int ck = __float_as_int(d[33]);
+        if (ck != cur) {
+          if (cur >= 0) commit();
+          cur = ck;
+        }
+        if (ck < 0) continue;
+        const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
+        const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
+        const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
+        const float wyz = wqy.x * wqz.x;
+        const float g0 = wqx.y * wyz, g1 = wqy.y * (wqx.x * wqz.x), g2 = wqz.y * (wqx.x * wqy.x);
+        // (G grad w)_r = G(r,0) g0 + G(r,1) g1 + G(r,2) g2, G(r,c) = d[24 + c*3 + r]
+        s0 = fmaf(d[24], g0, fmaf(d[27], g1, fmaf(d[30], g2, s0)));
+        s1 = fmaf(d[25], g0, fmaf(d[28], g1, fmaf(d[31], g2, s1)));
+        s2 = fmaf(d[26], g0, fmaf(d[29], g1, fmaf(d[32], g2, s2)));
+      }
+      __builtin_amdgcn_wave_barrier();  // all reads of the batch done before the next one is staged
     }
-    if (cell < 0) continue;
-    const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
-    const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
-    const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
-    const float wyz = wqy.x * wqz.x;
-    const float g0 = wqx.y * wyz, g1 = wqy.y * (wqx.x * wqz.x), g2 = wqz.y * (wqx.x * wqy.x);
-    // (G grad w)_r = G(r,0) g0 + G(r,1) g1 + G(r,2) g2, G(r,c) = d[24 + c*3 + r]
-    s0 = fmaf(d[24], g0, fmaf(d[27], g1, fmaf(d[30], g2, s0)));
-    s1 = fmaf(d[25], g0, fmaf(d[28], g1, fmaf(d[31], g2, s1)));
-    s2 = fmaf(d[26], g0, fmaf(d[29], g1, fmaf(d[32], g2, s2)));
+    if (cur >= 0) commit();
   }
-  if (cur >= 0) commit();
+  if (!boxed) return;
+  __syncthreads();
+  const int vol = ex * ey * ez;
+  for (int k = t; k < vol; k += 256) {
+    const int ix = k % ex, iy = (k / ex) % ey, iz = k / (ex * ey);
+    const int64_t node = S0.node_off + (bx0 + ix) + int64_t(S0.D[0]) * ((by0 + iy) + int64_t(S0.D[1]) * (bz0 + iz));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const unsigned long long v = s_acc[3 * k + q];
+      if (v) atomicAdd(acc + 3 * node + q, v);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -387,7 +503,7 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
                                                           const float* __restrict__ mass, float4* __restrict__ vec,
                                                           long long* __restrict__ acc,
                                                           const ImpScal* __restrict__ scal, int64_t total_nodes,
-                                                          double* __restrict__ part) {
+                                                          double* __restrict__ part, const uint32_t* __restrict__ gslots) {
   const ImpBlock B = blocks[blockIdx.x];
   const ImpScal& Z = scal[B.scene];
   if (PHASE != IMP_INIT && Z.done) return;  // the scene's partials are not read again
@@ -398,7 +514,7 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
   float4* P = vec + 2 * total_nodes;
   float4* KP = vec + 3 * total_nodes;
   float4* KR = vec + 4 * total_nodes;
-  const double inv = (PHASE == IMP_INIT || PHASE == IMP_KR) ? ldexp(1.0, -imp_exponent(Z.gmax, S.p_cnt)) : 0.0;
+  const double inv = (PHASE == IMP_INIT || PHASE == IMP_KR) ? ldexp(1.0, -imp_exponent(imp_gmax(gslots, B.scene), S.p_cnt)) : 0.0;
   const float bdt = Z.beta_dt;
   const float al = float(Z.alpha), be = float(Z.beta);
   double sum[3] = {0.0, 0.0, 0.0};
@@ -407,6 +523,9 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
     const int64_t nd = B.first + j * IMP_THREADS + threadIdx.x;
     if (nd >= end) break;
     const float m = mass[nd];
+    // vectors are 0 where m <= 0 (set so by IMP_INIT) and the accumulator too (no particle with
+    // mass has a weight gradient there), so the later phases touch only nodes with m > 0
+    if (PHASE != IMP_INIT && !(m > 0.0f)) continue;
     if (PHASE == IMP_INIT || PHASE == IMP_KR) {
       // df = -acc 2^-e; K v = m v - beta dt df = m v + beta dt acc 2^-e
       const float a0 = float(double(acc[3 * nd]) * inv), a1 = float(double(acc[3 * nd + 1]) * inv),
@@ -429,9 +548,8 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
         sum[0] += double(b.x) * b.x + double(b.y) * b.y + double(b.z) * b.z;
         sum[1] += double(r.x) * r.x + double(r.y) * r.y + double(r.z) * r.z;
       } else {
-        float4 kr = make_float4(0.f, 0.f, 0.f, 0.f);
         const float4 r = Rv[nd];
-        if (m > 0.0f) kr = make_float4(fmaf(bdt, a0, m * r.x), fmaf(bdt, a1, m * r.y), fmaf(bdt, a2, m * r.z), 0.f);
+        const float4 kr = make_float4(fmaf(bdt, a0, m * r.x), fmaf(bdt, a1, m * r.y), fmaf(bdt, a2, m * r.z), 0.f);
         KR[nd] = kr;
         sum[0] += double(r.x) * kr.x + double(r.y) * kr.y + double(r.z) * kr.z;
       }
@@ -458,15 +576,19 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
 __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const uint32_t* __restrict__ scene_blocks,
                                                              uint32_t nblocks, const double* __restrict__ part,
                                                              ImpScal* __restrict__ scal, const SceneDev* __restrict__ scenes,
-                                                             float dt_override, float beta, float tol, int max_iter) {
+                                                             float dt_override, float beta, float tol, int max_iter,
+                                                             uint32_t* __restrict__ gslots) {
   const int sid = blockIdx.x;
   ImpScal& Z = scal[sid];
+  // the bound slots of the next application start at 0 (after the start, and after each phase
+  // that consumed them)
+  if ((phase < 0 || phase == IMP_INIT || phase == IMP_KR) && threadIdx.x < IMP_GSLOTS)
+    gslots[sid * IMP_GSLOTS + threadIdx.x] = 0u;
   if (phase < 0) {
     if (threadIdx.x == 0) {
       const float dt = dt_override > 0.0f ? dt_override : scenes[sid].dt;
       Z.rho = Z.alpha = Z.beta = Z.bb = Z.rr = 0.0;
       Z.tol2 = double(tol) * double(tol);
-      Z.gmax = 0u;
       Z.iters = 0;
       Z.done = 0;
       Z.max_iter = max_iter;
@@ -495,12 +617,10 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const ui
   if (phase == IMP_INIT) {
     Z.bb = t0;
     Z.rr = t1;
-    Z.gmax = 0u;
     Z.done = (t1 <= Z.tol2 * t0 || Z.max_iter <= 0) ? 1 : 0;
   } else if (phase == IMP_KR) {
     Z.beta = Z.iters == 0 ? 0.0 : t0 / Z.rho;
     Z.rho = t0;
-    Z.gmax = 0u;
   } else if (phase == IMP_P) {
     if (t0 > 0.0) Z.alpha = Z.rho / t0;
     else Z.done = 1;  // K p = 0: no further progress
