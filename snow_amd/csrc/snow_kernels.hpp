@@ -1169,9 +1169,10 @@ __device__ __forceinline__ bool g2p_gather(const SceneDev& S, float x, float y, 
   } else if (interior) {
     const float2* p0 = g_node + 3 * (S.node_off + (bx - 1) + int64_t(D0) * ((by - 1) + int64_t(D1) * (bz - 1)));
     const int64_t sy = 3 * int64_t(D0), sz = 3 * int64_t(D0) * D1;
+    // two rows in flight (a fully unrolled row loop holds 4 more VGPRs than the fused kernel's 128)
 #pragma unroll 1
     for (int oz = 0; oz < 4; ++oz)
-#pragma unroll
+#pragma unroll 2
       for (int oy = 0; oy < 4; ++oy)
         g2p_row(p0 + oz * sz + oy * sy, wx, qx, wy[oy] * wz[oz], qy[oy] * wz[oz], wy[oy] * qz[oz], vpn, dvp, 0, 3);
   } else {
