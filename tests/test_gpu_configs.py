@@ -1,7 +1,8 @@
 """The benchmark configurations on the GPU, checked against the FP64 CPU oracle.
 
-* two snowballs, 2^20 particles on 128^3, dt 5e-5: 100 free-running steps, compared with the
-  oracle after steps 1, 10 and 100 (positions within 1e-4 of the domain size at step 100);
+* two snowballs, 2^20 particles on 128^3, dt 5e-5: 100 free-running steps compared with the oracle
+  after every step (positions within 1e-4 of the domain size at step 100), and the configuration's
+  1000 steps by properties;
 * the slab impact on 256^3 (box collider, sphere impactor, physical weight gradient), plain and
   mixed precision: 2^20 particles against the oracle, and the full 2^24 particles by properties;
 * the 64-scene parameter sweep at full size (graph steps, autotuned launch shapes): four scenes
@@ -53,43 +54,60 @@ def rms(p, o, k):
 
 
 def test_two_snowballs_1m_100_steps(oracle_mod):
-    """The 2^20-particle headline scene stepped 100 times, compared with the FP64 oracle after steps
-    1, 10 and 100 (explicittimeupdate.cpp:36-63 per step)."""
+    """The 2^20-particle headline scene stepped 100 times, compared with the FP64 oracle after every
+    step (explicittimeupdate.cpp:36-63 per step): GPU and oracle advance in lockstep and the GPU state
+    is read back after each step."""
     sc = scenes.two_snowballs()
-    checkpoints = (1, 10, 100)
-    gpu = {}
-    log(f"two snowballs: {len(sc.particles)} particles, 128^3, GPU 100 steps")
-    with SnowContext(sc, FLAG_KEEP_GRID) as ctx:
-        ctx.init()
-        done = 0
-        for c in checkpoints:
-            for _ in range(c - done):
-                ctx.step(sc.dt)
-            done = c
-            gpu[c] = (ctx.read_particles(), ctx.read_grid(fields=("m", "mv")))
-        ctx.check_finite()
-    o = oracle_mod.Oracle(sc, nthreads=THREADS)
-    o.init()
-    done = 0
-    # Bounds per checkpoint: x (domain size 1), v (relative), F_E, F_P (Frobenius). The node fields
-    # follow the particles: a position error dx moves a node's share of a particle's mass by about
-    # |dN/du| dx / h, so the grid bounds (relative to the largest node value) are 100 x_bound / h.
+    steps = 100
     h = sc.grid.h
+    # Bounds at steps 1, 10 and 100: x (domain size 1), v (relative), F_E, F_P (Frobenius); between
+    # them the bound of the next checkpoint applies. The node fields follow the particles: a position
+    # error dx moves a node's share of a particle's mass by about |dN/du| dx / h, so the grid bounds
+    # (relative to the largest node value) are 100 x_err / h.
     bounds = {1: dict(x=1e-6, v=1e-4, fe=1e-4, fp=1e-4), 10: dict(x=1e-5, v=1e-3, fe=1e-3, fp=1e-3),
               100: dict(x=1e-4, v=2e-3, fe=2e-3, fp=2e-3)}
-    errs = {}
-    for c in checkpoints:
-        t0 = time.time()
-        o.step(sc.dt, c - done)
-        done = c
-        errs[c] = errors(gpu[c][0], gpu[c][1], o)
-        log(f"step {c} (oracle {time.time() - t0:.1f} s): " + " ".join(f"{k}={v:.3e}" for k, v in errs[c].items()))
-    for c in checkpoints:
-        e, b = errs[c], dict(bounds[c])
-        b["grid_m"] = b["grid_mv"] = 100 * e["x"] / h + 1e-6
-        for k, lim in b.items():
-            assert e[k] < lim, (c, k, e[k], lim)
-    assert errs[100]["x"] < 1e-4  # positions within 1e-4 of the domain size after 100 steps
+    log(f"two snowballs: {len(sc.particles)} particles, 128^3, {steps} steps in lockstep with the oracle")
+    o = oracle_mod.Oracle(sc, nthreads=THREADS)
+    o.init()
+    worst = {}
+    with SnowContext(sc, FLAG_KEEP_GRID) as ctx:
+        ctx.init()
+        for s in range(1, steps + 1):
+            ctx.step(sc.dt)
+            o.step(sc.dt, 1)
+            e = errors(ctx.read_particles(), ctx.read_grid(fields=("m", "mv")), o)
+            b = dict(bounds[next(c for c in (1, 10, 100) if s <= c)])
+            b["grid_m"] = b["grid_mv"] = 100 * e["x"] / h + 1e-6
+            for k, lim in b.items():
+                assert e[k] < lim, (s, k, e[k], lim)
+                worst[k] = max(worst.get(k, 0.0), e[k])
+            if s in (1, 10, 100):
+                log(f"step {s}: " + " ".join(f"{k}={v:.3e}" for k, v in e.items()))
+        ctx.check_finite()
+    log("largest over the 100 steps: " + " ".join(f"{k}={v:.3e}" for k, v in worst.items()))
+    assert e["x"] < 1e-4  # positions within 1e-4 of the domain size after 100 steps
+
+
+def test_two_snowballs_1m_1000_steps_properties():
+    """The configuration's 1000 steps (the balls meet and spread on the ground): finite state, every
+    particle inside the grid, the particles' mass all on the grid (P2G conserves it), and the mean
+    velocity only changed by gravity and the ground."""
+    sc = scenes.two_snowballs()
+    m = np.asarray(sc.particles.mass, np.float64)
+    with SnowContext(sc, FLAG_KEEP_GRID) as ctx:
+        ctx.init()
+        for s in range(1000):
+            ctx.step(sc.dt)
+        ctx.check_finite()
+        p = ctx.read_particles()
+        g = ctx.read_grid(fields=("m",))
+    x = p["x"]
+    assert np.all((x > 0.0) & (x < 1.0))
+    assert np.all(x[:, 1] > 0.05 - 2 * sc.grid.h)  # on or above the ground plane
+    assert abs(g["m"].sum() - m.sum()) <= 1e-5 * m.sum()
+    mom = (m[:, None] * p["v"]).sum(0) / m.sum()
+    log(f"1000 steps: mean velocity {mom}, grid mass / particle mass - 1 = {g['m'].sum() / m.sum() - 1:.2e}")
+    assert abs(mom[0]) < 0.05 and abs(mom[2]) < 0.05  # the two balls' +-3 m/s cancel along x
 
 
 def _slab_forms(oracle_mod, steps):
