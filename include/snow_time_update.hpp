@@ -90,46 +90,36 @@ class SnowExplicitTimeUpdate : public TimeUpdate {
 
   snow_ctx* handle() { return ctx_; }
 
- private:
+ protected:
   void check(int rc) { check(rc, ctx_); }
   static void check(int rc, const snow_ctx* c) {
     if (rc != SNOW_OK) throw std::runtime_error("snow: " + std::string(snow_last_error(c)));
   }
+
+ private:
   snow_ctx* ctx_ = nullptr;
 };
 
 // SemiImplicitTimeUpdate (src/snow/simulation/timeUpdate/semiimplicittimeupdate.hpp:6-13, whose
 // init/update are empty in the reference, semiimplicittimeupdate.cpp:3-5): the explicit step with
 // the semi-implicit grid velocity solve of snow_step_semi_implicit (Stomakhin et al. 2013, eq. 9).
-class SnowSemiImplicitTimeUpdate : public TimeUpdate {
+// init() and the state readback are the explicit class's.
+class SnowSemiImplicitTimeUpdate : public SnowExplicitTimeUpdate {
  public:
   explicit SnowSemiImplicitTimeUpdate(const snow_scene_desc& scene, double beta = 1.0, double tol = 1e-5,
                                       int32_t max_iter = 100, uint32_t flags = 0)
-      : beta_(beta), tol_(tol), max_iter_(max_iter) {
-    if (snow_create(&scene, flags, &ctx_) != SNOW_OK)
-      throw std::runtime_error("snow: " + std::string(snow_last_error(nullptr)));
-  }
-  ~SnowSemiImplicitTimeUpdate() override {
-    if (ctx_) snow_destroy(ctx_);
-  }
-  SnowSemiImplicitTimeUpdate(const SnowSemiImplicitTimeUpdate&) = delete;
-  SnowSemiImplicitTimeUpdate& operator=(const SnowSemiImplicitTimeUpdate&) = delete;
+      : SnowExplicitTimeUpdate(scene, flags), beta_(beta), tol_(tol), max_iter_(max_iter) {}
 
-  void init() override { check(snow_init(ctx_)); }
-  void update(double dt) override { check(snow_step_semi_implicit(ctx_, dt, beta_, tol_, max_iter_)); }
-  // iterations and relative residual of the last solve
-  void stats(int32_t* iterations, double* rel_residual) {
-    check(snow_implicit_stats(ctx_, 0, iterations, rel_residual));
+  // SemiImplicitTimeUpdate::update(dt): one step of every scene with the semi-implicit solve.
+  void update(double dt) override { check(snow_step_semi_implicit(handle(), dt, beta_, tol_, max_iter_)); }
+  // iterations and relative residual of a scene's last solve
+  void stats(int32_t scene, int32_t* iterations, double* rel_residual) {
+    check(snow_implicit_stats(handle(), scene, iterations, rel_residual));
   }
-  snow_ctx* handle() { return ctx_; }
 
  private:
-  void check(int rc) {
-    if (rc != SNOW_OK) throw std::runtime_error("snow: " + std::string(snow_last_error(ctx_)));
-  }
   double beta_, tol_;
   int32_t max_iter_;
-  snow_ctx* ctx_ = nullptr;
 };
 
 }  // namespace snow_amd

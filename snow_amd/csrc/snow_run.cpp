@@ -2,10 +2,12 @@
 // (include/snow_time_update.hpp), the way the reference's launchSnow drives ExplicitTimeUpdate
 // (src/snow/utils/launchHelper/explicit.cpp:2-116), without a window or renderer.
 //
-//   snow_run SCENE_FILE STEPS OUT_FILE [flags] [--ply PREFIX [--every K]]
+//   snow_run SCENE_FILE STEPS OUT_FILE [flags] [--ply PREFIX [--every K]] [--semi-implicit BETA]
 //   snow_run --scene NAME [--asset FILE] STEPS OUT_FILE [flags] [--ply PREFIX [--every K]]
 //   snow_run --scene NAME [--asset FILE] --dump-scene SCENE_FILE
 //
+// --semi-implicit BETA steps with SnowSemiImplicitTimeUpdate (the reference's SemiImplicitTimeUpdate,
+// empty there) at that beta, tolerance 1e-5 and at most 100 solver iterations per step.
 // --scene builds one of the reference's scene programs (src/exe/<NAME>.cpp, include/snow_scenes.hpp)
 // instead of reading a scene file; --dump-scene writes that scene as a scene file and exits.
 // --ply writes PREFIX_<step>.ply frames (x, y, z, mass) every K steps (default: the last step only).
@@ -17,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -88,7 +91,7 @@ bool write_scene_file(const char* path, const Scene& s) {
 
 int usage(const char* prog) {
   std::fprintf(stderr,
-               "usage: %s SCENE_FILE STEPS OUT_FILE [flags] [--ply PREFIX [--every K]]\n"
+               "usage: %s SCENE_FILE STEPS OUT_FILE [flags] [--ply PREFIX [--every K]] [--semi-implicit BETA]\n"
                "       %s --scene NAME [--asset FILE] STEPS OUT_FILE [flags] [--ply PREFIX [--every K]]\n"
                "       %s --scene NAME [--asset FILE] --dump-scene SCENE_FILE\n"
                "scenes:",
@@ -103,6 +106,7 @@ int usage(const char* prog) {
 int main(int argc, char** argv) {
   std::string scene_name, asset, dump, ply;
   long every = 0;
+  double beta = -1.0;  // < 0: the explicit step
   std::vector<const char*> pos;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -112,6 +116,7 @@ int main(int argc, char** argv) {
     else if (a == "--dump-scene" && has_value) dump = argv[++i];
     else if (a == "--ply" && has_value) ply = argv[++i];
     else if (a == "--every" && has_value) every = std::strtol(argv[++i], nullptr, 10);
+    else if (a == "--semi-implicit" && has_value) beta = std::strtod(argv[++i], nullptr);
     else if (a.rfind("--", 0) == 0) return usage(argv[0]);
     else pos.push_back(argv[i]);
   }
@@ -143,7 +148,10 @@ int main(int argc, char** argv) {
   const int64_t n = s.size();
 
   try {
-    snow_amd::SnowExplicitTimeUpdate sim(d, flags);
+    std::unique_ptr<snow_amd::SnowExplicitTimeUpdate> owner;
+    if (beta >= 0.0) owner.reset(new snow_amd::SnowSemiImplicitTimeUpdate(d, beta, 1e-5, 100, flags));
+    else owner.reset(new snow_amd::SnowExplicitTimeUpdate(d, flags));
+    snow_amd::SnowExplicitTimeUpdate& sim = *owner;  // TimeUpdate-shaped: init() and update(dt)
     sim.init();
     char frame[4096];
     for (long step = 1; step <= steps; ++step) {
