@@ -138,6 +138,7 @@ struct snow_ctx {
   uint32_t* d_chunk_lo = nullptr;  // first sorted position of each P2G chunk, as the P2G chose it
   uint32_t* d_chunk_w = nullptr;   // planned window start of each P2G chunk (before tile alignment)
   uint32_t* d_chunk_runs = nullptr;  // runs (tiles) of each chunk in the last P2G
+  uint32_t* d_fold_job = nullptr;    // per chunk: the split tile it folds in the last P2G (or ~0)
   unsigned long long* d_plan_pref = nullptr;  // cost prefix scratch of the chunk plan
   // G2P waves: each scene's sorted range cut into waves of 64 positions, (first position, scene)
   // per wave; only for batches of several scenes (one scene: wave w starts at 64 w)
@@ -388,10 +389,9 @@ int run_p2g(snow_ctx* c, int mass_only) {
         c->d_scenes, c->d_tile_scene, c->d_keys[1], c->d_slot, c->d_tile_c0, c->d_tile_c1, c->d_tile_stamp,
         c->d_ctl, c->d_list_ctr, c->d_tile_list, c->total_ctiles, c->d_rec[c->cur], c->d_fa, c->N, c->d_tpart,
         c->d_hpart, c->d_chunk_scene, c->d_pbound, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs,
-        mass_only ? 0 : 1, c->total_chunks <= uint32_t(P2G_BLOCKS_PER_CU * c->num_cu) ? 1 : 0);
+        mass_only ? 0 : 1, c->total_chunks <= uint32_t(P2G_BLOCKS_PER_CU * c->num_cu) ? 1 : 0, c->d_fold_job);
     LAUNCHCHK(c);
-    k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_scenes, c->d_chunk_scene, c->d_keys[1],
-                                                            c->d_tile_c0, c->d_tile_c1, c->d_chunk_lo, c->d_tpart,
+    k_fold_partials<<<c->total_chunks, 1024, 0, c->stream>>>(c->d_fold_job, c->d_tile_c0, c->d_chunk_lo, c->d_tpart,
                                                             c->d_hpart);
     LAUNCHCHK(c);
   }
@@ -611,7 +611,7 @@ void release(snow_ctx* c) {
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_fa, c->d_pid[0],
                   c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp,
-                  c->d_tile_c0, c->d_tile_c1, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
+                  c->d_tile_c0, c->d_tile_c1, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_fold_job, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage,
                   c->d_imp_vec, c->d_imp_mass, c->d_imp_acc, c->d_imp_pd, c->d_imp_G, c->d_imp_part,
                   c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal, c->d_imp_gslots};
@@ -803,6 +803,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_chunk_lo, std::max<uint32_t>(1, c->total_chunks)));
   CREATECHK(dalloc(&c->d_chunk_w, std::max<uint32_t>(1, c->total_chunks)));
   CREATECHK(dalloc(&c->d_chunk_runs, std::max<uint32_t>(1, c->total_chunks)));
+  CREATECHK(dalloc(&c->d_fold_job, std::max<uint32_t>(1, c->total_chunks)));
   CREATECHK(dalloc(&c->d_plan_pref, std::max<uint32_t>(1, c->total_chunks)));
   {
     std::vector<uint2> waves;

@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
     uint32_t list_cap, const float4* __restrict__ rec, const float* __restrict__ fa, uint32_t np,
     float* __restrict__ tpart, float* __restrict__ hpart, const uint16_t* __restrict__ chunk_scene,
     uint32_t* __restrict__ pbound, uint32_t* __restrict__ chunk_lo, const uint32_t* __restrict__ chunk_w,
-    uint32_t* __restrict__ chunk_runs, int skip_heavy, int by_progress) {
+    uint32_t* __restrict__ chunk_runs, int skip_heavy, int by_progress, uint32_t* __restrict__ fold_job) {
   constexpr int NT = 64 * P2G_WAVES;
   constexpr int PER = P2G_SEG_MAX / NT;  // chunk positions per thread in the prologue
   __shared__ double s_acc[NQ * HALO_NODES];  // integer-valued fixed-point sums
@@ -550,7 +550,17 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   const float inv_h = S.inv_h, uo0 = S.uo[0], uo1 = S.uo[1], uo2 = S.uo[2];
   __syncthreads();
   const uint32_t nrun = s_nrun;
-  if (t == 0) chunk_runs[chunk] = nrun;  // for the plan of the next P2G's windows
+  if (t == 0) {
+    chunk_runs[chunk] = nrun;  // for the plan of the next P2G's windows
+    // the fold of this chunk: the tile its first run continues from an earlier chunk, if that run
+    // also ends here (then this chunk is the tile's last and adds the continuation partials)
+    uint32_t job = 0xffffffffu;
+    if (nrun > 0 && s_cell[s_run[0]] != P2G_DEAD) {
+      const uint32_t tf = keys[lo + s_run[0]] / TILE_CELLS;
+      if (s_edge[0] == tf && (nrun > 1 || s_edge[1] != tf)) job = tf;
+    }
+    fold_job[chunk] = job;
+  }
 
   // ---- the wave's batches over all runs, with the records of the next batch loaded ahead
   // cursor: run nr, batch [nc, min(nc + P2G_CHUNK, ne)) of the wave's share [., ne) of run nr
@@ -740,30 +750,21 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
 
 // ---------------------------------------------------------------------------------------
 // A tile whose particles span several P2G chunks has its first run in tpart[tile] and one
-// continuation partial per later chunk. The block of the tile's last chunk adds the
-// continuations to tpart[tile] in chunk order (double sums of the float partials).
-__global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restrict__ scenes,
-                                                       const uint16_t* __restrict__ chunk_scene,
-                                                       const uint32_t* __restrict__ keys,
+// continuation partial per later chunk. The block of the tile's last chunk (fold_job, recorded by
+// that chunk in the P2G) adds the continuations to tpart[tile] in chunk order (double sums of the
+// float partials).
+__global__ void __launch_bounds__(1024) k_fold_partials(const uint32_t* __restrict__ fold_job,
                                                        const uint32_t* __restrict__ tile_c0,
-                                                       const uint32_t* __restrict__ tile_c1,
                                                        const uint32_t* __restrict__ chunk_lo,
                                                        float* __restrict__ tpart,
                                                        const float* __restrict__ hpart) {
   const uint32_t chunk = blockIdx.x;
-  const SceneDev& S = scenes[chunk_scene[chunk]];
-  const uint32_t end = uint32_t(S.p_off + S.p_cnt);
-  const uint32_t last = S.chunk_off + uint32_t((S.p_cnt + S.seg - 1) / S.seg) - 1u;
-  auto chunk_hi = [&](uint32_t c) { return c < last ? chunk_lo[c + 1] : end; };
-  const uint32_t lo = chunk_lo[chunk];  // the bounds the P2G used
-  if (lo >= chunk_hi(chunk)) return;   // an empty chunk
-  const uint32_t tile = keys[lo] / TILE_CELLS;
-  if (tile == S.dead_tile) return;
+  const uint32_t tile = fold_job[chunk];
+  if (tile == 0xffffffffu) return;  // no tile ends here that began in an earlier chunk
   const uint32_t c0 = tile_c0[tile];
-  if (c0 == chunk || tile_c1[tile] != chunk) return;  // no continuation here, or a later chunk folds
   float* dst = tpart + size_t(tile) * (NQ * HALO_NODES);
   // the chunks after the first that hold particles (each wrote a continuation partial; a chunk
-  // whose planned window was empty wrote nothing)
+  // whose planned window was empty wrote nothing; this chunk holds the tile's last run)
   for (int i = threadIdx.x; i < NQ * HALO_NODES; i += blockDim.x) {
     double acc = double(dst[i]);
     // eight loads in flight at a time; the sum stays in chunk order (adding 0.0 is exact)
@@ -771,7 +772,7 @@ __global__ void __launch_bounds__(1024) k_fold_partials(const SceneDev* __restri
       double v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        v[u] = (c + u <= chunk && chunk_lo[c + u] < chunk_hi(c + u))
+        v[u] = (c + u <= chunk && (c + u == chunk || chunk_lo[c + u] < chunk_lo[c + u + 1]))
                    ? double(hpart[size_t(c + u) * (NQ * HALO_NODES) + i]) : 0.0;
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc += v[u];
