@@ -846,6 +846,7 @@ template <class R> void implicit_solve(Scene<R>& s, R dt, R beta, R tol, int max
   std::fill(p.begin(), p.end(), R(0));
   std::fill(Kp.begin(), Kp.end(), R(0));
   while (rr > tol * tol * bb && it < max_iter) {
+    if (it > 0 && !(rho != R(0))) break;  // r . K r vanished (K is not definite): no next direction
     implicit_apply(s, dt, beta, r.data(), Kr.data());
     const R rho_new = dotv(r, Kr);
     const R bc = it == 0 ? R(0) : rho_new / rho;

@@ -576,6 +576,8 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
 int step_semi_implicit_impl(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
   if (!c->initialized) return fail(c, SNOW_E_STATE, "snow_init must run before stepping");
   int rc;
+  // the solver's memory first, so that a failed allocation leaves the step undone
+  if ((rc = ensure_implicit(c))) return rc;
   if ((rc = run_force_prepass(c))) return rc;
   if ((rc = run_sort(c, dt))) return rc;
   if ((rc = run_p2g(c, 0))) return rc;

@@ -619,6 +619,10 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const ui
     Z.rr = t1;
     Z.done = (t1 <= Z.tol2 * t0 || Z.max_iter <= 0) ? 1 : 0;
   } else if (phase == IMP_KR) {
+    if (Z.iters > 0 && !(Z.rho != 0.0)) {  // r . K r vanished (K is not definite): no next direction
+      Z.done = 1;
+      return;
+    }
     Z.beta = Z.iters == 0 ? 0.0 : t0 / Z.rho;
     Z.rho = t0;
   } else if (phase == IMP_P) {
@@ -627,7 +631,7 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const ui
   } else {
     Z.rr = t0;
     Z.iters += 1;
-    Z.done = (t0 <= Z.tol2 * Z.bb || Z.iters >= Z.max_iter) ? 1 : 0;
+    Z.done = (t0 <= Z.tol2 * Z.bb || Z.iters >= Z.max_iter || !(t0 == t0)) ? 1 : 0;
   }
 }
 
