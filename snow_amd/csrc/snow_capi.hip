@@ -149,13 +149,13 @@ struct snow_ctx {
   float4* d_imp_vec = nullptr;           // IMP_VECS dense node vectors
   float* d_imp_mass = nullptr;           // node masses of the step (0 where unused)
   long long* d_imp_acc = nullptr;        // fixed-point df, 3 per node
-  float* d_imp_pd = nullptr;             // IMP_PLANES planes of prepared particle data
-  float* d_imp_G = nullptr;              // 9 planes of G_p
+  float4* d_imp_pd = nullptr;            // IMP_PLANES float4 planes of prepared particle data
+  float* d_imp_G = nullptr;              // G_p: float4 planes G0..G3, G4..G7, then a float plane G8
   double* d_imp_part = nullptr;          // 3 partial sums per node block
   ImpBlock* d_imp_blocks = nullptr;      // node blocks (scene, first node)
   uint32_t* d_imp_scene_blocks = nullptr;  // first node block of every scene, then the total
   ImpScal* d_imp_scal = nullptr;         // per-scene solver state
-  uint32_t* d_imp_gslots = nullptr;      // IMP_GSLOTS bound slots per scene
+  uint32_t* d_imp_gslots = nullptr;      // IMP_GSLOTS bound slots per scene, IMP_GSTRIDE words apart
   ImpScal* h_imp_scal = nullptr;         // pinned host copy for the convergence polls
   uint32_t imp_nblocks = 0;
 };
@@ -484,7 +484,7 @@ int ensure_implicit(snow_ctx* c) {
       bad(dalloc(&c->d_imp_G, np * 9)) || bad(dalloc(&c->d_imp_part, 3 * std::max<size_t>(1, blocks.size()))) ||
       bad(dalloc(&c->d_imp_blocks, std::max<size_t>(1, blocks.size()))) ||
       bad(dalloc(&c->d_imp_scene_blocks, first.size())) || bad(dalloc(&c->d_imp_scal, size_t(c->nscenes))) ||
-      bad(dalloc(&c->d_imp_gslots, size_t(c->nscenes) * IMP_GSLOTS)) ||
+      bad(dalloc(&c->d_imp_gslots, size_t(c->nscenes) * IMP_GSLOTS * IMP_GSTRIDE)) ||
       bad(hipHostMalloc(reinterpret_cast<void**>(&c->h_imp_scal), sizeof(ImpScal) * c->nscenes)))
     return fail(c, SNOW_E_CAPACITY, "semi-implicit solve: no device memory for its node vectors");
   HIPCHK(c, hipMemsetAsync(c->d_imp_acc, 0, sizeof(long long) * 3 * nn, c->stream));
@@ -507,6 +507,7 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
   const int64_t nn = c->total_nodes;
   const uint2* waves = c->nscenes > 1 ? c->d_waves : nullptr;
   const unsigned pg = grid_for(c->nwaves, 4u);
+  const unsigned sg = grid_for(c->nwaves, 4u * IMP_WPW);
   auto scalars = [&](int phase) -> int {
     k_imp_scalars<<<c->nscenes, IMP_THREADS, 0, c->stream>>>(phase, c->d_imp_scene_blocks, nb, c->d_imp_part,
                                                              c->d_imp_scal, c->d_scenes, dt, beta, tol, max_iter,
@@ -521,7 +522,7 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
                                              c->d_imp_vec + size_t(v) * nn, c->d_imp_G, c->d_imp_scal,
                                              c->d_imp_gslots);
       LAUNCHCHK(c);
-      k_imp_scatter<<<pg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_imp_pd, c->d_imp_G, c->N,
+      k_imp_scatter<IMP_WPW><<<sg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_imp_pd, c->d_imp_G, c->N,
                                                reinterpret_cast<unsigned long long*>(c->d_imp_acc), c->d_imp_scal,
                                                c->d_imp_gslots);
       LAUNCHCHK(c);

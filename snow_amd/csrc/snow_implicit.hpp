@@ -16,8 +16,9 @@
 //
 // Layout: dense node vectors over the context's nodes (float4 per node: x, r, p, Kp, Kr), the node
 // masses of this step (0 outside the node tiles the grid update wrote or where m <= 0), a dense
-// fixed-point accumulator (3 x int64 per node) for df, and per sorted particle 40 planes of data
-// prepared once per step plus 9 planes of G_p = gs V_p0 A_p F_Ep^T per application.
+// fixed-point accumulator (3 x int64 per node) for df, and per sorted particle 6 float4 planes of
+// data prepared once per step (x, gs V_p0, F_E, R, 2 mu, lambda) plus G_p = gs V_p0 A_p F_Ep^T per
+// application (two float4 planes and one float plane).
 // Sums of df are exact: each wave reduces its particles per cell in registers and adds the
 // rounded fixed-point values with integer atomics, so the result does not depend on their order.
 // Dot products are block partials summed in a fixed order, so a solve is bit-reproducible.
@@ -49,11 +50,11 @@ constexpr int IMP_THREADS = 256;
 constexpr int IMP_NODES_PER_THREAD = 4;
 constexpr int IMP_NODES_PER_BLOCK = IMP_THREADS * IMP_NODES_PER_THREAD;
 constexpr int IMP_VECS = 5;       // x, r, p, Kp, Kr
-constexpr int IMP_PLANES = 40;    // prepared particle data, below
+// float4 planes of prepared particle data: (x, y, z, gs V_p0), (F0..F3), (F4..F7), (F8, R0, R1, R2),
+// (R3..R6), (R7, R8, 2 mu, lambda); F and R column-major
+constexpr int IMP_PLANES = 6;
 constexpr int IMP_STAGE = 36;     // LDS floats per staged particle in the scatter
 constexpr int IMP_MAX_DIM = 1021; // largest grid dimension: base cells + 2 fit 10 bits per axis
-// plane offsets of the prepared data
-constexpr int IP_F = 0, IP_R = 9, IP_MI = 18, IP_COF = 24, IP_C = 33, IP_X = 37;
 
 // A node-vector block: scene and first node (the blocks of a scene are consecutive).
 struct ImpBlock {
@@ -79,14 +80,30 @@ __device__ __forceinline__ int imp_exponent(uint32_t gmax_bits, int64_t count) {
   return 62 - ex;
 }
 
+// (tr S I - S)^-1 as (m00, m11, m22, m01, m02, m12), S the symmetric part of R^T F, from the
+// adjugate (tr S I - S is symmetric, about 2 I for F near I); 0 where it is singular.
+__device__ __forceinline__ void imp_spin_inverse(const float F[9], const float Rot[9], float mi[6]) {
+  float Sm[9];
+  mul_at(Rot, F, Sm);  // R^T F
+  const float s00 = Sm[0], s11 = Sm[4], s22 = Sm[8];
+  const float s01 = 0.5f * (Sm[1] + Sm[3]), s02 = 0.5f * (Sm[2] + Sm[6]), s12 = 0.5f * (Sm[5] + Sm[7]);
+  const float t00 = s11 + s22, t11 = s00 + s22, t22 = s00 + s11, t01 = -s01, t02 = -s02, t12 = -s12;
+  const float a00 = t11 * t22 - t12 * t12, a11 = t00 * t22 - t02 * t02, a22 = t00 * t11 - t01 * t01;
+  const float a01 = t02 * t12 - t01 * t22, a02 = t01 * t12 - t02 * t11, a12 = t01 * t02 - t00 * t12;
+  const float det = t00 * a00 + t01 * a01 + t02 * a02;
+  const float id = fabsf(det) > 1e-20f ? 1.0f / det : 0.0f;
+  mi[0] = a00 * id; mi[1] = a11 * id; mi[2] = a22 * id;
+  mi[3] = a01 * id; mi[4] = a02 * id; mi[5] = a12 * id;
+}
+
 // ---------------------------------------------------------------------------------------
-// Once per step, per sorted position: F_E, R, (tr S I - S)^-1 (symmetric), cof F_E, 2 mu, lambda,
-// lambda (J - 1), gs V_p0 and x. Waves hold particles of one scene (the G2P wave table).
+// Once per step, per sorted position: x, gs V_p0, F_E, R, 2 mu and lambda (the planes of pd). Waves
+// hold particles of one scene (the G2P wave table).
 __global__ void __launch_bounds__(256) k_imp_prep(const SceneDev* __restrict__ scenes, const uint2* __restrict__ waves,
                                                   uint32_t nwaves, const uint32_t* __restrict__ vals,
                                                   const float4* __restrict__ rec, const float* __restrict__ fe,
                                                   const float* __restrict__ fp, uint32_t n,
-                                                  float* __restrict__ pd, int skip_heavy) {
+                                                  float4* __restrict__ pd, int skip_heavy) {
   const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (w >= nwaves) return;
   const uint2 ws = waves ? waves[w] : make_uint2(64u * w, 0u);
@@ -100,37 +117,21 @@ __global__ void __launch_bounds__(256) k_imp_prep(const SceneDev* __restrict__ s
   for (int e = 0; e < 9; ++e) { F[e] = ld_plane(fe + size_t(e) * n, p); FP[e] = ld_plane(fp + size_t(e) * n, p); }
   const float hf = hardening_factor(S.hardening, det3(FP));
   const float mu = S.mu0 * hf, la = S.lambda0 * hf;
-  float W[9], Sg[9], V[9], Rot[9], Sm[9];
+  float W[9], Sg[9], V[9], Rot[9];
   svd3(F, W, Sg, V);
   mul_bt(W, V, Rot);
 #pragma unroll
   for (int e = 0; e < 9; ++e) Rot[e] = round_to(Rot[e], 1e9f, 1e-9f);
-  mul_at(Rot, F, Sm);  // R^T F
-  const float s00 = Sm[0], s11 = Sm[4], s22 = Sm[8];
-  const float s01 = 0.5f * (Sm[1] + Sm[3]), s02 = 0.5f * (Sm[2] + Sm[6]), s12 = 0.5f * (Sm[5] + Sm[7]);
-  // T = tr(S) I - S and its inverse from the adjugate (T is symmetric, about 2 I for F near I)
-  const float t00 = s11 + s22, t11 = s00 + s22, t22 = s00 + s11, t01 = -s01, t02 = -s02, t12 = -s12;
-  const float a00 = t11 * t22 - t12 * t12, a11 = t00 * t22 - t02 * t02, a22 = t00 * t11 - t01 * t01;
-  const float a01 = t02 * t12 - t01 * t22, a02 = t01 * t12 - t02 * t11, a12 = t01 * t02 - t00 * t12;
-  const float det = t00 * a00 + t01 * a01 + t02 * a02;
-  const float id = fabsf(det) > 1e-20f ? 1.0f / det : 0.0f;
-  float cof[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  cross_add(F + 3, F + 6, cof);
-  cross_add(F + 6, F, cof + 3);
-  cross_add(F, F + 3, cof + 6);
-  const float J = det3(F);
   const float m = xm.w, r0 = vr.w;
   const bool use = r0 > 0.0f && !(skip_heavy && fabsf(m) > 100.0f);  // as the explicit force
   const float v0 = use ? S.gscale * (m / r0) : 0.0f;
-  const float out[IMP_PLANES] = {
-      F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8],
-      Rot[0], Rot[1], Rot[2], Rot[3], Rot[4], Rot[5], Rot[6], Rot[7], Rot[8],
-      a00 * id, a11 * id, a22 * id, a01 * id, a02 * id, a12 * id,
-      cof[0], cof[1], cof[2], cof[3], cof[4], cof[5], cof[6], cof[7], cof[8],
-      2.0f * mu, la, la * (J - 1.0f), v0,
-      xm.x, xm.y, xm.z};
-#pragma unroll
-  for (int e = 0; e < IMP_PLANES; ++e) st_plane(pd + size_t(e) * n, i, out[e]);
+  const size_t N = n;
+  pd[i] = make_float4(xm.x, xm.y, xm.z, v0);
+  pd[N + i] = make_float4(F[0], F[1], F[2], F[3]);
+  pd[2 * N + i] = make_float4(F[4], F[5], F[6], F[7]);
+  pd[3 * N + i] = make_float4(F[8], Rot[0], Rot[1], Rot[2]);
+  pd[4 * N + i] = make_float4(Rot[3], Rot[4], Rot[5], Rot[6]);
+  pd[5 * N + i] = make_float4(Rot[7], Rot[8], 2.0f * mu, la);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -176,27 +177,43 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_setup(const SceneDev* __res
 
 // ---------------------------------------------------------------------------------------
 // The largest row sum of |G| of an application is published as atomic maxima into IMP_GSLOTS slots
-// per scene (wave w into slot w mod IMP_GSLOTS), so that the maxima of many waves do not serialise
-// on one address; readers take the maximum of the slots.
+// per scene, one 128-byte line apart (workgroup b into slot b mod IMP_GSLOTS), so that the maxima of
+// many workgroups do not serialise on one line; readers take the maximum of the slots.
 constexpr int IMP_GSLOTS = 64;
+constexpr int IMP_GSTRIDE = 32;  // words from one slot to the next
+__device__ __forceinline__ uint32_t* imp_slot(uint32_t* gslots, int sid, int k) {
+  return gslots + size_t(sid * IMP_GSLOTS + k) * IMP_GSTRIDE;
+}
 __device__ __forceinline__ uint32_t imp_gmax(const uint32_t* __restrict__ gslots, int sid) {
-  return uint32_t(wave_max_i(int(gslots[sid * IMP_GSLOTS + (threadIdx.x & 63)])));
+  return uint32_t(wave_max_i(int(gslots[size_t(sid * IMP_GSLOTS + (threadIdx.x & 63)) * IMP_GSTRIDE])));
 }
 
 constexpr int IMP_BOX = 384;       // nodes of a wave's LDS gather box (as the G2P box)
 constexpr int IMP_ACC_BOX = 768;   // nodes of a workgroup's LDS accumulation box
+constexpr int IMP_WPW = 4;         // wave-table entries (64 sorted positions each) per scatter wave
 
 // First half of K applied to the node vector vin: per particle dF = dt (sum_j vin_j grad w_jp^T) F,
-// A = d2Psi/dF2 : dF and G = gs V0 A F^T (9 planes, column-major), and the scene's largest row sum
-// of |G| for the fixed-point scale. A full wave of interior particles gathers from an LDS copy of the
-// node box its stencils cover (sorted particles span a few cells), else from global memory.
+// A = d2Psi/dF2 : dF and G = gs V0 A F^T (planes G0..G3, G4..G7 as float4, G8 as float; G
+// column-major), and the scene's largest row sum of |G| for the fixed-point scale. The particle's
+// prepared data is loaded first, so that it arrives while the wave gathers. A full wave of interior
+// particles gathers from an LDS copy of the node box its stencils cover (sorted particles span a
+// few cells; all of its loads are in flight together), else from global memory.
 __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ scenes, const uint2* __restrict__ waves,
-                                                   uint32_t nwaves, const float* __restrict__ pd, uint32_t n,
+                                                   uint32_t nwaves, const float4* __restrict__ pd, uint32_t n,
                                                    const float4* __restrict__ vin, float* __restrict__ G,
                                                    const ImpScal* __restrict__ scal, uint32_t* __restrict__ gslots) {
   __shared__ float4 s_box[4][IMP_BOX];
-  const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  __shared__ uint32_t s_gm[2];  // the workgroup's maximum, waves that have added theirs
+  if (threadIdx.x == 0) {
+    s_gm[0] = 0u;
+    s_gm[1] = 0u;
+  }
+  __syncthreads();
+  const uint32_t w0 = blockIdx.x * 4u, w = w0 + uint32_t(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6));
   if (w >= nwaves) return;
+  const uint32_t wlast = min(w0 + 3u, nwaves - 1u);
+  // the workgroup's waves hold one scene (then the done flag, too, is the same for all of them)
+  const bool one_scene = !waves || waves[wlast].y == waves[w0].y;
   const uint2 ws = waves ? waves[w] : make_uint2(64u * w, 0u);
   const int sid = int(ws.y);
   if (scal[sid].done) return;
@@ -204,14 +221,16 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
   const int lane = threadIdx.x & 63;
   const uint32_t i = ws.x + uint32_t(lane);
   const bool valid = int64_t(i) < S.p_off + S.p_cnt;
-  float x = 0.f, y = 0.f, z = 0.f, v0 = 0.f;
+  const size_t N = n;
+  float4 q[IMP_PLANES];
+#pragma unroll
+  for (int k = 0; k < IMP_PLANES; ++k) q[k] = make_float4(0.f, 0.f, 0.f, 0.f);
   if (valid) {
-    v0 = ld_plane(pd + size_t(IP_C + 3) * n, i);
-    x = ld_plane(pd + size_t(IP_X) * n, i);
-    y = ld_plane(pd + size_t(IP_X + 1) * n, i);
-    z = ld_plane(pd + size_t(IP_X + 2) * n, i);
+#pragma unroll
+    for (int k = 0; k < IMP_PLANES; ++k) q[k] = pd[size_t(k) * N + i];
   }
-  const float ux = grid_u(S, x, 0), uy = grid_u(S, y, 1), uz = grid_u(S, z, 2);
+  const float v0 = q[0].w;
+  const float ux = grid_u(S, q[0].x, 0), uy = grid_u(S, q[0].y, 1), uz = grid_u(S, q[0].z, 2);
   const bool inr = valid && in_stencil_range(S, ux, uy, uz);
   const bool work = inr && v0 != 0.0f;
   const int D0 = S.D[0], D1 = S.D[1], D2 = S.D[2];
@@ -227,10 +246,20 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
     if (vol <= IMP_BOX) {
       boxed = true;
       float4* box = s_box[threadIdx.x >> 6];
-      for (int idx = lane; idx < vol; idx += 64) {
-        const int ix = idx % ex, iy = (idx / ex) % ey, iz = idx / (ex * ey);
-        box[idx] = vin[S.node_off + (xmin + ix) + int64_t(D0) * ((ymin + iy) + int64_t(D1) * (zmin + iz))];
+      constexpr int BU = IMP_BOX / 64;
+      float4 tmp[BU];
+#pragma unroll
+      for (int u = 0; u < BU; ++u) {
+        const int idx = lane + 64 * u;
+        tmp[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (idx < vol) {
+          const int ix = idx % ex, iy = (idx / ex) % ey, iz = idx / (ex * ey);
+          tmp[u] = vin[S.node_off + (xmin + ix) + int64_t(D0) * ((ymin + iy) + int64_t(D1) * (zmin + iz))];
+        }
       }
+#pragma unroll
+      for (int u = 0; u < BU; ++u)
+        if (lane + 64 * u < vol) box[lane + 64 * u] = tmp[u];
       __builtin_amdgcn_wave_barrier();  // the wave's own LDS writes complete before its reads
       if (work) {
         const float4* p0 = box + ((bx - 1 - xmin) + ex * ((by - 1 - ymin) + ey * (bz - 1 - zmin)));
@@ -278,27 +307,28 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
     const float k = scal[sid].dt * S.gscale;
 #pragma unroll
     for (int e = 0; e < 9; ++e) L[e] *= k;
-    float F[9], Rot[9], C[9], dF[9], D[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) {
-      F[e] = ld_plane(pd + size_t(IP_F + e) * n, i);
-      Rot[e] = ld_plane(pd + size_t(IP_R + e) * n, i);
-      C[e] = ld_plane(pd + size_t(IP_COF + e) * n, i);
-    }
+    const float F[9] = {q[1].x, q[1].y, q[1].z, q[1].w, q[2].x, q[2].y, q[2].z, q[2].w, q[3].x};
+    const float Rot[9] = {q[3].y, q[3].z, q[3].w, q[4].x, q[4].y, q[4].z, q[4].w, q[5].x, q[5].y};
+    const float mu2 = q[5].z, la = q[5].w;
+    float dF[9], D[9], mi[6];
     mul(L, F, dF);       // dF = dt L F
     mul_at(Rot, dF, D);  // R^T dF
+    imp_spin_inverse(F, Rot, mi);
     // axial vector of D - D^T: (D(2,1) - D(1,2), D(0,2) - D(2,0), D(1,0) - D(0,1)), D(r,c) = D[c*3+r]
     const float a0 = D[5] - D[7], a1 = D[6] - D[2], a2 = D[1] - D[3];
-    const float m00 = ld_plane(pd + size_t(IP_MI) * n, i), m11 = ld_plane(pd + size_t(IP_MI + 1) * n, i),
-                m22 = ld_plane(pd + size_t(IP_MI + 2) * n, i), m01 = ld_plane(pd + size_t(IP_MI + 3) * n, i),
-                m02 = ld_plane(pd + size_t(IP_MI + 4) * n, i), m12 = ld_plane(pd + size_t(IP_MI + 5) * n, i);
-    const float w0 = m00 * a0 + m01 * a1 + m02 * a2;
-    const float w1 = m01 * a0 + m11 * a1 + m12 * a2;
-    const float w2 = m02 * a0 + m12 * a1 + m22 * a2;
+    const float w0 = mi[0] * a0 + mi[3] * a1 + mi[4] * a2;
+    const float w1 = mi[3] * a0 + mi[1] * a1 + mi[5] * a2;
+    const float w2 = mi[4] * a0 + mi[5] * a1 + mi[2] * a2;
     // [w]x column-major: (1,0) = w2, (2,0) = -w1, (0,1) = -w2, (2,1) = w0, (0,2) = w1, (1,2) = -w0
     const float Om[9] = {0.f, w2, -w1, -w2, 0.f, w0, w1, -w0, 0.f};
     float dR[9];
     mul(Rot, Om, dR);
+    // cof F (columns: cross products of the other two) and its derivative along dF
+    float C[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    cross_add(F + 3, F + 6, C);
+    cross_add(F + 6, F, C + 3);
+    cross_add(F, F + 3, C + 6);
+    const float laj = la * (det3(F) - 1.0f);
     float dJ = 0.f;
 #pragma unroll
     for (int e = 0; e < 9; ++e) dJ = fmaf(C[e], dF[e], dJ);
@@ -309,8 +339,6 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
     cross_add(F + 6, dF, dC + 3);
     cross_add(dF, F + 3, dC + 6);
     cross_add(F, dF + 3, dC + 6);
-    const float mu2 = ld_plane(pd + size_t(IP_C) * n, i), la = ld_plane(pd + size_t(IP_C + 1) * n, i),
-                laj = ld_plane(pd + size_t(IP_C + 2) * n, i);
     float A[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) A[e] = mu2 * (dF[e] - dR[e]) + (la * dJ) * C[e] + laj * dC[e];
@@ -319,64 +347,115 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
     for (int e = 0; e < 9; ++e) Gm[e] *= v0;
   }
   if (valid) {
-#pragma unroll
-    for (int e = 0; e < 9; ++e) st_plane(G + size_t(e) * n, i, Gm[e]);
+    float4* G4 = reinterpret_cast<float4*>(G);
+    G4[i] = make_float4(Gm[0], Gm[1], Gm[2], Gm[3]);
+    G4[N + i] = make_float4(Gm[4], Gm[5], Gm[6], Gm[7]);
+    G[8 * N + i] = Gm[8];
   }
   // the scene's largest row sum of |G| (non-negative floats compare as their bits)
   float rs = 0.f;
 #pragma unroll
   for (int r = 0; r < 3; ++r) rs = fmaxf(rs, fabsf(Gm[r]) + fabsf(Gm[3 + r]) + fabsf(Gm[6 + r]));
   const int mx = wave_max_i(int(__float_as_uint(rs)));
-  if (lane == 0 && mx > 0) atomicMax(&gslots[sid * IMP_GSLOTS + int(w % IMP_GSLOTS)], uint32_t(mx));
+  if (!one_scene) {
+    if (lane == 0 && mx > 0) atomicMax(imp_slot(gslots, sid, int(w % IMP_GSLOTS)), uint32_t(mx));
+    return;
+  }
+  // one atomic per workgroup: the last of its live waves publishes the workgroup's maximum
+  if (lane == 0) {
+    if (mx > 0) atomicMax(&s_gm[0], uint32_t(mx));
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's maximum is in LDS
+    if (atomicAdd(&s_gm[1], 1u) == wlast - w0) {
+      const uint32_t m = s_gm[0];
+      if (m) atomicMax(imp_slot(gslots, sid, int(blockIdx.x % IMP_GSLOTS)), m);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------
-// Second half: acc_i += sum_p G_p grad w_ip (unscaled grad w: G holds gs) in fixed point. A wave
-// takes 64 sorted positions of one scene in two batches of 32: lane k stages particle k's weights
-// and G in LDS, then lane l is the stencil node getIJK(l) and sums the staged particles in
-// registers while the base cell stays the same, adding the rounded sum at each change. When the
-// workgroup's waves are of one scene and their stencils cover at most IMP_ACC_BOX nodes, the sums
-// go into an LDS box first (integer LDS atomics) and the box is added to acc once; otherwise
-// straight to acc.
+// Second half: acc_i += sum_p G_p grad w_ip (unscaled grad w: G holds gs) in fixed point. A workgroup
+// takes 4 * WPW consecutive entries of the wave table, WPW per wave. For each entry (64 sorted
+// positions of one scene) the wave stages 32 particles at a time in LDS (lane k: particle k's
+// weights and G), then lane l is the stencil node getIJK(l) and sums the staged particles in
+// registers while the base cell stays the same, adding the rounded sum where it changes (a scalar
+// mask of the changes, taken before the loop). The wave loads the data of all its entries first.
+// When the workgroup's entries are of one scene and their stencils cover at most IMP_ACC_BOX nodes,
+// the sums go into an LDS box first (integer LDS atomics) and the box is added to acc once;
+// otherwise straight to acc.
+template <int WPW>
 __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict__ scenes,
                                                      const uint2* __restrict__ waves, uint32_t nwaves,
-                                                     const float* __restrict__ pd, const float* __restrict__ G,
+                                                     const float4* __restrict__ pd, const float* __restrict__ G,
                                                      uint32_t n, unsigned long long* __restrict__ acc,
                                                      const ImpScal* __restrict__ scal,
                                                      const uint32_t* __restrict__ gslots) {
   __shared__ __attribute__((aligned(16))) float s_stage[4][32 * IMP_STAGE];
   __shared__ unsigned long long s_acc[3 * IMP_ACC_BOX];
   __shared__ int s_ext[6];  // box min (x, y, z), max (x, y, z) of the base cells + 2
-  const int t = threadIdx.x, lane = t & 63;
-  const uint32_t w0 = blockIdx.x * 4u, w = w0 + uint32_t(t >> 6);
-  const uint32_t wlast = min(w0 + 3u, nwaves - 1u);
-  const int sid0 = waves ? int(waves[w0].y) : 0;
-  // one scene for the whole workgroup (the scene of a wave is constant along the table)
-  const bool one_scene = !waves || int(waves[wlast].y) == sid0;
-  const bool live_wave = w < nwaves;
-  const uint2 ws = live_wave ? (waves ? waves[w] : make_uint2(64u * w, 0u)) : make_uint2(0u, uint32_t(sid0));
-  const int sid = int(ws.y);
-  const SceneDev& S = scenes[sid];
-  const bool run = live_wave && !scal[sid].done;
+  // wave index, scene and loop bounds are wave-uniform (scalar registers, scalar loads and branches)
+  const int t = threadIdx.x, lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const uint32_t e0 = blockIdx.x * uint32_t(4 * WPW);
+  const uint32_t elast = min(e0 + uint32_t(4 * WPW - 1), nwaves - 1u);
+  const int sid0 = waves ? int(waves[e0].y) : 0;
+  // one scene for the whole workgroup (the scene of an entry is constant along the table)
+  const bool one_scene = !waves || int(waves[elast].y) == sid0;
   if (t < 3) s_ext[t] = 0x7fffffff;
-  if (t >= 3 && t < 6) s_ext[t] = -1;
-  const uint32_t cnt = run ? uint32_t(min<int64_t>(64, S.p_off + S.p_cnt - int64_t(ws.x))) : 0u;
-  // this lane's particle (lane < cnt): grid coordinate and base cell + 2 (-1: no in-grid node)
-  float ux = 0.f, uy = 0.f, uz = 0.f;
-  int cell = -1;
-  if (uint32_t(lane) < cnt) {
-    const uint32_t i = ws.x + uint32_t(lane);
-    ux = grid_u(S, ld_plane(pd + size_t(IP_X) * n, i), 0);
-    uy = grid_u(S, ld_plane(pd + size_t(IP_X + 1) * n, i), 1);
-    uz = grid_u(S, ld_plane(pd + size_t(IP_X + 2) * n, i), 2);
-    if (in_stencil_range(S, ux, uy, uz))  // base cell in [-2, D] per axis, + 2, 10 bits per axis
-      cell = (int(floorf(ux)) + 2) | ((int(floorf(uy)) + 2) << 10) | ((int(floorf(uz)) + 2) << 20);
+  else if (t < 6) s_ext[t] = -1;
+  const size_t N = n;
+  // this lane's particle of each of the wave's entries: grid coordinate and base cell + 2 (10 bits
+  // per axis; -1: none, out of the grid or the scene's solve is done)
+  // (all loads of the wave are issued here, before any wait)
+  float ux[WPW], uy[WPW], uz[WPW];
+  int cell[WPW];
+  float4 xq[WPW], ga[WPW], gb[WPW];
+  float gc[WPW];
+  bool live[WPW];
+#pragma unroll
+  for (int j = 0; j < WPW; ++j) {
+    const uint32_t e = e0 + uint32_t(wv * WPW + j);
+    xq[j] = ga[j] = gb[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    gc[j] = 0.f;
+    live[j] = false;
+    if (e < nwaves) {
+      const uint2 ws = waves ? waves[e] : make_uint2(64u * e, 0u);
+      const SceneDev& S = scenes[ws.y];
+      const uint32_t i = ws.x + uint32_t(lane);
+      live[j] = !scal[ws.y].done && int64_t(i) < S.p_off + S.p_cnt;
+      if (live[j]) {
+        xq[j] = pd[i];
+        ga[j] = reinterpret_cast<const float4*>(G)[i];
+        gb[j] = reinterpret_cast<const float4*>(G)[N + i];
+        gc[j] = G[8 * N + i];
+      }
+    }
   }
-  __syncthreads();
-  if (one_scene && cell >= 0) {
-    const int cx = cell & 1023, cy = (cell >> 10) & 1023, cz = cell >> 20;
-    atomicMin(&s_ext[0], cx); atomicMin(&s_ext[1], cy); atomicMin(&s_ext[2], cz);
-    atomicMax(&s_ext[3], cx); atomicMax(&s_ext[4], cy); atomicMax(&s_ext[5], cz);
+  int cmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, cmax[3] = {-1, -1, -1};
+#pragma unroll
+  for (int j = 0; j < WPW; ++j) {
+    ux[j] = uy[j] = uz[j] = 0.f;
+    cell[j] = -1;
+    if (live[j]) {
+      const uint32_t e = e0 + uint32_t(wv * WPW + j);
+      const SceneDev& S = scenes[waves ? waves[e].y : 0u];
+      ux[j] = grid_u(S, xq[j].x, 0);
+      uy[j] = grid_u(S, xq[j].y, 1);
+      uz[j] = grid_u(S, xq[j].z, 2);
+      if (in_stencil_range(S, ux[j], uy[j], uz[j])) {  // base cell in [-2, D] per axis
+        const int cx = int(floorf(ux[j])) + 2, cy = int(floorf(uy[j])) + 2, cz = int(floorf(uz[j])) + 2;
+        cell[j] = cx | (cy << 10) | (cz << 20);
+        cmin[0] = min(cmin[0], cx); cmin[1] = min(cmin[1], cy); cmin[2] = min(cmin[2], cz);
+        cmax[0] = max(cmax[0], cx); cmax[1] = max(cmax[1], cy); cmax[2] = max(cmax[2], cz);
+      }
+    }
+  }
+  __syncthreads();  // s_ext initialised
+  if (one_scene) {
+    const int lx = wave_min_i(cmin[0]), ly = wave_min_i(cmin[1]), lz = wave_min_i(cmin[2]);
+    const int hx = wave_max_i(cmax[0]), hy = wave_max_i(cmax[1]), hz = wave_max_i(cmax[2]);
+    if (lane == 0 && hx >= 0) {
+      atomicMin(&s_ext[0], lx); atomicMin(&s_ext[1], ly); atomicMin(&s_ext[2], lz);
+      atomicMax(&s_ext[3], hx); atomicMax(&s_ext[4], hy); atomicMax(&s_ext[5], hz);
+    }
   }
   __syncthreads();
   // the accumulation box: nodes (c - 2) - 1 .. (c - 2) + 2 of the base cells, clipped to the grid
@@ -391,13 +470,16 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
   if (boxed)
     for (int k = t; k < 3 * ex * ey * ez; k += 256) s_acc[k] = 0ull;
   __syncthreads();
-  const double scale = run ? ldexp(1.0, imp_exponent(imp_gmax(gslots, sid), S.p_cnt)) : 0.0;
   const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
+  // register sums of stencil node o of cell cur of scene *Sc, in fixed point with 2^e = scale
   int cur = -1;
   float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  int csid = -1, cD0 = 0, cD1 = 0, cD2 = 0;  // scene of the sums and its grid
+  int64_t cnode = 0;
+  double scale = 0.0;
   auto commit = [&]() {
     const int nx = (cur & 1023) - 3 + ox, ny = ((cur >> 10) & 1023) - 3 + oy, nz = (cur >> 20) - 3 + oz;
-    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < S.D[0] && ny < S.D[1] && nz < S.D[2]) {
+    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < cD0 && ny < cD1 && nz < cD2) {
       const long long q0 = llrint(double(s0) * scale), q1 = llrint(double(s1) * scale), q2 = llrint(double(s2) * scale);
       if (boxed) {
         unsigned long long* a = s_acc + 3 * ((nx - bx0) + ex * ((ny - by0) + ey * (nz - bz0)));
@@ -405,7 +487,7 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
         if (q1) atomicAdd(a + 1, (unsigned long long)q1);
         if (q2) atomicAdd(a + 2, (unsigned long long)q2);
       } else {
-        const int64_t node = S.node_off + nx + int64_t(S.D[0]) * (ny + int64_t(S.D[1]) * nz);
+        const int64_t node = cnode + nx + int64_t(cD0) * (ny + int64_t(cD1) * nz);
         if (q0) atomicAdd(acc + 3 * node, (unsigned long long)q0);
         if (q1) atomicAdd(acc + 3 * node + 1, (unsigned long long)q1);
         if (q2) atomicAdd(acc + 3 * node + 2, (unsigned long long)q2);
@@ -413,51 +495,71 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
     }
     s0 = s1 = s2 = 0.f;
   };
-  if (run) {
-    float* st = s_stage[t >> 6];
+  float* st = s_stage[wv];
+#pragma unroll
+  for (int j = 0; j < WPW; ++j) {
+    const uint32_t e = e0 + uint32_t(wv * WPW + j);
+    if (e >= nwaves) break;
+    const uint2 ws = waves ? waves[e] : make_uint2(64u * e, 0u);
+    const int sid = __builtin_amdgcn_readfirstlane(int(ws.y));
+    if (scal[sid].done) continue;
+    const SceneDev& Sj = scenes[sid];
+    if (sid != csid) {  // the sums so far belong to another scene
+      if (cur >= 0) commit();
+      cur = -1;
+      csid = sid;
+      cD0 = Sj.D[0];
+      cD1 = Sj.D[1];
+      cD2 = Sj.D[2];
+      cnode = Sj.node_off;
+      scale = ldexp(1.0, imp_exponent(imp_gmax(gslots, sid), Sj.p_cnt));
+    }
+    const uint32_t cnt = uint32_t(min<int64_t>(64, Sj.p_off + Sj.p_cnt - int64_t(ws.x)));
+    const int cj = cell[j];
+    const int prevc = __shfl_up(cj, 1, 64);
     for (uint32_t b0 = 0; b0 < cnt; b0 += 32u) {
       const uint32_t bc = min(32u, cnt - b0);
-      // lanes b0 .. b0 + bc - 1 stage their particle
-      if (uint32_t(lane) >= b0 && uint32_t(lane) < b0 + bc) {
-        float* d = st + (lane - int(b0)) * IMP_STAGE;
-        if (cell >= 0) {
-          const float f[3] = {ux - floorf(ux), uy - floorf(uy), uz - floorf(uz)};
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int o = 0; o < 4; ++o) {
-              d[8 * k + 2 * o] = spline_w(o, f[k]);
-              d[8 * k + 2 * o + 1] = spline_q(o, f[k]);
-            }
-        }
-        const uint32_t i = ws.x + uint32_t(lane);
-#pragma unroll
-        for (int e = 0; e < 9; ++e) d[24 + e] = ld_plane(G + size_t(e) * n, i);
-        d[33] = __int_as_float(cell);
+      const bool mine = uint32_t(lane) >= b0 && uint32_t(lane) < b0 + bc;
+      if (mine && cj >= 0) {
+        float4* d4 = reinterpret_cast<float4*>(st + (lane - int(b0)) * IMP_STAGE);
+        const float fx = ux[j] - floorf(ux[j]), fy = uy[j] - floorf(uy[j]), fz = uz[j] - floorf(uz[j]);
+        d4[0] = make_float4(spline_w(0, fx), spline_q(0, fx), spline_w(1, fx), spline_q(1, fx));
+        d4[1] = make_float4(spline_w(2, fx), spline_q(2, fx), spline_w(3, fx), spline_q(3, fx));
+        d4[2] = make_float4(spline_w(0, fy), spline_q(0, fy), spline_w(1, fy), spline_q(1, fy));
+        d4[3] = make_float4(spline_w(2, fy), spline_q(2, fy), spline_w(3, fy), spline_q(3, fy));
+        d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
+        d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
+        d4[6] = ga[j];
+        d4[7] = gb[j];
+        d4[8] = make_float4(gc[j], 0.f, 0.f, 0.f);
       }
       __builtin_amdgcn_wave_barrier();  // the wave's own staging writes complete before its reads
+      // bit b0 + k: staged particle k has another cell than the one before it (bit b0: than cur)
+      const uint64_t chg = __ballot(mine && (uint32_t(lane) == b0 ? cj != cur : cj != prevc));
       for (uint32_t k = 0; k < bc; ++k) {
-        const float* d = st + k * IMP_STAGE;
-        const int ck = __float_as_int(d[33]);
-        if (ck != cur) {
+        if ((chg >> (b0 + k)) & 1ull) {
           if (cur >= 0) commit();
-          cur = ck;
+          cur = __builtin_amdgcn_readlane(cj, int(b0 + k));
         }
-        if (ck < 0) continue;
+        if (cur < 0) continue;
+        const float* d = st + k * IMP_STAGE;
         const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
         const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
         const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
+        const float4 u0 = reinterpret_cast<const float4*>(d)[6];
+        const float4 u1 = reinterpret_cast<const float4*>(d)[7];
+        const float u2 = d[32];
         const float wyz = wqy.x * wqz.x;
         const float g0 = wqx.y * wyz, g1 = wqy.y * (wqx.x * wqz.x), g2 = wqz.y * (wqx.x * wqy.x);
-        // (G grad w)_r = G(r,0) g0 + G(r,1) g1 + G(r,2) g2, G(r,c) = d[24 + c*3 + r]
-        s0 = fmaf(d[24], g0, fmaf(d[27], g1, fmaf(d[30], g2, s0)));
-        s1 = fmaf(d[25], g0, fmaf(d[28], g1, fmaf(d[31], g2, s1)));
-        s2 = fmaf(d[26], g0, fmaf(d[29], g1, fmaf(d[32], g2, s2)));
+        // (G grad w)_r = G(r,0) g0 + G(r,1) g1 + G(r,2) g2, G(r,c) = G[c*3 + r]
+        s0 = fmaf(u0.x, g0, fmaf(u0.w, g1, fmaf(u1.z, g2, s0)));
+        s1 = fmaf(u0.y, g0, fmaf(u1.x, g1, fmaf(u1.w, g2, s1)));
+        s2 = fmaf(u0.z, g0, fmaf(u1.y, g1, fmaf(u2, g2, s2)));
       }
       __builtin_amdgcn_wave_barrier();  // all reads of the batch done before the next one is staged
     }
-    if (cur >= 0) commit();
   }
+  if (cur >= 0) commit();
   if (!boxed) return;
   __syncthreads();
   const int vol = ex * ey * ez;
@@ -583,7 +685,7 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const ui
   // the bound slots of the next application start at 0 (after the start, and after each phase
   // that consumed them)
   if ((phase < 0 || phase == IMP_INIT || phase == IMP_KR) && threadIdx.x < IMP_GSLOTS)
-    gslots[sid * IMP_GSLOTS + threadIdx.x] = 0u;
+    *imp_slot(gslots, sid, int(threadIdx.x)) = 0u;
   if (phase < 0) {
     if (threadIdx.x == 0) {
       const float dt = dt_override > 0.0f ? dt_override : scenes[sid].dt;
