@@ -470,15 +470,29 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
   if (boxed)
     for (int k = t; k < 3 * ex * ey * ez; k += 256) s_acc[k] = 0ull;
   __syncthreads();
-  const int ox = lane & 3, oy = (lane >> 2) & 3, oz = lane >> 4;
-  // register sums of stencil node o of cell cur of scene *Sc, in fixed point with 2^e = scale
+  // lane = (ox, oy) of the stencil and a subgroup sg of the particles: the lane sums the nodes
+  // (ox, oy, 0..3) for the particles k = 4i + sg of the current cell, so one LDS read of a staged
+  // particle serves four nodes; at a change of cell the four subgroups' sums are reduced and lane
+  // (ox, oy, sg) adds node (ox, oy, sg)
+  const int ox = lane & 3, oy = (lane >> 2) & 3, sg = lane >> 4;
+  // register sums of the nodes (ox, oy, oz) of cell cur, in fixed point with 2^e = scale
   int cur = -1;
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  float sx[4] = {0.f, 0.f, 0.f, 0.f}, sy[4] = {0.f, 0.f, 0.f, 0.f}, sz[4] = {0.f, 0.f, 0.f, 0.f};
   int csid = -1, cD0 = 0, cD1 = 0, cD2 = 0;  // scene of the sums and its grid
   int64_t cnode = 0;
   double scale = 0.0;
   auto commit = [&]() {
-    const int nx = (cur & 1023) - 3 + ox, ny = ((cur >> 10) & 1023) - 3 + oy, nz = (cur >> 20) - 3 + oz;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int z = 0; z < 4; ++z) {
+      // sum over the subgroups (lanes l, l ^ 16, l ^ 32, l ^ 48), the same order in every lane
+      float a = sx[z], b = sy[z], c = sz[z];
+      a += __shfl_xor(a, 16, 64); b += __shfl_xor(b, 16, 64); c += __shfl_xor(c, 16, 64);
+      a += __shfl_xor(a, 32, 64); b += __shfl_xor(b, 32, 64); c += __shfl_xor(c, 32, 64);
+      if (sg == z) { s0 = a; s1 = b; s2 = c; }
+      sx[z] = sy[z] = sz[z] = 0.f;
+    }
+    const int nx = (cur & 1023) - 3 + ox, ny = ((cur >> 10) & 1023) - 3 + oy, nz = (cur >> 20) - 3 + sg;
     if (nx >= 0 && ny >= 0 && nz >= 0 && nx < cD0 && ny < cD1 && nz < cD2) {
       const long long q0 = llrint(double(s0) * scale), q1 = llrint(double(s1) * scale), q2 = llrint(double(s2) * scale);
       if (boxed) {
@@ -493,9 +507,10 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
         if (q2) atomicAdd(acc + 3 * node + 2, (unsigned long long)q2);
       }
     }
-    s0 = s1 = s2 = 0.f;
   };
   float* st = s_stage[wv];
+  // this lane's read offsets in a staged particle (floats): (w, q) of ox and of oy
+  const int lx = 2 * ox + IMP_STAGE * sg, ly = 8 + 2 * oy + IMP_STAGE * sg, lz = IMP_STAGE * sg;
 #pragma unroll
   for (int j = 0; j < WPW; ++j) {
     const uint32_t e = e0 + uint32_t(wv * WPW + j);
@@ -521,40 +536,56 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
       const uint32_t bc = min(32u, cnt - b0);
       const bool mine = uint32_t(lane) >= b0 && uint32_t(lane) < b0 + bc;
       if (mine && cj >= 0) {
+        // (w_o, q_o) interleaved for x and y, then w_0..3 and q_0..3 for z, then G
         float4* d4 = reinterpret_cast<float4*>(st + (lane - int(b0)) * IMP_STAGE);
         const float fx = ux[j] - floorf(ux[j]), fy = uy[j] - floorf(uy[j]), fz = uz[j] - floorf(uz[j]);
         d4[0] = make_float4(spline_w(0, fx), spline_q(0, fx), spline_w(1, fx), spline_q(1, fx));
         d4[1] = make_float4(spline_w(2, fx), spline_q(2, fx), spline_w(3, fx), spline_q(3, fx));
         d4[2] = make_float4(spline_w(0, fy), spline_q(0, fy), spline_w(1, fy), spline_q(1, fy));
         d4[3] = make_float4(spline_w(2, fy), spline_q(2, fy), spline_w(3, fy), spline_q(3, fy));
-        d4[4] = make_float4(spline_w(0, fz), spline_q(0, fz), spline_w(1, fz), spline_q(1, fz));
-        d4[5] = make_float4(spline_w(2, fz), spline_q(2, fz), spline_w(3, fz), spline_q(3, fz));
+        d4[4] = make_float4(spline_w(0, fz), spline_w(1, fz), spline_w(2, fz), spline_w(3, fz));
+        d4[5] = make_float4(spline_q(0, fz), spline_q(1, fz), spline_q(2, fz), spline_q(3, fz));
         d4[6] = ga[j];
         d4[7] = gb[j];
         d4[8] = make_float4(gc[j], 0.f, 0.f, 0.f);
       }
       __builtin_amdgcn_wave_barrier();  // the wave's own staging writes complete before its reads
-      // bit b0 + k: staged particle k has another cell than the one before it (bit b0: than cur)
-      const uint64_t chg = __ballot(mine && (uint32_t(lane) == b0 ? cj != cur : cj != prevc));
-      for (uint32_t k = 0; k < bc; ++k) {
-        if ((chg >> (b0 + k)) & 1ull) {
+      // bit k: staged particle k has another cell than the one before it (bit 0: than cur)
+      const uint32_t chg =
+          uint32_t(__ballot(mine && (uint32_t(lane) == b0 ? cj != cur : cj != prevc)) >> b0);
+      uint32_t k = 0;
+      while (k < bc) {  // segments of one cell: [k, kend)
+        if ((chg >> k) & 1u) {
           if (cur >= 0) commit();
           cur = __builtin_amdgcn_readlane(cj, int(b0 + k));
         }
-        if (cur < 0) continue;
-        const float* d = st + k * IMP_STAGE;
-        const float2 wqx = reinterpret_cast<const float2*>(d)[ox];
-        const float2 wqy = reinterpret_cast<const float2*>(d + 8)[oy];
-        const float2 wqz = reinterpret_cast<const float2*>(d + 16)[oz];
-        const float4 u0 = reinterpret_cast<const float4*>(d)[6];
-        const float4 u1 = reinterpret_cast<const float4*>(d)[7];
-        const float u2 = d[32];
-        const float wyz = wqy.x * wqz.x;
-        const float g0 = wqx.y * wyz, g1 = wqy.y * (wqx.x * wqz.x), g2 = wqz.y * (wqx.x * wqy.x);
-        // (G grad w)_r = G(r,0) g0 + G(r,1) g1 + G(r,2) g2, G(r,c) = G[c*3 + r]
-        s0 = fmaf(u0.x, g0, fmaf(u0.w, g1, fmaf(u1.z, g2, s0)));
-        s1 = fmaf(u0.y, g0, fmaf(u1.x, g1, fmaf(u1.w, g2, s1)));
-        s2 = fmaf(u0.z, g0, fmaf(u1.y, g1, fmaf(u2, g2, s2)));
+        const uint32_t rest = k + 1u < 32u ? chg >> (k + 1u) : 0u;
+        const uint32_t kend = rest ? min(bc, k + 1u + uint32_t(__builtin_ctz(rest))) : bc;
+        if (cur >= 0) {
+          for (uint32_t kk = k; kk < kend; kk += 4u) {
+            if (kk + uint32_t(sg) < kend) {
+              const float* d = st + kk * IMP_STAGE;
+              const float2 wqx = *reinterpret_cast<const float2*>(d + lx);
+              const float2 wqy = *reinterpret_cast<const float2*>(d + ly);
+              const float4 wz = *reinterpret_cast<const float4*>(d + lz + 16);
+              const float4 qz = *reinterpret_cast<const float4*>(d + lz + 20);
+              const float4 u0 = *reinterpret_cast<const float4*>(d + lz + 24);
+              const float4 u1 = *reinterpret_cast<const float4*>(d + lz + 28);
+              const float u2 = d[lz + 32];
+              const float wxy = wqx.x * wqy.x, qxwy = wqx.y * wqy.x, wxqy = wqx.x * wqy.y;
+              const float wzv[4] = {wz.x, wz.y, wz.z, wz.w}, qzv[4] = {qz.x, qz.y, qz.z, qz.w};
+#pragma unroll
+              for (int z = 0; z < 4; ++z) {
+                const float g0 = qxwy * wzv[z], g1 = wxqy * wzv[z], g2 = wxy * qzv[z];
+                // (G grad w)_r = G(r,0) g0 + G(r,1) g1 + G(r,2) g2, G(r,c) = G[c*3 + r]
+                sx[z] = fmaf(u0.x, g0, fmaf(u0.w, g1, fmaf(u1.z, g2, sx[z])));
+                sy[z] = fmaf(u0.y, g0, fmaf(u1.x, g1, fmaf(u1.w, g2, sy[z])));
+                sz[z] = fmaf(u0.z, g0, fmaf(u1.y, g1, fmaf(u2, g2, sz[z])));
+              }
+            }
+          }
+        }
+        k = kend;
       }
       __builtin_amdgcn_wave_barrier();  // all reads of the batch done before the next one is staged
     }
