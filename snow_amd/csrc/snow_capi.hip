@@ -102,7 +102,8 @@ struct snow_ctx {
   float2* d_g_node = nullptr;  // 3 float2 per node
   float* d_g_mass = nullptr;   // node mass of the last grid update
   Collider* d_cols = nullptr;
-  unsigned long long* d_counts = nullptr;  // [nscenes] active nodes, then [1] fp64 count
+  // [nscenes][ACTIVE_SLOTS] active-node slots (ACTIVE_STRIDE words apart), then [1] fp64 count
+  unsigned long long* d_counts = nullptr;
   int* d_bad = nullptr;
   float* d_aux = nullptr;
   void* d_stage = nullptr;
@@ -161,6 +162,11 @@ struct snow_ctx {
 };
 
 namespace {
+
+// the count of particles that took the FP64 path, after the active-node slots of d_counts
+unsigned long long* fp64_counter(snow_ctx* c) {
+  return c->d_counts + size_t(c->nscenes) * ACTIVE_SLOTS * ACTIVE_STRIDE;
+}
 
 int fail(snow_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg; else g_create_error = msg;
@@ -362,8 +368,8 @@ int run_force_prepass(snow_ctx* c) {
 // Step start (stamp, counters, rigid advance when dt_rigid >= 0) + sort. The step start rides on
 // the sort's first launch; it runs on its own only when there is nothing to sort.
 int run_sort(snow_ctx* c, float dt_rigid) {
-  const StepBegin begin{c->d_ctl, c->d_counts, c->nscenes + 1, c->nscenes, c->d_scenes, c->d_cols, c->d_list_ctr,
-                        dt_rigid};
+  const StepBegin begin{c->d_ctl,    c->d_counts, c->nscenes * ACTIVE_SLOTS, fp64_counter(c),
+                        c->nscenes, c->d_scenes, c->d_cols,                c->d_list_ctr, dt_rigid};
   int rc = run_keys(c);
   if (rc) return rc;
   StageTimer tm(c, ST_SORT);
@@ -427,7 +433,7 @@ int run_grid(snow_ctx* c, bool keep, bool update, float dt, bool plan) {
 int run_g2p_advance(snow_ctx* c, float dt) {
   StageTimer tm(c, ST_G2P);
   const int a = c->cur, b = 1 - c->cur;
-  unsigned long long* f64 = c->d_counts + c->nscenes;
+  unsigned long long* f64 = fp64_counter(c);
   const bool mixed = (c->flags & SNOW_FLAG_MIXED) != 0;
   const bool comp = (c->flags & SNOW_FLAG_COMPENSATED_G2P) != 0;
   if (c->N) {
@@ -832,7 +838,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(dalloc(&c->d_g_node, 3 * size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_g_mass, size_t(c->total_nodes)));
   CREATECHK(dalloc(&c->d_cols, std::max<size_t>(1, c->cols.size())));
-  CREATECHK(dalloc(&c->d_counts, size_t(c->nscenes + 1)));
+  CREATECHK(dalloc(&c->d_counts, size_t(c->nscenes) * ACTIVE_SLOTS * ACTIVE_STRIDE + 1));
   CREATECHK(dalloc(&c->d_bad, 1));
 
   // tables
@@ -902,7 +908,7 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
   CREATECHK(hipMemset(c->d_g_f, 0, sizeof(float4) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_node, 0, 3 * sizeof(float2) * std::max<int64_t>(1, c->total_nodes)));
   CREATECHK(hipMemset(c->d_g_mass, 0, sizeof(float) * std::max<int64_t>(1, c->total_nodes)));
-  CREATECHK(hipMemset(c->d_counts, 0, sizeof(unsigned long long) * (c->nscenes + 1)));
+  CREATECHK(hipMemset(c->d_counts, 0, sizeof(unsigned long long) * (size_t(c->nscenes) * ACTIVE_SLOTS * ACTIVE_STRIDE + 1)));
   CREATECHK(hipDeviceSynchronize());
 #undef CREATECHK
   c->cur = 0;
@@ -1281,7 +1287,7 @@ int snow_stage_g2p_advance(snow_ctx* c, double dt) {
                                c->key_bits, c->sort_bits, c->stream, c->num_cu));
   }
   // the FP64-path counter of this call; the next P2G's bound from this call's records alone
-  k_fill_u32<<<1, 64, 0, c->stream>>>(reinterpret_cast<uint32_t*>(c->d_counts + c->nscenes), 0u, 2);
+  k_fill_u32<<<1, 64, 0, c->stream>>>(reinterpret_cast<uint32_t*>(fp64_counter(c)), 0u, 2);
   LAUNCHCHK(c);
   if ((rc = clear_bounds(c))) return rc;
   return run_g2p_advance(c, fdt);
@@ -1573,10 +1579,13 @@ int snow_positions_view(snow_ctx* c, int32_t scene, const float** device_ptr, in
 }
 int snow_active_nodes(snow_ctx* c, int32_t s, int64_t* out) {
   if (!c || s < 0 || s >= c->nscenes || !out) return SNOW_E_ARG;
-  unsigned long long v = 0;
+  std::vector<unsigned long long> v(size_t(ACTIVE_SLOTS) * ACTIVE_STRIDE);
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(&v, c->d_counts + s, sizeof v, hipMemcpyDeviceToHost));
-  *out = int64_t(v);
+  HIPCHK(c, hipMemcpy(v.data(), c->d_counts + size_t(s) * ACTIVE_SLOTS * ACTIVE_STRIDE,
+                      sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost));
+  unsigned long long sum = 0;
+  for (int k = 0; k < ACTIVE_SLOTS; ++k) sum += v[size_t(k) * ACTIVE_STRIDE];
+  *out = int64_t(sum);
   return 0;
 }
 
@@ -1584,7 +1593,7 @@ int snow_fp64_particles(snow_ctx* c, int64_t* out) {
   if (!c || !out) return SNOW_E_ARG;
   unsigned long long v = 0;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(&v, c->d_counts + c->nscenes, sizeof v, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&v, fp64_counter(c), sizeof v, hipMemcpyDeviceToHost));
   *out = int64_t(v);
   return 0;
 }

@@ -159,14 +159,20 @@ struct StepCtl {
 constexpr int TILE_LISTS = 64;
 constexpr int TILE_LIST_STRIDE = 32;
 
+// Active-node counts: ACTIVE_SLOTS slots per scene, one 128-byte line apart, so that the adds of
+// many waves spread over lines; a scene's count is the sum of its slots.
+constexpr int ACTIVE_SLOTS = 64;
+constexpr int ACTIVE_STRIDE = 16;  // 64-bit words from one slot to the next
+
 // Start of a step: new stamp, cleared counters, and the collider advance of rigidAdvance.glsl:14-18
 // (cx += dt*cv) when dt_rigid >= 0 (0 selects each scene's own dt). Nothing the sort reads is
 // touched, so the first sort launch carries this work in its workgroup 0 (k_radix_hist); the
 // kernel k_step_begin runs it alone when there is nothing to sort.
 struct StepBegin {
   StepCtl* ctl;
-  unsigned long long* counts;
+  unsigned long long* counts;  // active-node slots: ncounts slot heads, ACTIVE_STRIDE words apart
   int ncounts;
+  unsigned long long* fp64;    // particles that took the FP64 path this step
   int nscenes;
   const SceneDev* scenes;
   Collider* cols;
@@ -181,7 +187,8 @@ __device__ __forceinline__ void step_begin_block(const StepBegin& a, int t, int 
     a.ctl->n_active = 0u;
     a.ctl->spare = 0u;
   }
-  for (int i = t; i < a.ncounts; i += nt) a.counts[i] = 0ull;
+  for (int i = t; i < a.ncounts; i += nt) a.counts[size_t(i) * ACTIVE_STRIDE] = 0ull;
+  if (t == 0) *a.fp64 = 0ull;
   if (a.dt_rigid >= 0.0f) {
     for (int s = t; s < a.nscenes; s += nt) {
       const SceneDev& S = a.scenes[s];
@@ -607,7 +614,7 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g_chunk(
   }
 
   for (uint32_t r = 0; r < nrun; ++r) {
-    const uint32_t r_lo = s_run[r], r_hi = s_run[r + 1];
+    const uint32_t r_lo = s_run[r];
     if (s_cell[r_lo] == P2G_DEAD) continue;  // out-of-grid particles (uniform)
     const uint32_t tile = keys[lo + r_lo] / TILE_CELLS;
     const bool starts_tile = r > 0 || s_edge[0] != tile;
@@ -916,6 +923,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
     const uint32_t first = uint32_t(__builtin_amdgcn_readlane(int(incl - cnt), sub));
     return tile_list[size_t(sub) * list_cap + (e - first)];
   };
+  // the wave's active-node count, added to slot cnt_slot of its scene when the scene changes and
+  // at the end (one atomic per wave and scene instead of one per node tile)
+  int cnt_sid = -1;
+  const int cnt_slot = int((bid * uint32_t(GRID_THREADS / 64) + uint32_t(a)) % uint32_t(ACTIVE_SLOTS));
+  unsigned long long cnt_sum = 0ull;
   uint32_t k_next = bid < total ? entry(bid) : 0u;
   for (uint32_t e = bid; e < total; e += nblk) {
     const uint32_t k = k_next;
@@ -923,6 +935,12 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
     if (e + nblk < total) k_next = entry(e + nblk);
     const int sid = tile_scene ? tile_scene[k] : 0;  // no table: one scene, whose record loads at once
     const SceneDev& S = scenes[sid];
+    if (sid != cnt_sid) {  // the wave's count so far belongs to another scene
+      if (lane == 0 && cnt_sum && active_count)
+        atomicAdd(active_count + size_t(cnt_sid * ACTIVE_SLOTS + cnt_slot) * ACTIVE_STRIDE, cnt_sum);
+      cnt_sid = sid;
+      cnt_sum = 0ull;
+    }
     const uint32_t kl = k - S.ct_off;
     const int kx = int(kl % uint32_t(S.ct[0])), ky = int((kl / uint32_t(S.ct[0])) % uint32_t(S.ct[1]));
     const int kz = int(kl / uint32_t(S.ct[0] * S.ct[1]));
@@ -983,11 +1001,12 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid(const SceneDev* __restric
       }
       store_node(g_node, g_mass, node, vx, vy, vz, bx, by, bz, m);
     }
-    // active nodes of the scene: one atomic per wave (all lanes of the wave share the scene)
+    // active nodes of the scene, counted by lane 0 (node (4K) of the tile: always inside)
     const unsigned long long act = __ballot(m > 0.0f);
-    if (active_count && lane == __ffsll((long long)__ballot(true)) - 1 && act)
-      atomicAdd(active_count + sid, (unsigned long long)__popcll(act));
+    if (lane == 0) cnt_sum += (unsigned long long)__popcll(act);
   }
+  if (lane == 0 && cnt_sum && active_count)
+    atomicAdd(active_count + size_t(cnt_sid * ACTIVE_SLOTS + cnt_slot) * ACTIVE_STRIDE, cnt_sum);
 }
 
 // ---------------------------------------------------------------------------------------

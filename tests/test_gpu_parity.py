@@ -354,6 +354,29 @@ def test_batch_equals_single():
             assert np.array_equal(a[k], b[k]), k
 
 
+def test_active_nodes_count_the_nodes_with_mass():
+    """snow_active_nodes sums the per-scene count slots the grid update adds to; it must equal the
+    nodes with m > 0 of every scene, alone and in a batch, step after step, and on the 2^20-particle
+    scene whose grid update spreads the adds over all slots."""
+    sc = [scenes.tiny_block(n_side=8, dims=16, seed=s, velocity=(0.2 * s, -1.0, 0.1)) for s in range(3)]
+    sc[1] = scenes.tiny_block(n_side=5, dims=24, seed=9)
+    for batch in (sc[:1], sc):
+        with SnowContext(batch) as ctx:
+            ctx.init()
+            for _ in range(3):
+                ctx.step(1e-4)
+                for i in range(len(batch)):
+                    m = ctx.read_grid(i, fields=("m",))["m"]
+                    assert ctx.active_nodes(i) == int(np.count_nonzero(m > 0)), i
+    big = scenes.two_snowballs()
+    with SnowContext(big) as ctx:
+        ctx.init()
+        for _ in range(2):
+            ctx.step(big.dt)
+        m = ctx.read_grid(0, fields=("m",))["m"]
+        assert ctx.active_nodes(0) == int(np.count_nonzero(m > 0)) > 10000
+
+
 def test_readback_roundtrip_and_reference_export():
     sc = scenes.tiny_block(n_side=6, dims=16)
     with SnowContext(sc) as ctx:
