@@ -341,8 +341,8 @@ def semi_implicit(steps: int) -> dict:
     physical weight gradient (1/h; with the reference's gradient the stiffness term is h^2 smaller and
     the solve stops at its first residual), once the balls are in contact (600 explicit steps of 5e-5 s
     first), at four times the explicit dt: ms per step, solver iterations, particle-steps/s; and its
-    error against the FP64 oracle on the 64^3 snowball with a deformed F_E after 10 steps at five times
-    that scene's dt."""
+    error against the FP64 oracle on the 64^3 snowball with a deformed F_E and the same gradient after
+    10 steps at five times that scene's dt."""
     import numpy as np
     from oracle import pyoracle
     from snow_amd import SnowContext, scenes
@@ -374,8 +374,8 @@ def semi_implicit(steps: int) -> dict:
     out.update({"value": n * steps / el, "ms_per_step": el / steps * 1e3, "iterations": its,
                 "ms_per_iteration": (el - steps * el_ex / (4 * steps)) / max(1, sum(its)) * 1e3,
                 "explicit_ms_per_step": el_ex / (4 * steps) * 1e3})
-    # error against the oracle (deformed F_E so that the solve iterates)
-    small = scenes.snowball_drop()
+    # error against the oracle (deformed F_E and the physical gradient, so that the solve iterates)
+    small = scenes.snowball_drop(grad_scale=0.0)
     rng = np.random.default_rng(5)
     m = len(small.particles)
     fe = np.eye(3)[None] + 0.02 * rng.uniform(-1, 1, (m, 3, 3))
@@ -393,7 +393,7 @@ def semi_implicit(steps: int) -> dict:
         p = ctx.read_particles()
     for _ in range(ssteps):
         o_it, _ = o.step_semi_implicit(sdt, 1.0, 1e-6, 200)
-    err = {"workload": "snowball_drop_64^3, deformed F_E", "particles": m, "dt": sdt, "steps": ssteps,
+    err = {"workload": "snowball_drop_64^3, deformed F_E, gradient 1/h", "particles": m, "dt": sdt, "steps": ssteps,
            "iterations_last_step": {"gpu": g_it, "oracle": o_it}}
     for name, a, b in (("x", p["x"], o.x), ("v", p["v"], o.v), ("F_E", p["fe"], o.fe), ("F_P", p["fp"], o.fp)):
         d = np.sqrt(((np.asarray(a, np.float64) - b) ** 2).sum(-1))

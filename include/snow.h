@@ -139,10 +139,10 @@ int snow_run_graph(snow_ctx* ctx, int32_t steps);
  * before the transfer back to the particles, by the solution v of
  *   M v - beta dt df(dt v) = M v*   (Stomakhin et al. 2013, eq. 9),
  * df the Hessian action of the energy whose gradient is the explicit force. Solved per scene by
- * conjugate residuals from v*, until |r| <= tol |M v*| or max_iter direction updates. beta in [0, 1]
- * (0: the explicit step, bit for bit; 1: backward Euler in the velocities); dt > 0 for every scene,
- * or 0 for each scene's own dt. Allocates the solver's node vectors on first use (about 110 bytes
- * per grid node). */
+ * conjugate residuals preconditioned by the lumped mass, from v*, until |r| <= tol |M v*| or
+ * max_iter direction updates. beta in [0, 1] (0: the explicit step, bit for bit; 1: backward Euler
+ * in the velocities); dt > 0 for every scene, or 0 for each scene's own dt. Allocates the solver's
+ * node vectors on first use (about 130 bytes per grid node). */
 int snow_step_semi_implicit(snow_ctx* ctx, double dt, double beta, double tol, int32_t max_iter);
 /* Iterations and relative residual |r| / |M v*| of a scene's last semi-implicit solve. */
 int snow_implicit_stats(snow_ctx* ctx, int32_t scene, int32_t* iterations, double* rel_residual);

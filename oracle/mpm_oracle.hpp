@@ -639,7 +639,8 @@ template <class R> void step(Scene<R>& s, R dt) {
 //   d2Psi/dF2 : dF = 2 mu (dF - dR) + lambda (cof F : dF) cof F + lambda (J - 1) d(cof F),
 // with mu, lambda hardened by J_P as in the explicit force and V_p0 = m_p / rho_p0 (0 where the
 // explicit force skips the particle: rho0 <= 0 or |m| > 100, p2gTransfer.glsl:64-66/126).
-// K is symmetric, so it is solved by conjugate residuals (the paper's solver) from v*.
+// K is symmetric, so it is solved by conjugate residuals (the paper's solver) from v*, preconditioned
+// by the lumped mass.
 // This oracle takes the exact polar factor (scaled Newton iteration) and dR = R [omega]x with
 // (tr S I - S) omega = axial(R^T dF - dF^T R), S = R^T F; the device takes the factors of the
 // SVD the explicit force uses.
@@ -822,23 +823,31 @@ template <class R> void implicit_apply(const Scene<R>& s, R dt, R beta, const R*
       out[3 * i + k] = s.gm[i] > R(0) ? s.gm[i] * v[3 * i + k] - beta * dt * df[3 * i + k] : R(0);
 }
 
-// Conjugate residuals for K v = M v*, from v = v*, on the nodes with m > 0; v* (s.gvstar) is
-// replaced by the solution. Stops when |r| <= tol |M v*| or after max_iter updates.
+// Conjugate residuals for K v = M v*, preconditioned by the lumped mass (z = M^-1 r), from v = v*,
+// on the nodes with m > 0; v* (s.gvstar) is replaced by the solution. Stops when the residual
+// |r| <= tol |M v*| or after max_iter updates. The node masses span orders of magnitude (light
+// nodes at the surface of the material); without the preconditioner the residual criterion is met
+// before those nodes move from v*.
 template <class R> void implicit_solve(Scene<R>& s, R dt, R beta, R tol, int max_iter, int* iters, R* relres) {
   const int64_t n3 = 3 * s.nodes();
-  std::vector<R> x(s.gvstar, s.gvstar + n3), b(n3), r(n3), p(n3), Kp(n3), Kr(n3);
+  std::vector<R> x(s.gvstar, s.gvstar + n3), b(n3), r(n3), z(n3), p(n3), Kp(n3), Kz(n3), minv(n3);
   for (int64_t i = 0; i < s.nodes(); ++i)
     for (int k = 0; k < 3; ++k) {
-      x[3 * i + k] = s.gm[i] > R(0) ? x[3 * i + k] : R(0);
-      b[3 * i + k] = s.gm[i] > R(0) ? s.gm[i] * x[3 * i + k] : R(0);
+      const bool live = s.gm[i] > R(0);
+      x[3 * i + k] = live ? x[3 * i + k] : R(0);
+      b[3 * i + k] = live ? s.gm[i] * x[3 * i + k] : R(0);
+      minv[3 * i + k] = live ? R(1) / s.gm[i] : R(0);
     }
   auto dotv = [&](const std::vector<R>& a, const std::vector<R>& c) {
     R acc = 0;
     for (int64_t i = 0; i < n3; ++i) acc += a[i] * c[i];
     return acc;
   };
-  implicit_apply(s, dt, beta, x.data(), Kr.data());
-  for (int64_t i = 0; i < n3; ++i) r[i] = b[i] - Kr[i];
+  implicit_apply(s, dt, beta, x.data(), Kz.data());
+  for (int64_t i = 0; i < n3; ++i) {
+    r[i] = b[i] - Kz[i];
+    z[i] = minv[i] * r[i];
+  }
   const R bb = dotv(b, b);
   R rr = dotv(r, r);
   int it = 0;
@@ -846,16 +855,24 @@ template <class R> void implicit_solve(Scene<R>& s, R dt, R beta, R tol, int max
   std::fill(p.begin(), p.end(), R(0));
   std::fill(Kp.begin(), Kp.end(), R(0));
   while (rr > tol * tol * bb && it < max_iter) {
-    if (it > 0 && !(rho != R(0))) break;  // r . K r vanished (K is not definite): no next direction
-    implicit_apply(s, dt, beta, r.data(), Kr.data());
-    const R rho_new = dotv(r, Kr);
+    if (it > 0 && !(rho != R(0))) break;  // z . K z vanished (K is not definite): no next direction
+    implicit_apply(s, dt, beta, z.data(), Kz.data());
+    const R rho_new = dotv(z, Kz);
     const R bc = it == 0 ? R(0) : rho_new / rho;
     rho = rho_new;
-    for (int64_t i = 0; i < n3; ++i) { p[i] = r[i] + bc * p[i]; Kp[i] = Kr[i] + bc * Kp[i]; }
-    const R pp = dotv(Kp, Kp);
+    R pp = 0;  // Kp . M^-1 Kp
+    for (int64_t i = 0; i < n3; ++i) {
+      p[i] = z[i] + bc * p[i];
+      Kp[i] = Kz[i] + bc * Kp[i];
+      pp += Kp[i] * (minv[i] * Kp[i]);
+    }
     if (!(pp > R(0))) break;
     const R alpha = rho / pp;
-    for (int64_t i = 0; i < n3; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * Kp[i]; }
+    for (int64_t i = 0; i < n3; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * Kp[i];
+      z[i] -= alpha * (minv[i] * Kp[i]);
+    }
     rr = dotv(r, r);
     ++it;
   }

@@ -503,8 +503,9 @@ int ensure_implicit(snow_ctx* c) {
   return 0;
 }
 
-// Conjugate residuals for K v = M v* on the grid the grid update just wrote; the node records then
-// hold v instead of v*. Convergence is polled every IMP_POLL iterations (one small copy to the host).
+// Conjugate residuals preconditioned by the lumped mass for K v = M v* on the grid the grid update
+// just wrote; the node records then hold v instead of v*. Convergence is polled every IMP_POLL
+// iterations (one small copy to the host).
 int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
   int rc;
   if ((rc = ensure_implicit(c))) return rc;
@@ -561,7 +562,7 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
       for (int s = 0; s < c->nscenes; ++s) all = all && c->h_imp_scal[s].done;
       if (all) break;
     }
-    if ((rc = apply(1))) return rc;
+    if ((rc = apply(5))) return rc;  // K z
     k_imp_node<IMP_KR><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
     LAUNCHCHK(c);
     if ((rc = scalars(IMP_KR))) return rc;

@@ -11,10 +11,12 @@
 //   d2Psi/dF2 : dF = 2 mu (dF - dR) + lambda (cof F : dF) cof F + lambda (J - 1) d(cof F).
 // R is the rotation the explicit force takes (W V^T of the reference SVD, rounded to 1e-9);
 // R^T dR = [w]x with (tr S I - S) w = axial(R^T dF - dF^T R), S the symmetric part of R^T F.
-// K is symmetric; conjugate residuals (the paper's solver) run on the grid from v = v*, then the
-// transfer back to the particles takes v instead of v*.
+// K is symmetric; conjugate residuals (the paper's solver), preconditioned by the lumped mass
+// (z = M^-1 r), run on the grid from v = v*, then the transfer back to the particles takes v instead
+// of v*. The node masses span orders of magnitude at the surface of the material: without the
+// preconditioner the residual criterion is met before the light nodes move from v*.
 //
-// Layout: dense node vectors over the context's nodes (float4 per node: x, r, p, Kp, Kr), the node
+// Layout: dense node vectors over the context's nodes (float4 per node: x, r, p, Kp, Kz, z), the node
 // masses of this step (0 outside the node tiles the grid update wrote or where m <= 0), a dense
 // fixed-point accumulator (3 x int64 per node) for df, and per sorted particle 6 float4 planes of
 // data prepared once per step (x, gs V_p0, F_E, R, 2 mu, lambda) plus G_p = gs V_p0 A_p F_Ep^T per
@@ -29,7 +31,7 @@ namespace snow {
 
 // Per-scene solver state in device memory.
 struct ImpScal {
-  double rho;    // r . K r of the current update
+  double rho;    // z . K z of the current update
   double alpha;  // step length of the current update
   double beta;   // weight of the previous direction in the current update
   double bb;     // |M v*|^2
@@ -49,7 +51,7 @@ enum ImpPhase { IMP_INIT = 0, IMP_KR = 1, IMP_P = 2, IMP_X = 3 };
 constexpr int IMP_THREADS = 256;
 constexpr int IMP_NODES_PER_THREAD = 4;
 constexpr int IMP_NODES_PER_BLOCK = IMP_THREADS * IMP_NODES_PER_THREAD;
-constexpr int IMP_VECS = 5;       // x, r, p, Kp, Kr
+constexpr int IMP_VECS = 6;       // x, r, p, Kp, Kz, z
 // float4 planes of prepared particle data: (x, y, z, gs V_p0), (F0..F3), (F4..F7), (F8, R0, R1, R2),
 // (R3..R6), (R7, R8, 2 mu, lambda); F and R column-major
 constexpr int IMP_PLANES = 6;
@@ -625,11 +627,11 @@ __device__ __forceinline__ void imp_block_sum(double v[3], int nq, double* __res
     }
 }
 
-// Node phases of the solve (K = M v - beta dt df, df = -acc 2^-e):
-//   IMP_INIT: vin = x = v*: r = M v* - K x, p = Kp = 0; partials |M v*|^2, |r|^2
-//   IMP_KR:   vin = r:      Kr = K r; partial r . Kr
-//   IMP_P:    p = r + beta p, Kp = Kr + beta Kp; partial Kp . Kp
-//   IMP_X:    x += alpha p, r -= alpha Kp; partial |r|^2
+// Node phases of the solve (K = M v - beta dt df, df = -acc 2^-e; z = r / m):
+//   IMP_INIT: vin = x = v*: r = M v* - K x, z = r / m, p = Kp = 0; partials |M v*|^2, |r|^2
+//   IMP_KR:   vin = z:      Kz = K z; partial z . Kz
+//   IMP_P:    p = z + beta p, Kp = Kz + beta Kp; partial Kp . Kp / m
+//   IMP_X:    x += alpha p, r -= alpha Kp, z -= alpha Kp / m; partial |r|^2
 template <int PHASE>
 __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __restrict__ scenes,
                                                           const ImpBlock* __restrict__ blocks, uint32_t nblocks,
@@ -646,7 +648,8 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
   float4* Rv = vec + total_nodes;
   float4* P = vec + 2 * total_nodes;
   float4* KP = vec + 3 * total_nodes;
-  float4* KR = vec + 4 * total_nodes;
+  float4* KZ = vec + 4 * total_nodes;
+  float4* ZV = vec + 5 * total_nodes;
   const double inv = (PHASE == IMP_INIT || PHASE == IMP_KR) ? ldexp(1.0, -imp_exponent(imp_gmax(gslots, B.scene), S.p_cnt)) : 0.0;
   const float bdt = Z.beta_dt;
   const float al = float(Z.alpha), be = float(Z.beta);
@@ -667,36 +670,41 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
       acc[3 * nd + 1] = 0;
       acc[3 * nd + 2] = 0;
       if (PHASE == IMP_INIT) {
-        float4 r = make_float4(0.f, 0.f, 0.f, 0.f), b = r;
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f), b = r, z = r;
         if (m > 0.0f) {
           const float4 x = X[nd];
           b = make_float4(m * x.x, m * x.y, m * x.z, 0.f);
           const float kx = fmaf(bdt, a0, b.x), ky = fmaf(bdt, a1, b.y), kz = fmaf(bdt, a2, b.z);
           r = make_float4(b.x - kx, b.y - ky, b.z - kz, 0.f);
+          z = make_float4(r.x / m, r.y / m, r.z / m, 0.f);
         }
         Rv[nd] = r;
+        ZV[nd] = z;
         P[nd] = make_float4(0.f, 0.f, 0.f, 0.f);
         KP[nd] = make_float4(0.f, 0.f, 0.f, 0.f);
-        KR[nd] = make_float4(0.f, 0.f, 0.f, 0.f);
+        KZ[nd] = make_float4(0.f, 0.f, 0.f, 0.f);
         sum[0] += double(b.x) * b.x + double(b.y) * b.y + double(b.z) * b.z;
         sum[1] += double(r.x) * r.x + double(r.y) * r.y + double(r.z) * r.z;
       } else {
-        const float4 r = Rv[nd];
-        const float4 kr = make_float4(fmaf(bdt, a0, m * r.x), fmaf(bdt, a1, m * r.y), fmaf(bdt, a2, m * r.z), 0.f);
-        KR[nd] = kr;
-        sum[0] += double(r.x) * kr.x + double(r.y) * kr.y + double(r.z) * kr.z;
+        const float4 z = ZV[nd];
+        const float4 kz = make_float4(fmaf(bdt, a0, m * z.x), fmaf(bdt, a1, m * z.y), fmaf(bdt, a2, m * z.z), 0.f);
+        KZ[nd] = kz;
+        sum[0] += double(z.x) * kz.x + double(z.y) * kz.y + double(z.z) * kz.z;
       }
     } else if (PHASE == IMP_P) {
-      const float4 r = Rv[nd], kr = KR[nd], p = P[nd], kp = KP[nd];
-      const float4 pn = make_float4(fmaf(be, p.x, r.x), fmaf(be, p.y, r.y), fmaf(be, p.z, r.z), 0.f);
-      const float4 kpn = make_float4(fmaf(be, kp.x, kr.x), fmaf(be, kp.y, kr.y), fmaf(be, kp.z, kr.z), 0.f);
+      const float4 z = ZV[nd], kz = KZ[nd], p = P[nd], kp = KP[nd];
+      const float4 pn = make_float4(fmaf(be, p.x, z.x), fmaf(be, p.y, z.y), fmaf(be, p.z, z.z), 0.f);
+      const float4 kpn = make_float4(fmaf(be, kp.x, kz.x), fmaf(be, kp.y, kz.y), fmaf(be, kp.z, kz.z), 0.f);
       P[nd] = pn;
       KP[nd] = kpn;
-      sum[0] += double(kpn.x) * kpn.x + double(kpn.y) * kpn.y + double(kpn.z) * kpn.z;
+      const double im = 1.0 / double(m);
+      sum[0] += (double(kpn.x) * kpn.x + double(kpn.y) * kpn.y + double(kpn.z) * kpn.z) * im;
     } else {
-      const float4 p = P[nd], kp = KP[nd], x = X[nd], r = Rv[nd];
+      const float4 p = P[nd], kp = KP[nd], x = X[nd], r = Rv[nd], z = ZV[nd];
       X[nd] = make_float4(fmaf(al, p.x, x.x), fmaf(al, p.y, x.y), fmaf(al, p.z, x.z), 0.f);
       const float4 rn = make_float4(fmaf(-al, kp.x, r.x), fmaf(-al, kp.y, r.y), fmaf(-al, kp.z, r.z), 0.f);
+      const float am = al / m;
+      ZV[nd] = make_float4(fmaf(-am, kp.x, z.x), fmaf(-am, kp.y, z.y), fmaf(-am, kp.z, z.z), 0.f);
       Rv[nd] = rn;
       sum[0] += double(rn.x) * rn.x + double(rn.y) * rn.y + double(rn.z) * rn.z;
     }
@@ -752,7 +760,7 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const ui
     Z.rr = t1;
     Z.done = (t1 <= Z.tol2 * t0 || Z.max_iter <= 0) ? 1 : 0;
   } else if (phase == IMP_KR) {
-    if (Z.iters > 0 && !(Z.rho != 0.0)) {  // r . K r vanished (K is not definite): no next direction
+    if (Z.iters > 0 && !(Z.rho != 0.0)) {  // z . K z vanished (K is not definite): no next direction
       Z.done = 1;
       return;
     }

@@ -46,17 +46,20 @@ def test_beta_zero_is_the_explicit_step():
         assert np.array_equal(pa[k], pb[k]), k
 
 
-@pytest.mark.parametrize("beta", [1.0, 0.5])
-def test_one_step_teacher_forced(oracle_mod, beta):
+@pytest.mark.parametrize("beta,grad_scale", [(1.0, 1.0), (0.5, 1.0), (1.0, 0.0), (0.5, 0.0)])
+def test_one_step_teacher_forced(oracle_mod, beta, grad_scale):
     """From the same deformed state, one step at ten times the scene's dt: the GPU against the oracle,
-    and the implicit correction (semi-implicit minus explicit) well above that difference."""
-    sc = deformed(scenes.snowball_drop())
+    and the implicit correction (semi-implicit minus explicit) well above that difference. With the
+    reference's weight gradient (grad_scale 1) the stiffness term is h^2 smaller than with the
+    physical 1/h (grad_scale 0), whose solve takes several iterations."""
+    sc = deformed(scenes.snowball_drop(grad_scale=grad_scale))
     sc.particles.v[:] = np.random.default_rng(3).normal(0, 0.5, sc.particles.v.shape).astype(np.float32)
     dt, tol = 1e-3, 1e-6
     o = oracle_mod.Oracle(sc)
     o.init()
     o.rho0[:] = o.rho0.astype(np.float32)
-    print(f"\nscene: snowball drop, {len(sc.particles)} particles, dt {dt}, beta {beta}", flush=True)
+    print(f"\nscene: snowball drop, {len(sc.particles)} particles, dt {dt}, beta {beta}, grad_scale {grad_scale}",
+          flush=True)
     with SnowContext(sc) as ctx, SnowContext(sc) as ex:
         for c in (ctx, ex):
             c.init()
@@ -80,9 +83,10 @@ def test_one_step_teacher_forced(oracle_mod, beta):
     assert corr > 20 * dv
 
 
-def test_free_running_against_oracle(oracle_mod):
+@pytest.mark.parametrize("grad_scale", [1.0, 0.0])
+def test_free_running_against_oracle(oracle_mod, grad_scale):
     """20 semi-implicit steps of the deformed snowball at 5x its dt, free running."""
-    sc = deformed(scenes.snowball_drop(), amp=0.02)
+    sc = deformed(scenes.snowball_drop(grad_scale=grad_scale), amp=0.02)
     dt, steps = 5e-4, 20
     o = oracle_mod.Oracle(sc, nthreads=8)
     o.init()
@@ -93,14 +97,19 @@ def test_free_running_against_oracle(oracle_mod):
         for _ in range(steps):
             ctx.step_semi_implicit(dt, beta=1.0, tol=1e-6, max_iter=200)
         ctx.check_finite()
+        ctx_it = ctx.implicit_stats()[0]
         g = ctx.read_particles()
     for _ in range(steps):
         o.step_semi_implicit(dt, 1.0, 1e-6, 200)
     dx, dv = np.abs(g["x"] - o.x).max(), rel_err(g["v"], o.v)
     dfe = np.abs(g["fe"] - o.fe).max()
-    print(f"\n{steps} steps: x {dx:.2e}  v rel {dv:.2e}  F_E {dfe:.2e}")
+    print(f"\n{steps} steps (grad_scale {grad_scale}, last solve {ctx_it} iterations): x {dx:.2e}  v rel {dv:.2e}  "
+          f"F_E {dfe:.2e}")
     assert dx < 1e-5
-    assert dv < 1e-3
+    # FP32 against FP64 (and the reference SVD's rotation against the exact polar factor) drift apart
+    # over the free-running steps; with the physical gradient the system is stiffer and the drift
+    # of v larger
+    assert dv < (1e-3 if grad_scale == 1.0 else 5e-3)
     assert dfe < 1e-3
 
 
