@@ -159,6 +159,8 @@ struct snow_ctx {
   uint32_t* d_imp_gslots = nullptr;      // IMP_GSLOTS bound slots per scene, IMP_GSTRIDE words apart
   ImpScal* h_imp_scal = nullptr;         // pinned host copy for the convergence polls
   uint32_t imp_nblocks = 0;
+  uint8_t* d_imp_nlive = nullptr;        // per node tile: written by this step's grid update
+  uint8_t* d_imp_blive = nullptr;        // per node-vector block: holds a node of such a tile
 };
 
 namespace {
@@ -476,13 +478,15 @@ int ensure_implicit(snow_ctx* c) {
   auto bad = [&](hipError_t e) {
     if (e == hipSuccess) return false;
     void* ptrs[] = {c->d_imp_vec, c->d_imp_mass, c->d_imp_acc, c->d_imp_pd, c->d_imp_G, c->d_imp_part,
-                    c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal, c->d_imp_gslots};
+                    c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal, c->d_imp_gslots,
+                    c->d_imp_nlive, c->d_imp_blive};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (c->h_imp_scal) (void)hipHostFree(c->h_imp_scal);
     c->d_imp_vec = nullptr; c->d_imp_mass = nullptr; c->d_imp_acc = nullptr; c->d_imp_pd = nullptr;
     c->d_imp_G = nullptr; c->d_imp_part = nullptr; c->d_imp_blocks = nullptr; c->d_imp_scene_blocks = nullptr;
     c->d_imp_scal = nullptr; c->h_imp_scal = nullptr; c->d_imp_gslots = nullptr;
+    c->d_imp_nlive = nullptr; c->d_imp_blive = nullptr;
     return true;
   };
   if (bad(dalloc(&c->d_imp_vec, nn * IMP_VECS)) || bad(dalloc(&c->d_imp_mass, nn)) ||
@@ -491,9 +495,14 @@ int ensure_implicit(snow_ctx* c) {
       bad(dalloc(&c->d_imp_blocks, std::max<size_t>(1, blocks.size()))) ||
       bad(dalloc(&c->d_imp_scene_blocks, first.size())) || bad(dalloc(&c->d_imp_scal, size_t(c->nscenes))) ||
       bad(dalloc(&c->d_imp_gslots, size_t(c->nscenes) * IMP_GSLOTS * IMP_GSTRIDE)) ||
+      bad(dalloc(&c->d_imp_nlive, std::max<size_t>(1, c->total_ntiles))) ||
+      bad(dalloc(&c->d_imp_blive, std::max<size_t>(1, blocks.size()))) ||
       bad(hipHostMalloc(reinterpret_cast<void**>(&c->h_imp_scal), sizeof(ImpScal) * c->nscenes)))
     return fail(c, SNOW_E_CAPACITY, "semi-implicit solve: no device memory for its node vectors");
   HIPCHK(c, hipMemsetAsync(c->d_imp_acc, 0, sizeof(long long) * 3 * nn, c->stream));
+  // vectors and masses start at zero: nodes outside the written tiles are never written by the solve
+  HIPCHK(c, hipMemsetAsync(c->d_imp_vec, 0, sizeof(float4) * nn * IMP_VECS, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->d_imp_mass, 0, sizeof(float) * nn, c->stream));
   if (!blocks.empty())
     HIPCHK(c, hipMemcpyAsync(c->d_imp_blocks, blocks.data(), sizeof(ImpBlock) * blocks.size(),
                              hipMemcpyHostToDevice, c->stream));
@@ -537,7 +546,7 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
     return 0;
   };
 #define NODE_ARGS c->d_scenes, c->d_imp_blocks, nb, c->d_imp_mass, c->d_imp_vec, c->d_imp_acc, c->d_imp_scal, nn, \
-                  c->d_imp_part, c->d_imp_gslots
+                  c->d_imp_part, c->d_imp_gslots, c->d_imp_blive
   if ((rc = scalars(-1))) return rc;
   if (c->N) {
     k_imp_prep<<<pg, 256, 0, c->stream>>>(c->d_scenes, waves, c->nwaves, c->d_slot, c->d_rec[c->cur],
@@ -545,9 +554,13 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
     LAUNCHCHK(c);
   }
   if (nb == 0) return 0;
-  k_imp_setup<<<nb, IMP_THREADS, 0, c->stream>>>(c->d_scenes, c->d_imp_blocks, c->d_ctl, c->d_tile_stamp,
-                                                 c->d_g_node, c->d_g_mass, c->d_imp_mass, c->d_imp_vec,
-                                                 c->d_imp_acc, nn);
+  k_imp_ntiles<<<grid_for(c->total_ntiles, 256), 256, 0, c->stream>>>(
+      c->d_scenes, c->nscenes > 1 ? c->d_ntile_scene : nullptr, c->total_ntiles, c->d_ctl, c->d_tile_stamp,
+      c->d_imp_nlive);
+  LAUNCHCHK(c);
+  k_imp_setup<<<nb, IMP_THREADS, 0, c->stream>>>(c->d_scenes, c->d_imp_blocks, c->d_imp_nlive, c->d_g_node,
+                                                 c->d_g_mass, c->d_imp_mass, c->d_imp_vec, c->d_imp_acc,
+                                                 c->d_imp_blive);
   LAUNCHCHK(c);
   if ((rc = apply(0))) return rc;
   k_imp_node<IMP_INIT><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
@@ -575,7 +588,7 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
   }
 #undef NODE_ARGS
   k_imp_finish<<<nb, IMP_THREADS, 0, c->stream>>>(c->d_scenes, c->d_imp_blocks, c->d_imp_mass, c->d_imp_vec,
-                                                  c->d_g_node, c->d_g_mass);
+                                                  c->d_g_node, c->d_g_mass, c->d_imp_blive);
   LAUNCHCHK(c);
   return 0;
 }
@@ -624,7 +637,8 @@ void release(snow_ctx* c) {
                   c->d_tile_c0, c->d_tile_c1, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_fold_job, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage,
                   c->d_imp_vec, c->d_imp_mass, c->d_imp_acc, c->d_imp_pd, c->d_imp_G, c->d_imp_part,
-                  c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal, c->d_imp_gslots};
+                  c->d_imp_blocks, c->d_imp_scene_blocks, c->d_imp_scal, c->d_imp_gslots, c->d_imp_nlive,
+                  c->d_imp_blive};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_imp_scal) (void)hipHostFree(c->h_imp_scal);

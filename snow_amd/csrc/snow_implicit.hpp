@@ -137,31 +137,61 @@ __global__ void __launch_bounds__(256) k_imp_prep(const SceneDev* __restrict__ s
 }
 
 // ---------------------------------------------------------------------------------------
+// Node tiles the grid update wrote this step: node tile K is written when one of the cell tiles
+// K + {0,1}^3 holds particles (the rule of k_grid). One thread per node tile of the context.
+__global__ void __launch_bounds__(256) k_imp_ntiles(const SceneDev* __restrict__ scenes,
+                                                    const uint16_t* __restrict__ ntile_scene, uint32_t total_ntiles,
+                                                    const StepCtl* __restrict__ ctl,
+                                                    const uint32_t* __restrict__ tile_stamp,
+                                                    uint8_t* __restrict__ nlive) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total_ntiles) return;
+  const SceneDev& S = scenes[ntile_scene ? ntile_scene[t] : 0];
+  const uint32_t l = t - S.nt_off;
+  const int Kx = int(l % uint32_t(S.nt[0])), Ky = int((l / uint32_t(S.nt[0])) % uint32_t(S.nt[1]));
+  const int Kz = int(l / uint32_t(S.nt[0] * S.nt[1]));
+  const uint32_t step = ctl->step;
+  bool live = false;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) live |= tile_live(S, tile_stamp, step, Kx + (b & 1), Ky + ((b >> 1) & 1), Kz + (b >> 2));
+  nlive[t] = live ? 1 : 0;
+}
+
 // Node setup: the mass of every node of a node tile the grid update wrote this step (0 elsewhere),
-// x = v* where m > 0 (0 elsewhere), cleared accumulator.
+// x = v* where m > 0 (0 elsewhere), cleared accumulator; and the block's flag: a block with no such
+// node is skipped by every later kernel of the solve, which reads nothing of it (the gather and the
+// scatter reach only nodes of written tiles).
 __global__ void __launch_bounds__(IMP_THREADS) k_imp_setup(const SceneDev* __restrict__ scenes,
                                                            const ImpBlock* __restrict__ blocks,
-                                                           const StepCtl* __restrict__ ctl,
-                                                           const uint32_t* __restrict__ tile_stamp,
+                                                           const uint8_t* __restrict__ nlive,
                                                            const float2* __restrict__ g_node,
                                                            const float* __restrict__ g_mass,
                                                            float* __restrict__ mass, float4* __restrict__ vec,
-                                                           long long* __restrict__ acc, int64_t total_nodes) {
+                                                           long long* __restrict__ acc, uint8_t* __restrict__ blive) {
   const ImpBlock B = blocks[blockIdx.x];
   const SceneDev& S = scenes[B.scene];
   const int64_t end = S.node_off + int64_t(S.D[0]) * S.D[1] * S.D[2];
-  const uint32_t step = ctl->step;
+  bool live[IMP_NODES_PER_THREAD];
+  int any = 0;
+#pragma unroll
+  for (int j = 0; j < IMP_NODES_PER_THREAD; ++j) {
+    const int64_t n = B.first + j * IMP_THREADS + threadIdx.x;
+    live[j] = false;
+    if (n < end) {
+      const int64_t l = n - S.node_off;
+      const int nx = int(l % S.D[0]), ny = int((l / S.D[0]) % S.D[1]), nz = int(l / (int64_t(S.D[0]) * S.D[1]));
+      live[j] = nlive[S.nt_off + uint32_t(nx / TILE + S.nt[0] * (ny / TILE + S.nt[1] * (nz / TILE)))] != 0;
+      any |= live[j] ? 1 : 0;
+    }
+  }
+  const int blk = __syncthreads_or(any);
+  if (threadIdx.x == 0) blive[blockIdx.x] = blk ? 1 : 0;
+  if (!blk) return;
 #pragma unroll
   for (int j = 0; j < IMP_NODES_PER_THREAD; ++j) {
     const int64_t n = B.first + j * IMP_THREADS + threadIdx.x;
     if (n >= end) break;
-    const int64_t l = n - S.node_off;
-    const int nx = int(l % S.D[0]), ny = int((l / S.D[0]) % S.D[1]), nz = int(l / (int64_t(S.D[0]) * S.D[1]));
-    const int Kx = nx / TILE, Ky = ny / TILE, Kz = nz / TILE;
-    bool live = false;
-#pragma unroll
-    for (int b = 0; b < 8; ++b) live |= tile_live(S, tile_stamp, step, Kx + (b & 1), Ky + ((b >> 1) & 1), Kz + (b >> 2));
-    float m = live ? g_mass[n] : 0.0f;
+    float m = live[j] ? g_mass[n] : 0.0f;
     m = m > 0.0f ? m : 0.0f;
     float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
     if (m > 0.0f) {
@@ -174,7 +204,6 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_setup(const SceneDev* __res
     acc[3 * n + 1] = 0;
     acc[3 * n + 2] = 0;
   }
-  (void)total_nodes;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -638,10 +667,15 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
                                                           const float* __restrict__ mass, float4* __restrict__ vec,
                                                           long long* __restrict__ acc,
                                                           const ImpScal* __restrict__ scal, int64_t total_nodes,
-                                                          double* __restrict__ part, const uint32_t* __restrict__ gslots) {
+                                                          double* __restrict__ part, const uint32_t* __restrict__ gslots,
+                                                          const uint8_t* __restrict__ blive) {
   const ImpBlock B = blocks[blockIdx.x];
   const ImpScal& Z = scal[B.scene];
   if (PHASE != IMP_INIT && Z.done) return;  // the scene's partials are not read again
+  if (!blive[blockIdx.x]) {  // no node of a written tile: zero partials
+    if (threadIdx.x < (PHASE == IMP_INIT ? 2 : 1)) part[size_t(threadIdx.x) * nblocks + blockIdx.x] = 0.0;
+    return;
+  }
   const SceneDev& S = scenes[B.scene];
   const int64_t end = S.node_off + int64_t(S.D[0]) * S.D[1] * S.D[2];
   float4* X = vec;
@@ -781,7 +815,9 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_finish(const SceneDev* __re
                                                             const ImpBlock* __restrict__ blocks,
                                                             const float* __restrict__ mass,
                                                             const float4* __restrict__ vec,
-                                                            float2* __restrict__ g_node, float* __restrict__ g_mass) {
+                                                            float2* __restrict__ g_node, float* __restrict__ g_mass,
+                                                            const uint8_t* __restrict__ blive) {
+  if (!blive[blockIdx.x]) return;
   const ImpBlock B = blocks[blockIdx.x];
   const SceneDev& S = scenes[B.scene];
   const int64_t end = S.node_off + int64_t(S.D[0]) * S.D[1] * S.D[2];
