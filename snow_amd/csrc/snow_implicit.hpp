@@ -48,7 +48,7 @@ struct ImpScal {
 
 enum ImpPhase { IMP_INIT = 0, IMP_KR = 1, IMP_P = 2, IMP_X = 3 };
 
-constexpr int IMP_THREADS = 256;
+constexpr int IMP_THREADS = 1024;
 constexpr int IMP_NODES_PER_THREAD = 4;
 constexpr int IMP_NODES_PER_BLOCK = IMP_THREADS * IMP_NODES_PER_THREAD;
 constexpr int IMP_VECS = 6;       // x, r, p, Kp, Kz, z
