@@ -1335,9 +1335,9 @@ __device__ __forceinline__ bool needs_fp64(const float d[3], float crit_c, float
 }
 
 // ---------------------------------------------------------------------------------------
-// Waves per SIMD the G2P register allocation aims at: 4 for the FP32 kernel (128 VGPRs), 3 with
-// the FP64 path compiled in (168 VGPRs: at 4 waves it spills in the hot path, at 2 waves latency
-// is exposed).
+// Least waves per SIMD the G2P register allocation must keep: 4 for the FP32 kernel (at most 128
+// VGPRs), 3 with the FP64 path compiled in (at most 168 VGPRs, so that the FP64 branch never
+// spills; the compiler currently fits it in 128).
 template <bool MIXED> struct G2PWaves { static constexpr int value = MIXED ? 3 : 4; };
 
 // Fused G2P + particle advance (g2pTransfer.glsl:53-143, particleAdvance.glsl:75-217).
@@ -1411,7 +1411,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G2PWav
     float FEo[9], FPo[9], A[9];
     float W[9], V[9], sv[3];
     deform_update<float>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, W, V, sv);
-    if (MIXED && needs_fp64(sv, S.crit_comp, S.crit_stretch)) {
+    const bool use64 = MIXED && needs_fp64(sv, S.crit_comp, S.crit_stretch);
+    if (use64) {
       double Wd[9], Vd[9], svd[3], FEd[9], FPd[9], Ad[9];
       deform_update<double>(dvp, dt, FE, FP, S.crit_comp, S.crit_stretch, FEo, FPo, Wd, Vd, svd);
 #pragma unroll
@@ -1420,9 +1421,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G2PWav
                               double(S.hardening), double(S.gscale), Ad);
 #pragma unroll
       for (int e = 0; e < 9; ++e) A[e] = float(Ad[e]);
-      if (fp64_count) atomicAdd(fp64_count, 1ull);
     } else {
       force_matrix_wv<float>(FEo, FPo, W, V, xm.w, vr.w, S.mu0, S.lambda0, S.hardening, S.gscale, A);
+    }
+    if (MIXED && fp64_count) {  // one add per wave: its lanes that took the FP64 path
+      const unsigned long long b64 = __ballot(use64);
+      if (b64 && int(threadIdx.x & 63) == __ffsll((long long)__ballot(true)) - 1)
+        atomicAdd(fp64_count, (unsigned long long)__popcll(b64));
     }
 #pragma unroll
     for (int e = 0; e < 9; ++e) { st_plane(fe_out + size_t(e) * n, i, FEo[e]); st_plane(fp_out + size_t(e) * n, i, FPo[e]); }
