@@ -56,6 +56,7 @@ constexpr int IMP_VECS = 6;       // x, r, p, Kp, Kz, z
 // (R3..R6), (R7, R8, 2 mu, lambda); F and R column-major
 constexpr int IMP_PLANES = 6;
 constexpr int IMP_STAGE = 36;     // LDS floats per staged particle in the scatter
+constexpr int IMP_BATCH = 16;     // particles a scatter wave stages at a time
 constexpr int IMP_MAX_DIM = 1021; // largest grid dimension: base cells + 2 fit 10 bits per axis
 
 // A node-vector block: scene and first node (the blocks of a scene are consecutive).
@@ -406,10 +407,11 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
 // ---------------------------------------------------------------------------------------
 // Second half: acc_i += sum_p G_p grad w_ip (unscaled grad w: G holds gs) in fixed point. A workgroup
 // takes 4 * WPW consecutive entries of the wave table, WPW per wave. For each entry (64 sorted
-// positions of one scene) the wave stages 32 particles at a time in LDS (lane k: particle k's
+// positions of one scene) the wave stages IMP_BATCH particles at a time in LDS (lane k: particle k's
 // weights and G), then lane l is the stencil node getIJK(l) and sums the staged particles in
 // registers while the base cell stays the same, adding the rounded sum where it changes (a scalar
-// mask of the changes, taken before the loop). The wave loads the data of all its entries first.
+// mask of the changes, taken before the loop). The positions of all the wave's entries are loaded
+// first, G entry by entry (fewer live registers).
 // When the workgroup's entries are of one scene and their stencils cover at most IMP_ACC_BOX nodes,
 // the sums go into an LDS box first (integer LDS atomics) and the box is added to acc once;
 // otherwise straight to acc.
@@ -420,7 +422,7 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
                                                      uint32_t n, unsigned long long* __restrict__ acc,
                                                      const ImpScal* __restrict__ scal,
                                                      const uint32_t* __restrict__ gslots) {
-  __shared__ __attribute__((aligned(16))) float s_stage[4][32 * IMP_STAGE];
+  __shared__ __attribute__((aligned(16))) float s_stage[4][IMP_BATCH * IMP_STAGE];
   __shared__ unsigned long long s_acc[3 * IMP_ACC_BOX];
   __shared__ int s_ext[6];  // box min (x, y, z), max (x, y, z) of the base cells + 2
   // wave index, scene and loop bounds are wave-uniform (scalar registers, scalar loads and branches)
@@ -435,29 +437,22 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
   const size_t N = n;
   // this lane's particle of each of the wave's entries: grid coordinate and base cell + 2 (10 bits
   // per axis; -1: none, out of the grid or the scene's solve is done)
-  // (all loads of the wave are issued here, before any wait)
+  // (the positions of all entries load together, before any wait)
   float ux[WPW], uy[WPW], uz[WPW];
   int cell[WPW];
-  float4 xq[WPW], ga[WPW], gb[WPW];
-  float gc[WPW];
+  float4 xq[WPW];
   bool live[WPW];
 #pragma unroll
   for (int j = 0; j < WPW; ++j) {
     const uint32_t e = e0 + uint32_t(wv * WPW + j);
-    xq[j] = ga[j] = gb[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    gc[j] = 0.f;
+    xq[j] = make_float4(0.f, 0.f, 0.f, 0.f);
     live[j] = false;
     if (e < nwaves) {
       const uint2 ws = waves ? waves[e] : make_uint2(64u * e, 0u);
       const SceneDev& S = scenes[ws.y];
       const uint32_t i = ws.x + uint32_t(lane);
       live[j] = !scal[ws.y].done && int64_t(i) < S.p_off + S.p_cnt;
-      if (live[j]) {
-        xq[j] = pd[i];
-        ga[j] = reinterpret_cast<const float4*>(G)[i];
-        gb[j] = reinterpret_cast<const float4*>(G)[N + i];
-        gc[j] = G[8 * N + i];
-      }
+      if (live[j]) xq[j] = pd[i];
     }
   }
   int cmin[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, cmax[3] = {-1, -1, -1};
@@ -563,8 +558,16 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
     const uint32_t cnt = uint32_t(min<int64_t>(64, Sj.p_off + Sj.p_cnt - int64_t(ws.x)));
     const int cj = cell[j];
     const int prevc = __shfl_up(cj, 1, 64);
-    for (uint32_t b0 = 0; b0 < cnt; b0 += 32u) {
-      const uint32_t bc = min(32u, cnt - b0);
+    float4 gaj = make_float4(0.f, 0.f, 0.f, 0.f), gbj = gaj;
+    float gcj = 0.f;
+    if (cj >= 0) {
+      const uint32_t i = ws.x + uint32_t(lane);
+      gaj = reinterpret_cast<const float4*>(G)[i];
+      gbj = reinterpret_cast<const float4*>(G)[N + i];
+      gcj = G[8 * N + i];
+    }
+    for (uint32_t b0 = 0; b0 < cnt; b0 += uint32_t(IMP_BATCH)) {
+      const uint32_t bc = min(uint32_t(IMP_BATCH), cnt - b0);
       const bool mine = uint32_t(lane) >= b0 && uint32_t(lane) < b0 + bc;
       if (mine && cj >= 0) {
         // (w_o, q_o) interleaved for x and y, then w_0..3 and q_0..3 for z, then G
@@ -576,9 +579,9 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
         d4[3] = make_float4(spline_w(2, fy), spline_q(2, fy), spline_w(3, fy), spline_q(3, fy));
         d4[4] = make_float4(spline_w(0, fz), spline_w(1, fz), spline_w(2, fz), spline_w(3, fz));
         d4[5] = make_float4(spline_q(0, fz), spline_q(1, fz), spline_q(2, fz), spline_q(3, fz));
-        d4[6] = ga[j];
-        d4[7] = gb[j];
-        d4[8] = make_float4(gc[j], 0.f, 0.f, 0.f);
+        d4[6] = gaj;
+        d4[7] = gbj;
+        d4[8] = make_float4(gcj, 0.f, 0.f, 0.f);
       }
       __builtin_amdgcn_wave_barrier();  // the wave's own staging writes complete before its reads
       // bit k: staged particle k has another cell than the one before it (bit 0: than cur)
@@ -590,7 +593,7 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
           if (cur >= 0) commit();
           cur = __builtin_amdgcn_readlane(cj, int(b0 + k));
         }
-        const uint32_t rest = k + 1u < 32u ? chg >> (k + 1u) : 0u;
+        const uint32_t rest = k + 1u < uint32_t(IMP_BATCH) ? chg >> (k + 1u) : 0u;
         const uint32_t kend = rest ? min(bc, k + 1u + uint32_t(__builtin_ctz(rest))) : bc;
         if (cur >= 0) {
           for (uint32_t kk = k; kk < kend; kk += 4u) {
