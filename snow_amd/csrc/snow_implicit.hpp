@@ -224,13 +224,29 @@ constexpr int IMP_BOX = 384;       // nodes of a wave's LDS gather box (as the G
 constexpr int IMP_ACC_BOX = 768;   // nodes of a workgroup's LDS accumulation box
 constexpr int IMP_WPW = 4;         // wave-table entries (64 sorted positions each) per scatter wave
 
+// One row of 4 nodes along x of the gather L += sum v grad(w)^T (column c = sum v dw/dx_c): the row
+// is reduced with the x weights first (sx = sum qx v, sw = sum wx v), then
+// grad(w) = (qx wy wz, wx qy wz, wx wy qz) takes the row's y/z factors.
+__device__ __forceinline__ void imp_gather_row(const float4 v[4], const float wx[4], const float qx[4], float wyz,
+                                               float qyz, float wqz, float L[9]) {
+  float sx0 = 0.f, sx1 = 0.f, sx2 = 0.f, sw0 = 0.f, sw1 = 0.f, sw2 = 0.f;
+#pragma unroll
+  for (int ox = 0; ox < 4; ++ox) {
+    sx0 = fmaf(qx[ox], v[ox].x, sx0); sx1 = fmaf(qx[ox], v[ox].y, sx1); sx2 = fmaf(qx[ox], v[ox].z, sx2);
+    sw0 = fmaf(wx[ox], v[ox].x, sw0); sw1 = fmaf(wx[ox], v[ox].y, sw1); sw2 = fmaf(wx[ox], v[ox].z, sw2);
+  }
+  L[0] = fmaf(sx0, wyz, L[0]); L[1] = fmaf(sx1, wyz, L[1]); L[2] = fmaf(sx2, wyz, L[2]);
+  L[3] = fmaf(sw0, qyz, L[3]); L[4] = fmaf(sw1, qyz, L[4]); L[5] = fmaf(sw2, qyz, L[5]);
+  L[6] = fmaf(sw0, wqz, L[6]); L[7] = fmaf(sw1, wqz, L[7]); L[8] = fmaf(sw2, wqz, L[8]);
+}
+
 // First half of K applied to the node vector vin: per particle dF = dt (sum_j vin_j grad w_jp^T) F,
 // A = d2Psi/dF2 : dF and G = gs V0 A F^T (planes G0..G3, G4..G7 as float4, G8 as float; G
 // column-major), and the scene's largest row sum of |G| for the fixed-point scale. The particle's
 // prepared data is loaded first, so that it arrives while the wave gathers. A full wave of interior
 // particles gathers from an LDS copy of the node box its stencils cover (sorted particles span a
 // few cells; all of its loads are in flight together), else from global memory.
-__global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ scenes, const uint2* __restrict__ waves,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_imp_apply(const SceneDev* __restrict__ scenes, const uint2* __restrict__ waves,
                                                    uint32_t nwaves, const float4* __restrict__ pd, uint32_t n,
                                                    const float4* __restrict__ vin, float* __restrict__ G,
                                                    const ImpScal* __restrict__ scal, uint32_t* __restrict__ gslots) {
@@ -298,16 +314,11 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
         for (int oz = 0; oz < 4; ++oz)
 #pragma unroll
           for (int oy = 0; oy < 4; ++oy) {
-            const float wyz = wy[oy] * wz[oz], qyz = qy[oy] * wz[oz], wqz = wy[oy] * qz[oz];
             const float4* row = p0 + ex * (oy + ey * oz);
+            float4 v4[4];
 #pragma unroll
-            for (int ox = 0; ox < 4; ++ox) {
-              const float4 v = row[ox];
-              const float g0 = qx[ox] * wyz, g1 = wx[ox] * qyz, g2 = wx[ox] * wqz;
-              L[0] = fmaf(v.x, g0, L[0]); L[1] = fmaf(v.y, g0, L[1]); L[2] = fmaf(v.z, g0, L[2]);
-              L[3] = fmaf(v.x, g1, L[3]); L[4] = fmaf(v.y, g1, L[4]); L[5] = fmaf(v.z, g1, L[5]);
-              L[6] = fmaf(v.x, g2, L[6]); L[7] = fmaf(v.y, g2, L[7]); L[8] = fmaf(v.z, g2, L[8]);
-            }
+            for (int ox = 0; ox < 4; ++ox) v4[ox] = row[ox];
+            imp_gather_row(v4, wx, qx, wy[oy] * wz[oz], qy[oy] * wz[oz], wy[oy] * qz[oz], L);
           }
       }
     }
@@ -319,18 +330,14 @@ __global__ void __launch_bounds__(256) k_imp_apply(const SceneDev* __restrict__ 
       for (int oy = 0; oy < 4; ++oy) {
         const int iy = by + oy - 1;
         if (iy < 0 || iy >= D1) continue;
-        const float wyz = wy[oy] * wz[oz], qyz = qy[oy] * wz[oz], wqz = wy[oy] * qz[oz];
         const int64_t row = S.node_off + int64_t(D0) * (iy + int64_t(D1) * iz);
+        float4 v4[4];
 #pragma unroll
         for (int ox = 0; ox < 4; ++ox) {
           const int ix = bx + ox - 1;
-          if (ix < 0 || ix >= D0) continue;
-          const float4 v = vin[row + ix];
-          const float g0 = qx[ox] * wyz, g1 = wx[ox] * qyz, g2 = wx[ox] * wqz;
-          L[0] = fmaf(v.x, g0, L[0]); L[1] = fmaf(v.y, g0, L[1]); L[2] = fmaf(v.z, g0, L[2]);
-          L[3] = fmaf(v.x, g1, L[3]); L[4] = fmaf(v.y, g1, L[4]); L[5] = fmaf(v.z, g1, L[5]);
-          L[6] = fmaf(v.x, g2, L[6]); L[7] = fmaf(v.y, g2, L[7]); L[8] = fmaf(v.z, g2, L[8]);
+          v4[ox] = (ix >= 0 && ix < D0) ? vin[row + ix] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        imp_gather_row(v4, wx, qx, wy[oy] * wz[oz], qy[oy] * wz[oz], wy[oy] * qz[oz], L);
       }
     }
   }
