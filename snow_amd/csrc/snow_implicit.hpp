@@ -614,14 +614,17 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
               const float4 u1 = *reinterpret_cast<const float4*>(d + lz + 28);
               const float u2 = d[lz + 32];
               const float wxy = wqx.x * wqy.x, qxwy = wqx.y * wqy.x, wxqy = wqx.x * wqy.y;
+              // G grad(w) of node (ox, oy, z) = wz[z] a + qz[z] b with a = G(:,0) qx wy + G(:,1) wx qy
+              // and b = G(:,2) wx wy (grad(w) = (qx wy wz, wx qy wz, wx wy qz)); G(r,c) = G[c*3 + r]
+              const float a0 = fmaf(u0.x, qxwy, u0.w * wxqy), a1 = fmaf(u0.y, qxwy, u1.x * wxqy),
+                          a2 = fmaf(u0.z, qxwy, u1.y * wxqy);
+              const float b0 = u1.z * wxy, b1 = u1.w * wxy, b2 = u2 * wxy;
               const float wzv[4] = {wz.x, wz.y, wz.z, wz.w}, qzv[4] = {qz.x, qz.y, qz.z, qz.w};
 #pragma unroll
               for (int z = 0; z < 4; ++z) {
-                const float g0 = qxwy * wzv[z], g1 = wxqy * wzv[z], g2 = wxy * qzv[z];
-                // (G grad w)_r = G(r,0) g0 + G(r,1) g1 + G(r,2) g2, G(r,c) = G[c*3 + r]
-                sx[z] = fmaf(u0.x, g0, fmaf(u0.w, g1, fmaf(u1.z, g2, sx[z])));
-                sy[z] = fmaf(u0.y, g0, fmaf(u1.x, g1, fmaf(u1.w, g2, sy[z])));
-                sz[z] = fmaf(u0.z, g0, fmaf(u1.y, g1, fmaf(u2, g2, sz[z])));
+                sx[z] = fmaf(wzv[z], a0, fmaf(qzv[z], b0, sx[z]));
+                sy[z] = fmaf(wzv[z], a1, fmaf(qzv[z], b1, sy[z]));
+                sz[z] = fmaf(wzv[z], a2, fmaf(qzv[z], b2, sz[z]));
               }
             }
           }
