@@ -159,6 +159,7 @@ struct snow_ctx {
   uint32_t* d_imp_gslots = nullptr;      // IMP_GSLOTS bound slots per scene, IMP_GSTRIDE words apart
   ImpScal* h_imp_scal = nullptr;         // pinned host copy for the convergence polls
   uint32_t imp_nblocks = 0;
+  int imp_last_iters = 0;                // iterations of the last solve (for the first poll)
   uint8_t* d_imp_nlive = nullptr;        // per node tile: written by this step's grid update
   uint8_t* d_imp_blive = nullptr;        // per node-vector block: holds a node of such a tile
 };
@@ -566,14 +567,27 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
   k_imp_node<IMP_INIT><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
   LAUNCHCHK(c);
   if ((rc = scalars(IMP_INIT))) return rc;
+  // Polls: before the first update, then where the context's last solve stopped (solves of
+  // consecutive steps take about as many iterations), then every IMP_POLL. Updates launched after
+  // a scene is done return at once (its flag), so a late poll costs only those launches.
+  int next_poll = 0, polled_iters = 0;
+  bool finished = false;
   for (int it = 0; it < max_iter; ++it) {
-    if (it % IMP_POLL == 0) {
+    if (it == next_poll) {
       HIPCHK(c, hipMemcpyAsync(c->h_imp_scal, c->d_imp_scal, sizeof(ImpScal) * c->nscenes, hipMemcpyDeviceToHost,
                                c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
       bool all = true;
-      for (int s = 0; s < c->nscenes; ++s) all = all && c->h_imp_scal[s].done;
-      if (all) break;
+      polled_iters = 0;
+      for (int s = 0; s < c->nscenes; ++s) {
+        all = all && c->h_imp_scal[s].done;
+        polled_iters = std::max(polled_iters, int(c->h_imp_scal[s].iters));
+      }
+      if (all) {
+        finished = true;
+        break;
+      }
+      next_poll = c->imp_last_iters > it ? c->imp_last_iters : it + IMP_POLL;
     }
     if ((rc = apply(5))) return rc;  // K z
     k_imp_node<IMP_KR><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
@@ -587,6 +601,14 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
     if ((rc = scalars(IMP_X))) return rc;
   }
 #undef NODE_ARGS
+  if (!finished) {  // the loop ran out: read where the scenes stopped
+    HIPCHK(c, hipMemcpyAsync(c->h_imp_scal, c->d_imp_scal, sizeof(ImpScal) * c->nscenes, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    polled_iters = 0;
+    for (int s = 0; s < c->nscenes; ++s) polled_iters = std::max(polled_iters, int(c->h_imp_scal[s].iters));
+  }
+  c->imp_last_iters = polled_iters;
   k_imp_finish<<<nb, IMP_THREADS, 0, c->stream>>>(c->d_scenes, c->d_imp_blocks, c->d_imp_mass, c->d_imp_vec,
                                                   c->d_g_node, c->d_g_mass, c->d_imp_blive);
   LAUNCHCHK(c);
