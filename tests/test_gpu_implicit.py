@@ -174,3 +174,29 @@ def test_cpp_driver_semi_implicit_matches_python_binding(tmp_path):
         ref = ctx.read_particles()
     for k in ("x", "v", "fe", "fp"):
         assert np.array_equal(got[k], ref[k]), k
+
+
+def test_empty_scene_in_a_batch():
+    """A scene without particles has no node block with mass: its solve is done at its first residual
+    (0 iterations), and the other scene of the batch gives the same bits as alone."""
+    from snow_amd import ParticleSystem
+    empty = scenes.tiny_block(n_side=4, dims=16)
+    empty.particles = ParticleSystem()
+    full = deformed(scenes.snowball_drop(grad_scale=0.0), seed=4)
+    dt = 5e-4
+    with SnowContext(full) as ctx:
+        ctx.init()
+        for _ in range(3):
+            ctx.step_semi_implicit(dt, beta=1.0, tol=1e-6, max_iter=100)
+        single = ctx.read_particles()
+        it_single = ctx.implicit_stats()[0]
+    with SnowContext([empty, full]) as ctx:
+        ctx.init()
+        for _ in range(3):
+            ctx.step_semi_implicit(dt, beta=1.0, tol=1e-6, max_iter=100)
+        ctx.check_finite()
+        assert ctx.implicit_stats(0)[0] == 0
+        assert ctx.implicit_stats(1)[0] == it_single > 0
+        batch = ctx.read_particles(1)
+    for k in ("x", "v", "fe", "fp"):
+        assert np.array_equal(single[k], batch[k]), k
