@@ -200,3 +200,27 @@ def test_empty_scene_in_a_batch():
         batch = ctx.read_particles(1)
     for k in ("x", "v", "fe", "fp"):
         assert np.array_equal(single[k], batch[k]), k
+
+
+def test_partial_tiles_against_oracle(oracle_mod):
+    """A 37^3 grid (the last node tiles are partial) with the block near the grid's edge, physical
+    gradient: one step from a deformed state against the oracle, as the teacher-forced test."""
+    sc = deformed(scenes.tiny_block(n_side=14, dims=37, center=(0.5, 0.25, 0.8), velocity=(0.4, -1.5, 0.3),
+                                    grad_scale=0.0), seed=9)
+    dt, tol = 1e-3, 1e-6
+    o = oracle_mod.Oracle(sc)
+    o.init()
+    o.rho0[:] = o.rho0.astype(np.float32)
+    with SnowContext(sc) as ctx:
+        ctx.init()
+        ctx.write_particles(rho0=o.rho0.astype(np.float32))
+        ctx.step_semi_implicit(dt, beta=1.0, tol=tol, max_iter=200)
+        it_gpu, rr_gpu = ctx.implicit_stats()
+        g = ctx.read_particles()
+    it_o, rr_o = o.step_semi_implicit(dt, 1.0, tol, 200)
+    dx, dv = np.abs(g["x"] - o.x).max(), rel_err(g["v"], o.v)
+    print(f"\niterations GPU {it_gpu} ({rr_gpu:.2e}), oracle {it_o} ({rr_o:.2e}); x {dx:.2e}  v rel {dv:.2e}")
+    assert rr_gpu <= tol and rr_o <= tol
+    assert it_gpu > 0 and abs(it_gpu - it_o) <= 1
+    assert dx < 1e-6
+    assert dv < 1e-4
