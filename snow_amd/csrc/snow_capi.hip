@@ -593,12 +593,9 @@ int run_implicit(snow_ctx* c, float dt, float beta, float tol, int max_iter) {
     k_imp_node<IMP_KR><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
     LAUNCHCHK(c);
     if ((rc = scalars(IMP_KR))) return rc;
-    k_imp_node<IMP_P><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
+    k_imp_node<IMP_PX><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
     LAUNCHCHK(c);
-    if ((rc = scalars(IMP_P))) return rc;
-    k_imp_node<IMP_X><<<nb, IMP_THREADS, 0, c->stream>>>(NODE_ARGS);
-    LAUNCHCHK(c);
-    if ((rc = scalars(IMP_X))) return rc;
+    if ((rc = scalars(IMP_PX))) return rc;
   }
 #undef NODE_ARGS
   if (!finished) {  // the loop ran out: read where the scenes stopped

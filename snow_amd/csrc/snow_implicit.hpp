@@ -37,6 +37,7 @@ struct ImpScal {
   double bb;     // |M v*|^2
   double rr;     // |r|^2
   double tol2;   // squared relative residual at which the solve stops
+  double pp;     // Kp . M^-1 Kp of the current direction
   uint32_t spare;
   int32_t iters; // direction updates done
   int32_t done;  // converged, broke down or out of iterations
@@ -46,7 +47,7 @@ struct ImpScal {
   uint32_t pad[2];
 };
 
-enum ImpPhase { IMP_INIT = 0, IMP_KR = 1, IMP_P = 2, IMP_X = 3 };
+enum ImpPhase { IMP_INIT = 0, IMP_KR = 1, IMP_PX = 2 };
 
 constexpr int IMP_THREADS = 1024;
 constexpr int IMP_NODES_PER_THREAD = 4;
@@ -671,9 +672,11 @@ __device__ __forceinline__ void imp_block_sum(double v[3], int nq, double* __res
 
 // Node phases of the solve (K = M v - beta dt df, df = -acc 2^-e; z = r / m):
 //   IMP_INIT: vin = x = v*: r = M v* - K x, z = r / m, p = Kp = 0; partials |M v*|^2, |r|^2
-//   IMP_KR:   vin = z:      Kz = K z; partial z . Kz
-//   IMP_P:    p = z + beta p, Kp = Kz + beta Kp; partial Kp . Kp / m
-//   IMP_X:    x += alpha p, r -= alpha Kp, z -= alpha Kp / m; partial |r|^2
+//   IMP_KR:   vin = z:      Kz = K z; partials z . Kz, Kz . Kz / m, Kz . Kp / m (Kp of the last
+//                           direction), from which one reduction gives beta and the new direction's
+//                           Kp . M^-1 Kp = a + 2 beta b + beta^2 pp (Kp = Kz + beta Kp), hence alpha
+//   IMP_PX:   p = z + beta p, Kp = Kz + beta Kp, x += alpha p, r -= alpha Kp, z -= alpha Kp / m;
+//             partial |r|^2
 template <int PHASE>
 __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __restrict__ scenes,
                                                           const ImpBlock* __restrict__ blocks, uint32_t nblocks,
@@ -685,8 +688,9 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
   const ImpBlock B = blocks[blockIdx.x];
   const ImpScal& Z = scal[B.scene];
   if (PHASE != IMP_INIT && Z.done) return;  // the scene's partials are not read again
+  constexpr int NQ = PHASE == IMP_INIT ? 2 : (PHASE == IMP_KR ? 3 : 1);
   if (!blive[blockIdx.x]) {  // no node of a written tile: zero partials
-    if (threadIdx.x < (PHASE == IMP_INIT ? 2 : 1)) part[size_t(threadIdx.x) * nblocks + blockIdx.x] = 0.0;
+    if (threadIdx.x < NQ) part[size_t(threadIdx.x) * nblocks + blockIdx.x] = 0.0;
     return;
   }
   const SceneDev& S = scenes[B.scene];
@@ -733,34 +737,34 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_node(const SceneDev* __rest
         sum[0] += double(b.x) * b.x + double(b.y) * b.y + double(b.z) * b.z;
         sum[1] += double(r.x) * r.x + double(r.y) * r.y + double(r.z) * r.z;
       } else {
-        const float4 z = ZV[nd];
+        const float4 z = ZV[nd], kp = KP[nd];
         const float4 kz = make_float4(fmaf(bdt, a0, m * z.x), fmaf(bdt, a1, m * z.y), fmaf(bdt, a2, m * z.z), 0.f);
         KZ[nd] = kz;
+        const double im = 1.0 / double(m);
         sum[0] += double(z.x) * kz.x + double(z.y) * kz.y + double(z.z) * kz.z;
+        sum[1] += (double(kz.x) * kz.x + double(kz.y) * kz.y + double(kz.z) * kz.z) * im;
+        sum[2] += (double(kz.x) * kp.x + double(kz.y) * kp.y + double(kz.z) * kp.z) * im;
       }
-    } else if (PHASE == IMP_P) {
-      const float4 z = ZV[nd], kz = KZ[nd], p = P[nd], kp = KP[nd];
+    } else {
+      const float4 z = ZV[nd], kz = KZ[nd], p = P[nd], kp = KP[nd], x = X[nd], r = Rv[nd];
       const float4 pn = make_float4(fmaf(be, p.x, z.x), fmaf(be, p.y, z.y), fmaf(be, p.z, z.z), 0.f);
       const float4 kpn = make_float4(fmaf(be, kp.x, kz.x), fmaf(be, kp.y, kz.y), fmaf(be, kp.z, kz.z), 0.f);
       P[nd] = pn;
       KP[nd] = kpn;
-      const double im = 1.0 / double(m);
-      sum[0] += (double(kpn.x) * kpn.x + double(kpn.y) * kpn.y + double(kpn.z) * kpn.z) * im;
-    } else {
-      const float4 p = P[nd], kp = KP[nd], x = X[nd], r = Rv[nd], z = ZV[nd];
-      X[nd] = make_float4(fmaf(al, p.x, x.x), fmaf(al, p.y, x.y), fmaf(al, p.z, x.z), 0.f);
-      const float4 rn = make_float4(fmaf(-al, kp.x, r.x), fmaf(-al, kp.y, r.y), fmaf(-al, kp.z, r.z), 0.f);
+      X[nd] = make_float4(fmaf(al, pn.x, x.x), fmaf(al, pn.y, x.y), fmaf(al, pn.z, x.z), 0.f);
+      const float4 rn = make_float4(fmaf(-al, kpn.x, r.x), fmaf(-al, kpn.y, r.y), fmaf(-al, kpn.z, r.z), 0.f);
       const float am = al / m;
-      ZV[nd] = make_float4(fmaf(-am, kp.x, z.x), fmaf(-am, kp.y, z.y), fmaf(-am, kp.z, z.z), 0.f);
+      ZV[nd] = make_float4(fmaf(-am, kpn.x, z.x), fmaf(-am, kpn.y, z.y), fmaf(-am, kpn.z, z.z), 0.f);
       Rv[nd] = rn;
       sum[0] += double(rn.x) * rn.x + double(rn.y) * rn.y + double(rn.z) * rn.z;
     }
   }
-  imp_block_sum(sum, PHASE == IMP_INIT ? 2 : 1, part, nblocks);
+  imp_block_sum(sum, NQ, part, nblocks);
 }
 
 // One workgroup per scene: the scene's block partials summed in a fixed order, then the scalars
-// of the phase. START (phase < 0) resets the state for a new solve.
+// of the phase (after IMP_KR: beta, the direction's Kp . M^-1 Kp and alpha; after IMP_PX: the
+// residual and the stop test). START (phase < 0) resets the state for a new solve.
 __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const uint32_t* __restrict__ scene_blocks,
                                                              uint32_t nblocks, const double* __restrict__ part,
                                                              ImpScal* __restrict__ scal, const SceneDev* __restrict__ scenes,
@@ -775,7 +779,7 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const ui
   if (phase < 0) {
     if (threadIdx.x == 0) {
       const float dt = dt_override > 0.0f ? dt_override : scenes[sid].dt;
-      Z.rho = Z.alpha = Z.beta = Z.bb = Z.rr = 0.0;
+      Z.rho = Z.alpha = Z.beta = Z.bb = Z.rr = Z.pp = 0.0;
       Z.tol2 = double(tol) * double(tol);
       Z.iters = 0;
       Z.done = 0;
@@ -787,7 +791,7 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const ui
   }
   if (phase != IMP_INIT && Z.done) return;
   const uint32_t b0 = scene_blocks[sid], b1 = scene_blocks[sid + 1];
-  const int nq = phase == IMP_INIT ? 2 : 1;
+  const int nq = phase == IMP_INIT ? 2 : (phase == IMP_KR ? 3 : 1);
   __shared__ double s_v[3][IMP_THREADS];
   for (int q = 0; q < nq; ++q) {
     double s = 0.0;
@@ -801,20 +805,23 @@ __global__ void __launch_bounds__(IMP_THREADS) k_imp_scalars(int phase, const ui
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
-  const double t0 = s_v[0][0], t1 = s_v[1][0];
+  const double t0 = s_v[0][0], t1 = s_v[1][0], t2 = s_v[2][0];
   if (phase == IMP_INIT) {
     Z.bb = t0;
     Z.rr = t1;
+    Z.pp = 0.0;
     Z.done = (t1 <= Z.tol2 * t0 || Z.max_iter <= 0) ? 1 : 0;
   } else if (phase == IMP_KR) {
     if (Z.iters > 0 && !(Z.rho != 0.0)) {  // z . K z vanished (K is not definite): no next direction
       Z.done = 1;
       return;
     }
-    Z.beta = Z.iters == 0 ? 0.0 : t0 / Z.rho;
+    const double be = Z.iters == 0 ? 0.0 : t0 / Z.rho;
+    const double pp = t1 + 2.0 * be * t2 + be * be * Z.pp;  // Kp . M^-1 Kp of p = z + beta p
+    Z.beta = be;
     Z.rho = t0;
-  } else if (phase == IMP_P) {
-    if (t0 > 0.0) Z.alpha = Z.rho / t0;
+    Z.pp = pp;
+    if (pp > 0.0) Z.alpha = t0 / pp;
     else Z.done = 1;  // K p = 0: no further progress
   } else {
     Z.rr = t0;
