@@ -297,13 +297,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
       float4* box = s_box[threadIdx.x >> 6];
       constexpr int BU = IMP_BOX / 64;
       float4 tmp[BU];
+      const float4* v0 = vin + (S.node_off + xmin + int64_t(D0) * (ymin + int64_t(D1) * zmin));
+      const BoxSplit bs = box_split(ex, ey);
 #pragma unroll
       for (int u = 0; u < BU; ++u) {
         const int idx = lane + 64 * u;
         tmp[u] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (idx < vol) {
-          const int ix = idx % ex, iy = (idx / ex) % ey, iz = idx / (ex * ey);
-          tmp[u] = vin[S.node_off + (xmin + ix) + int64_t(D0) * ((ymin + iy) + int64_t(D1) * (zmin + iz))];
+          int ix, iy, iz;
+          box_coords(bs, idx, ix, iy, iz);
+          tmp[u] = v0[uint32_t(ix + D0 * (iy + D1 * iz))];
         }
       }
 #pragma unroll
@@ -639,8 +642,10 @@ __global__ void __launch_bounds__(256) k_imp_scatter(const SceneDev* __restrict_
   if (!boxed) return;
   __syncthreads();
   const int vol = ex * ey * ez;
+  const BoxSplit bs = box_split(ex, ey);
   for (int k = t; k < vol; k += 256) {
-    const int ix = k % ex, iy = (k / ex) % ey, iz = k / (ex * ey);
+    int ix, iy, iz;
+    box_coords(bs, k, ix, iy, iz);
     const int64_t node = S0.node_off + (bx0 + ix) + int64_t(S0.D[0]) * ((by0 + iy) + int64_t(S0.D[1]) * (bz0 + iz));
 #pragma unroll
     for (int q = 0; q < 3; ++q) {

@@ -304,6 +304,25 @@ __device__ __forceinline__ int wave_ext_i(int v) {
 __device__ __forceinline__ int wave_min_i(int v) { return wave_ext_i<false>(v); }
 __device__ __forceinline__ int wave_max_i(int v) { return wave_ext_i<true>(v); }
 
+// Coordinates of the box index idx = ix + ex (iy + ey iz) for extents uniform across the caller,
+// by float reciprocals instead of integer division (an integer division by a variable is ~20
+// VALU). With r = 1/ex in float (correctly rounded, or within 1 ulp), fma(idx, r, r/2) is within
+// a relative 2^-22 of (idx + 1/2)/ex, whose distance to the nearest integer is at least 1/(2 ex);
+// so truncation gives idx / ex exactly for 0 <= idx < 2^21, whatever ex >= 1.
+struct BoxSplit {
+  int ex, ey;
+  float rex, rey;
+};
+__device__ __forceinline__ BoxSplit box_split(int ex, int ey) {
+  return BoxSplit{ex, ey, 1.0f / float(ex), 1.0f / float(ey)};
+}
+__device__ __forceinline__ void box_coords(const BoxSplit& b, int idx, int& ix, int& iy, int& iz) {
+  const int t = int(fmaf(float(idx), b.rex, 0.5f * b.rex));  // idx / ex
+  ix = idx - t * b.ex;
+  iz = int(fmaf(float(t), b.rey, 0.5f * b.rey));  // t / ey
+  iy = t - iz * b.ey;
+}
+
 // A wave whose 64 lanes are all active and of one scene publishes its maxima once (DPP
 // reductions); otherwise each active lane publishes its own bound. NaN and infinite terms count
 // as the largest float.
@@ -1246,13 +1265,32 @@ __device__ __forceinline__ void g2p_gather_wave(const SceneDev& S, int sid, floa
     return;
   }
   const int lane = threadIdx.x & 63;
-  for (int idx = lane; idx < vol; idx += 64) {
-    const int ix = idx % ex, iy = (idx / ex) % ey, iz = idx / (ex * ey);
-    const int64_t node = S.node_off + (xmin + ix) + int64_t(D0) * ((ymin + iy) + int64_t(D1) * (zmin + iz));
-    const float2 a = g_node[3 * node], b = g_node[3 * node + 1], c = g_node[3 * node + 2];
-    box[3 * idx] = a;
-    box[3 * idx + 1] = b;
-    box[3 * idx + 2] = c;
+  // the box's first node; node offsets within the box fit 32 bits (D0 D1 D2 < 2^31)
+  const float2* g0 = g_node + 3 * (S.node_off + xmin + int64_t(D0) * (ymin + int64_t(D1) * zmin));
+  const BoxSplit bs = box_split(ex, ey);
+  // every load of the box in flight before the first LDS write
+  constexpr int BU = G2P_BOX / 64;
+  float2 ta[BU], tb[BU], tc[BU];
+#pragma unroll
+  for (int u = 0; u < BU; ++u) {
+    const int idx = lane + 64 * u;
+    if (idx < vol) {
+      int ix, iy, iz;
+      box_coords(bs, idx, ix, iy, iz);
+      const float2* src = g0 + 3u * uint32_t(ix + D0 * (iy + D1 * iz));
+      ta[u] = src[0];
+      tb[u] = src[1];
+      tc[u] = src[2];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < BU; ++u) {
+    const int idx = lane + 64 * u;
+    if (idx < vol) {
+      box[3 * idx] = ta[u];
+      box[3 * idx + 1] = tb[u];
+      box[3 * idx + 2] = tc[u];
+    }
   }
   // the wave's own LDS writes complete before its reads (in-order LDS per wave)
   __builtin_amdgcn_wave_barrier();
