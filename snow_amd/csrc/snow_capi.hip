@@ -88,6 +88,11 @@ struct snow_ctx {
   uint32_t* d_slot = nullptr;  // slot of the particle at each sorted position (the sort's permutation)
   void* d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
+  // The cell sort's first-pass histogram (rows of 2^hist_bits digits per 4096-key tile). The fused
+  // G2P counts the keys it writes into it, and the sort's first scatter clears it again.
+  uint32_t* d_hist0 = nullptr;
+  int hist_bits = 0;        // digit bits of its row layout (0: none yet)
+  bool hist_ready = false;  // it holds the counts of d_keys[0]; otherwise it is zero
   uint32_t *d_tile_c0 = nullptr, *d_tile_c1 = nullptr;  // first and last P2G chunk of each tile's particles
   float* d_tpart = nullptr;
   float* d_hpart = nullptr;
@@ -350,6 +355,27 @@ int run_keys(snow_ctx* c) {
     LAUNCHCHK(c);
   }
   c->keys_valid = true;
+  c->hist_ready = false;
+  return 0;
+}
+
+// The first-pass histogram of the current keys in d_hist0 (counted here unless the G2P that wrote
+// them did), in the row layout of the current digit width.
+int prepare_hist0(snow_ctx* c) {
+  if (!c->N) return 0;
+  const int bits0 = pass_bits(c->key_bits, c->sort_bits);
+  if (bits0 != c->hist_bits) {  // new row layout: start from zeros everywhere
+    const int64_t words = int64_t(hist0_words(c->N));
+    k_fill_u32<<<grid_for(words, 256), 256, 0, c->stream>>>(c->d_hist0, 0u, words);
+    LAUNCHCHK(c);
+    c->hist_bits = bits0;
+    c->hist_ready = false;
+  }
+  if (!c->hist_ready) {
+    HIPCHK(c, radix_count_first_pass(c->d_keys[0], c->N, c->key_bits, c->sort_bits, c->d_hist0, c->stream,
+                                     c->num_cu));
+    c->hist_ready = true;
+  }
   return 0;
 }
 
@@ -377,9 +403,11 @@ int run_sort(snow_ctx* c, float dt_rigid) {
   if (rc) return rc;
   StageTimer tm(c, ST_SORT);
   if (c->N) {
+    if ((rc = prepare_hist0(c))) return rc;
     size_t tb = c->sort_tmp_bytes;
     HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], nullptr, c->d_slot, c->N,
-                               c->key_bits, c->sort_bits, c->stream, c->num_cu, &begin));
+                               c->key_bits, c->sort_bits, c->stream, c->num_cu, &begin, c->d_hist0));
+    c->hist_ready = false;  // consumed: the scatter cleared the rows
   } else {
     k_step_begin<<<1, 256, 0, c->stream>>>(begin);
     LAUNCHCHK(c);
@@ -443,7 +471,7 @@ int run_g2p_advance(snow_ctx* c, float dt) {
 #define G2P_ARGS                                                                                          \
   c->d_scenes, c->nscenes > 1 ? c->d_waves : nullptr, c->nwaves, c->d_slot, c->d_rec[a], c->d_fe[a], c->d_fp[a], \
       c->d_pid[a], c->d_rec[b], c->d_fe[b], c->d_fp[b], c->d_pid[b], c->d_fa, c->d_keys[0], c->d_g_node,       \
-      c->d_cols, dt, c->N, f64, c->d_ctl, c->d_pbound
+      c->d_cols, dt, c->N, f64, c->d_ctl, c->d_pbound, c->hist_bits ? c->d_hist0 : nullptr, c->hist_bits
     const unsigned bs = unsigned(c->g2p_block);
     const unsigned g = grid_for(c->nwaves, bs / 64u);
     if (mixed && comp) k_g2p_advance<true, true><<<g, bs, 0, c->stream>>>(G2P_ARGS);
@@ -455,6 +483,7 @@ int run_g2p_advance(snow_ctx* c, float dt) {
   }
   c->cur = b;
   c->keys_valid = true;
+  c->hist_ready = c->N && c->hist_bits;  // counted into the zeros the sort's scatter left
   c->force_valid = true;
   c->bound_valid = true;
   c->aux_valid = false;
@@ -652,7 +681,7 @@ void release(snow_ctx* c) {
   }
   void* ptrs[] = {c->d_scenes, c->d_pscene, c->d_tile_scene, c->d_ntile_scene, c->d_rec[0], c->d_rec[1],
                   c->d_fe[0], c->d_fe[1], c->d_fp[0], c->d_fp[1], c->d_fa, c->d_pid[0],
-                  c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp,
+                  c->d_pid[1], c->d_keys[0], c->d_keys[1], c->d_slot, c->d_sort_tmp, c->d_hist0,
                   c->d_tile_c0, c->d_tile_c1, c->d_tpart, c->d_hpart, c->d_zero_part, c->d_list_ctr, c->d_tile_list, c->d_chunk_scene, c->d_chunk_lo, c->d_chunk_w, c->d_chunk_runs, c->d_fold_job, c->d_plan_pref, c->d_waves, c->d_ctl, c->d_pbound, c->d_tile_stamp, c->d_g_mmv, c->d_g_f, c->d_g_node, c->d_g_mass,
                   c->d_cols, c->d_counts, c->d_bad, c->d_aux, c->d_stage,
                   c->d_imp_vec, c->d_imp_mass, c->d_imp_acc, c->d_imp_pd, c->d_imp_G, c->d_imp_part,
@@ -837,6 +866,8 @@ int snow_create_batch(const snow_scene_desc* scenes, int32_t num_scenes, uint32_
                                std::max<uint32_t>(N, 1), c->key_bits, 11, c->stream, c->num_cu));
     c->sort_tmp_bytes = tb;
     CREATECHK(hipMalloc(&c->d_sort_tmp, std::max<size_t>(tb, 16)));
+    CREATECHK(dalloc(&c->d_hist0, hist0_words(std::max<uint32_t>(N, 1))));
+    CREATECHK(hipMemset(c->d_hist0, 0, sizeof(uint32_t) * hist0_words(std::max<uint32_t>(N, 1))));
   }
   CREATECHK(dalloc(&c->d_tile_c0, c->total_ctiles));
   CREATECHK(dalloc(&c->d_tile_c1, c->total_ctiles));
@@ -1060,6 +1091,7 @@ int snow_run_graph(snow_ctx* c, int32_t steps) {
   // of the state (the next step start makes parity step + 1 the current one)
   if ((rc = run_force_prepass(c))) return rc;
   if ((rc = run_keys(c))) return rc;
+  if ((rc = prepare_hist0(c))) return rc;  // the captured sorts take counted histograms
   if (!c->bound_valid && (rc = publish_bounds(c, 1u, 1))) return rc;
   if (!c->gexec[0]) {
     // One graph per buffer parity: the step reads buffers `cur` and writes `1 - cur`. Every launch
@@ -1151,6 +1183,7 @@ int snow_autotune(snow_ctx* c, int32_t steps, int32_t* g2p_block, int32_t* p2g_c
     LAUNCHCHK(c);
     c->cur = k;
     c->keys_valid = true;
+    c->hist_ready = false;  // the trials' G2P counted their own keys
     c->force_valid = true;
     c->bound_valid = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1300,6 +1333,7 @@ int snow_stage_particle_advance(snow_ctx* c, double dt) {
     LAUNCHCHK(c);
   }
   c->keys_valid = true;
+  c->hist_ready = false;
   c->force_valid = true;
   c->bound_valid = true;
   c->aux_valid = false;
@@ -1316,9 +1350,11 @@ int snow_stage_g2p_advance(snow_ctx* c, double dt) {
   // the cell sort of the current positions gives the slot permutation the fused kernel applies;
   // it runs without the step start, so the grid of the last P2G stays readable
   if (c->N) {
+    if ((rc = prepare_hist0(c))) return rc;
     size_t tb = c->sort_tmp_bytes;
     HIPCHK(c, radix_sort_pairs(c->d_sort_tmp, &tb, c->d_keys[0], c->d_keys[1], nullptr, c->d_slot, c->N,
-                               c->key_bits, c->sort_bits, c->stream, c->num_cu));
+                               c->key_bits, c->sort_bits, c->stream, c->num_cu, nullptr, c->d_hist0));
+    c->hist_ready = false;
   }
   // the FP64-path counter of this call; the next P2G's bound from this call's records alone
   k_fill_u32<<<1, 64, 0, c->stream>>>(reinterpret_cast<uint32_t*>(fp64_counter(c)), 0u, 2);
@@ -1397,7 +1433,10 @@ int snow_write_particles(snow_ctx* c, int32_t s, const float* x, const float* v,
                                                          mask);
   LAUNCHCHK(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (mask & 1) c->keys_valid = false;
+  if (mask & 1) {
+    c->keys_valid = false;
+    c->hist_ready = false;
+  }
   if (mask & (4 | 8 | 16)) c->force_valid = false;
   c->bound_valid = false;
   if (rho0) c->initialized = true;

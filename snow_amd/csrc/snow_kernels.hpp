@@ -29,6 +29,7 @@ constexpr int P2G_WAVES = 8;         // waves per P2G workgroup
 constexpr int P2G_SEG_MAX = 3072;    // largest P2G chunk in particles (sizes the LDS run lists)
 constexpr int P2G_BLOCKS_PER_CU = 2; // P2G workgroups resident per CU (about 70 KB of LDS each)
 constexpr int P2G_SEG_MEAN = 2048;   // largest mean chunk of a scene (its particles / its chunks)
+constexpr int SORT_TILE_KEYS = 4096;  // keys per tile of the cell sort (one histogram row each)
 constexpr uint32_t P2G_RUN_COST = 96; // planning cost of a run (tile) in particles: its barriers,
                                       // partial write and short batches (measured: ~2.6 us per
                                       // run against ~0.03 us per particle)
@@ -166,8 +167,9 @@ constexpr int ACTIVE_STRIDE = 16;  // 64-bit words from one slot to the next
 
 // Start of a step: new stamp, cleared counters, and the collider advance of rigidAdvance.glsl:14-18
 // (cx += dt*cv) when dt_rigid >= 0 (0 selects each scene's own dt). Nothing the sort reads is
-// touched, so the first sort launch carries this work in its workgroup 0 (k_radix_hist); the
-// kernel k_step_begin runs it alone when there is nothing to sort.
+// touched, so the first sort launch carries this work in its workgroup 0 (k_radix_hist, or
+// k_radix_colscan when the first histogram comes counted); the kernel k_step_begin runs it alone
+// when there is nothing to sort.
 struct StepBegin {
   StepCtl* ctl;
   unsigned long long* counts;  // active-node slots: ncounts slot heads, ACTIVE_STRIDE words apart
@@ -1372,6 +1374,25 @@ __device__ __forceinline__ bool needs_fp64(const float d[3], float crit_c, float
   return near;
 }
 
+// The next sort's first-pass histogram (rows of 2^hbits digits per tile of SORT_TILE keys, zero
+// on entry): the wave's active lanes are a prefix of consecutive positions, and each run of equal
+// (tile, digit) among them adds its length with one atomic from its first lane. Sorted order makes
+// the runs long, so a wave adds a few counts instead of 64.
+__device__ __forceinline__ void count_first_digit(uint32_t* __restrict__ hist0, int hbits, uint32_t i,
+                                                  uint32_t key) {
+  const int lane = int(threadIdx.x & 63);
+  const uint32_t tile = i / uint32_t(SORT_TILE_KEYS), d = key & ((1u << hbits) - 1u);
+  const uint32_t ptile = uint32_t(__shfl_up(int(tile), 1, 64)), pd = uint32_t(__shfl_up(int(d), 1, 64));
+  const bool head = lane == 0 || tile != ptile || d != pd;
+  const uint64_t heads = __ballot(head);
+  const int nact = __popcll(__ballot(true));
+  if (head) {
+    const uint64_t later = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+    const int end = later ? __ffsll((long long)later) - 1 : nact;
+    atomicAdd(hist0 + size_t(tile) * (size_t(1) << hbits) + d, uint32_t(end - lane));
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Least waves per SIMD the G2P register allocation must keep: 4 for the FP32 kernel (at most 128
 // VGPRs), 3 with the FP64 path compiled in (at most 168 VGPRs, so that the FP64 branch never
@@ -1396,7 +1417,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G2PWav
     float* __restrict__ fp_out, uint32_t* __restrict__ pid_out, float* __restrict__ fa_out,
     uint32_t* __restrict__ keys_next, const float2* __restrict__ g_node, const Collider* __restrict__ cols,
     float dt_override, uint32_t n, unsigned long long* __restrict__ fp64_count, const StepCtl* __restrict__ ctl,
-    uint32_t* __restrict__ pbound) {
+    uint32_t* __restrict__ pbound, uint32_t* __restrict__ hist0, int hbits) {
   __shared__ float2 s_box[4][3 * G2P_BOX];  // one node box per wave (blocks of up to 256 threads)
   __shared__ uint32_t s_pb[4];               // block maxima of the P2G bound, waves done
   const uint32_t wpb = blockDim.x >> 6;      // waves per block
@@ -1442,6 +1463,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G2PWav
     pid_out[i] = id;
     const uint32_t key = cell_key(S, x, y, z);
     keys_next[i] = key;
+    if (hist0) count_first_digit(hist0, hbits, i, key);
 
     // F_E, F_P update; the next step's force matrix (p2gTransfer.glsl:109-141, once per particle)
     // takes R_E = W V^T from the factors of this update's SVD: the stored F_E is W S' V^T with

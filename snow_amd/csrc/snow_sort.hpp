@@ -18,9 +18,11 @@
 //                    offsets and the exclusive scan of the digit totals, which every tile
 //                    recomputes.
 // Stability makes the result a deterministic function of the input order. No hipMemset and no
-// host synchronisation: the pass sequence can be captured in a HIP graph. The first histogram
-// launch can also carry the step's start (step_begin_block in its workgroup 0), which saves the
-// step a launch.
+// host synchronisation: the pass sequence can be captured in a HIP graph. The first launch can
+// also carry the step's start (step_begin_block in its workgroup 0), which saves the step a launch.
+// The first pass's histogram can also come counted beforehand (hist0: the writer of the keys adds
+// each run of equal (tile, digit) with one atomic); its scatter then clears each row after reading
+// it, so that the writer can count the next keys into zeros.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,7 +34,7 @@ namespace snow {
 namespace {
 constexpr int MAX_BITS = 11;
 constexpr int MAX_BINS = 1 << MAX_BITS;
-constexpr int SORT_TILE = 4096;  // keys per tile
+constexpr int SORT_TILE = SORT_TILE_KEYS;  // keys per tile
 
 template <int THREADS>
 __global__ void __launch_bounds__(THREADS) k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
@@ -86,8 +88,10 @@ constexpr int COL_THREADS = 1024;
 constexpr int COL_CACHE = 16;
 template <int DG>
 __global__ void __launch_bounds__(COL_THREADS) k_radix_colscan(uint32_t* __restrict__ hist, uint32_t ntiles,
-                                                               int bins, uint32_t* __restrict__ totals) {
+                                                               int bins, uint32_t* __restrict__ totals,
+                                                               StepBegin begin, int with_begin) {
   constexpr int PARTS = COL_THREADS / DG;
+  if (with_begin && blockIdx.x == 0) step_begin_block(begin, threadIdx.x, COL_THREADS);
   __shared__ uint32_t s_part[PARTS][DG];
   const int lane = threadIdx.x % DG, part = threadIdx.x / DG;
   const int d = int(blockIdx.x) * DG + lane;
@@ -141,8 +145,8 @@ __global__ void __launch_bounds__(COL_THREADS) k_radix_colscan(uint32_t* __restr
 template <int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, uint32_t n, int shift, int bits, const uint32_t* __restrict__ offsets,
-    const uint32_t* __restrict__ totals) {
+    uint32_t* __restrict__ vout, uint32_t n, int shift, int bits, uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ totals, int clear_rows) {
   constexpr int THREADS = 64 * WAVES;
   constexpr int ITEMS = SORT_TILE / THREADS;      // rounds of 64 keys per wave
   constexpr int WKEYS = SORT_TILE / WAVES;
@@ -178,6 +182,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_radix_scatter(
     tot[j] = on ? totals[d0 + j] : 0u;
     row[j] = on ? offsets[size_t(blockIdx.x) * bins + d0 + j] : 0u;
     part += tot[j];
+  }
+  if (clear_rows) {  // this tile's row is read by this workgroup only, and no longer needed
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (d0 + j < bins) offsets[size_t(blockIdx.x) * bins + d0 + j] = 0u;
   }
   __builtin_amdgcn_wave_barrier();
   // rank within the wave: running per-digit counters, one round of 64 keys at a time
@@ -239,6 +248,15 @@ __global__ void __launch_bounds__(64 * WAVES) k_radix_scatter(
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return uint32_t((a + b - 1) / b); }
 
+// Digit width of every pass: the key bits split into the fewest passes of at most digit_bits.
+inline int pass_bits(int end_bit, int digit_bits, int* passes_out = nullptr) {
+  const int eb = end_bit < 1 ? 1 : (end_bit > 32 ? 32 : end_bit);
+  const int db = digit_bits < 4 ? 4 : (digit_bits > MAX_BITS ? MAX_BITS : digit_bits);
+  const int passes = (eb + db - 1) / db;
+  if (passes_out) *passes_out = passes;
+  return (eb + passes - 1) / passes;
+}
+
 struct SortLayout {
   uint32_t ntiles;
   size_t hist_off, tot_off, ktmp_off, vtmp_off, total;
@@ -265,7 +283,7 @@ SortLayout layout(uint32_t n) {
 inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
                                    const uint32_t* vals_in, uint32_t* vals_out, uint32_t n, int end_bit,
                                    int digit_bits, hipStream_t stream, int num_cu,
-                                   const StepBegin* begin = nullptr) {
+                                   const StepBegin* begin = nullptr, uint32_t* hist0 = nullptr) {
   const SortLayout L = layout(n);
   if (!temp) {
     *temp_bytes = L.total;
@@ -279,9 +297,8 @@ inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_
   uint32_t* ktmp = reinterpret_cast<uint32_t*>(base + L.ktmp_off);
   uint32_t* vtmp = reinterpret_cast<uint32_t*>(base + L.vtmp_off);
   const int eb = end_bit < 1 ? 1 : (end_bit > 32 ? 32 : end_bit);
-  const int db = digit_bits < 4 ? 4 : (digit_bits > MAX_BITS ? MAX_BITS : digit_bits);
-  const int passes = (eb + db - 1) / db;
-  const int dbits = (eb + passes - 1) / passes;
+  int passes = 1;
+  const int dbits = pass_bits(end_bit, digit_bits, &passes);
   // 16 ranking waves per tile while all tiles fit one workgroup per CU; beyond that 4 waves per
   // tile, so that four tiles share a CU and no second round of workgroups is needed
   const bool wide = L.ntiles <= uint32_t(num_cu > 0 ? num_cu : 1);
@@ -295,30 +312,56 @@ inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_
     const int shift = p * dbits;
     const int bits = eb - shift < dbits ? eb - shift : dbits;
     const int bins = 1 << bits;
-    const int with_begin = (p == 0 && begin) ? 1 : 0;
-    const StepBegin sb = with_begin ? *begin : StepBegin{};
-    if (wide)
-      k_radix_hist<1024><<<L.ntiles, 1024, 0, stream>>>(kin, n, shift, bits, hist, sb, with_begin);
-    else
-      k_radix_hist<256><<<L.ntiles, 256, 0, stream>>>(kin, n, shift, bits, hist, sb, with_begin);
+    const bool counted = p == 0 && hist0;  // the first pass's histogram is already in hist0
+    uint32_t* h = counted ? hist0 : hist;
+    const int begin_here = (p == 0 && begin) ? 1 : 0;
+    const StepBegin sb = begin_here ? *begin : StepBegin{};
+    // the step start rides on the pass's first launch: the histogram, or the column scan
+    const int begin_hist = counted ? 0 : begin_here, begin_scan = counted ? begin_here : 0;
+    if (!counted) {
+      if (wide)
+        k_radix_hist<1024><<<L.ntiles, 1024, 0, stream>>>(kin, n, shift, bits, hist, sb, begin_hist);
+      else
+        k_radix_hist<256><<<L.ntiles, 256, 0, stream>>>(kin, n, shift, bits, hist, sb, begin_hist);
+    }
     if (L.ntiles <= uint32_t(16 * COL_CACHE))
-      k_radix_colscan<64><<<div_up(bins, 64), COL_THREADS, 0, stream>>>(hist, L.ntiles, bins, totals);
+      k_radix_colscan<64><<<div_up(bins, 64), COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb, begin_scan);
     else if (L.ntiles <= uint32_t(32 * COL_CACHE))
-      k_radix_colscan<32><<<div_up(bins, 32), COL_THREADS, 0, stream>>>(hist, L.ntiles, bins, totals);
+      k_radix_colscan<32><<<div_up(bins, 32), COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb, begin_scan);
     else
-      k_radix_colscan<16><<<div_up(bins, 16), COL_THREADS, 0, stream>>>(hist, L.ntiles, bins, totals);
+      k_radix_colscan<16><<<div_up(bins, 16), COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb, begin_scan);
+    const int clear = counted ? 1 : 0;
     if (wide)
       k_radix_scatter<16><<<L.ntiles, 64 * 16, (16 * bins + 16) * sizeof(uint32_t), stream>>>(
-          kin, vin, ko, vo, n, shift, bits, hist, totals);
+          kin, vin, ko, vo, n, shift, bits, h, totals, clear);
     else
       k_radix_scatter<4><<<L.ntiles, 64 * 4, (4 * bins + 4) * sizeof(uint32_t), stream>>>(
-          kin, vin, ko, vo, n, shift, bits, hist, totals);
+          kin, vin, ko, vo, n, shift, bits, h, totals, clear);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     kin = ko;
     vin = vo;
   }
   return hipSuccess;
+}
+
+// Words of a first-pass histogram for n keys: one row of up to 2^MAX_BITS digits per tile.
+inline size_t hist0_words(uint32_t n) { return size_t(layout(n).ntiles) * MAX_BINS; }
+
+// The first pass's histogram of keys counted into hist0 (rows overwritten), for a sort whose keys
+// were not counted by their writer.
+inline hipError_t radix_count_first_pass(const uint32_t* keys, uint32_t n, int end_bit, int digit_bits,
+                                         uint32_t* hist0, hipStream_t stream, int num_cu) {
+  if (n == 0) return hipSuccess;
+  const SortLayout L = layout(n);
+  const int eb = end_bit < 1 ? 1 : (end_bit > 32 ? 32 : end_bit);
+  const int dbits = pass_bits(end_bit, digit_bits);
+  const int bits = eb < dbits ? eb : dbits;
+  if (L.ntiles <= uint32_t(num_cu > 0 ? num_cu : 1))
+    k_radix_hist<1024><<<L.ntiles, 1024, 0, stream>>>(keys, n, 0, bits, hist0, StepBegin{}, 0);
+  else
+    k_radix_hist<256><<<L.ntiles, 256, 0, stream>>>(keys, n, 0, bits, hist0, StepBegin{}, 0);
+  return hipGetLastError();
 }
 
 }  // namespace snow
