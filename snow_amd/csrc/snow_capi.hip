@@ -92,7 +92,7 @@ struct snow_ctx {
   // G2P counts the keys it writes into it, and the sort's first scatter clears it again.
   uint32_t* d_hist0 = nullptr;
   int hist_bits = 0;        // digit bits of its row layout (0: none yet)
-  bool hist_ready = false;  // it holds the counts of d_keys[0]; otherwise it is zero
+  bool hist_ready = false;  // it holds the counts of d_keys[0]; if not, the next sort recounts them
   uint32_t *d_tile_c0 = nullptr, *d_tile_c1 = nullptr;  // first and last P2G chunk of each tile's particles
   float* d_tpart = nullptr;
   float* d_hpart = nullptr;

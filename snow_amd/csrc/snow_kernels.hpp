@@ -1374,7 +1374,7 @@ __device__ __forceinline__ bool needs_fp64(const float d[3], float crit_c, float
   return near;
 }
 
-// The next sort's first-pass histogram (rows of 2^hbits digits per tile of SORT_TILE keys, zero
+// The next sort's first-pass histogram (rows of 2^hbits digits per tile of SORT_TILE_KEYS keys, zero
 // on entry): the wave's active lanes are a prefix of consecutive positions, and each run of equal
 // (tile, digit) among them adds its length with one atomic from its first lane. Sorted order makes
 // the runs long, so a wave adds a few counts instead of 64.
