@@ -167,9 +167,9 @@ constexpr int ACTIVE_STRIDE = 16;  // 64-bit words from one slot to the next
 
 // Start of a step: new stamp, cleared counters, and the collider advance of rigidAdvance.glsl:14-18
 // (cx += dt*cv) when dt_rigid >= 0 (0 selects each scene's own dt). Nothing the sort reads is
-// touched, so the first sort launch carries this work in its workgroup 0 (k_radix_hist, or
-// k_radix_colscan when the first histogram comes counted); the kernel k_step_begin runs it alone
-// when there is nothing to sort.
+// touched, so the first sort launch carries this work (workgroup 0 of k_radix_hist, or an extra
+// workgroup of k_radix_colscan when the first histogram comes counted); the kernel k_step_begin
+// runs it alone when there is nothing to sort.
 struct StepBegin {
   StepCtl* ctl;
   unsigned long long* counts;  // active-node slots: ncounts slot heads, ACTIVE_STRIDE words apart

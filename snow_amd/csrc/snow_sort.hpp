@@ -91,7 +91,10 @@ __global__ void __launch_bounds__(COL_THREADS) k_radix_colscan(uint32_t* __restr
                                                                int bins, uint32_t* __restrict__ totals,
                                                                StepBegin begin, int with_begin) {
   constexpr int PARTS = COL_THREADS / DG;
-  if (with_begin && blockIdx.x == 0) step_begin_block(begin, threadIdx.x, COL_THREADS);
+  if (with_begin && blockIdx.x == gridDim.x - 1) {  // one workgroup beyond the digits runs the step start
+    step_begin_block(begin, threadIdx.x, COL_THREADS);
+    return;
+  }
   __shared__ uint32_t s_part[PARTS][DG];
   const int lane = threadIdx.x % DG, part = threadIdx.x / DG;
   const int d = int(blockIdx.x) * DG + lane;
@@ -316,7 +319,8 @@ inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_
     uint32_t* h = counted ? hist0 : hist;
     const int begin_here = (p == 0 && begin) ? 1 : 0;
     const StepBegin sb = begin_here ? *begin : StepBegin{};
-    // the step start rides on the pass's first launch: the histogram, or the column scan
+    // the step start rides on the pass's first launch: workgroup 0 of the histogram, or an extra
+    // last workgroup of the column scan
     const int begin_hist = counted ? 0 : begin_here, begin_scan = counted ? begin_here : 0;
     if (!counted) {
       if (wide)
@@ -325,11 +329,14 @@ inline hipError_t radix_sort_pairs(void* temp, size_t* temp_bytes, const uint32_
         k_radix_hist<256><<<L.ntiles, 256, 0, stream>>>(kin, n, shift, bits, hist, sb, begin_hist);
     }
     if (L.ntiles <= uint32_t(16 * COL_CACHE))
-      k_radix_colscan<64><<<div_up(bins, 64), COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb, begin_scan);
+      k_radix_colscan<64><<<div_up(bins, 64) + begin_scan, COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb,
+                                                                                     begin_scan);
     else if (L.ntiles <= uint32_t(32 * COL_CACHE))
-      k_radix_colscan<32><<<div_up(bins, 32), COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb, begin_scan);
+      k_radix_colscan<32><<<div_up(bins, 32) + begin_scan, COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb,
+                                                                                     begin_scan);
     else
-      k_radix_colscan<16><<<div_up(bins, 16), COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb, begin_scan);
+      k_radix_colscan<16><<<div_up(bins, 16) + begin_scan, COL_THREADS, 0, stream>>>(h, L.ntiles, bins, totals, sb,
+                                                                                     begin_scan);
     const int clear = counted ? 1 : 0;
     if (wide)
       k_radix_scatter<16><<<L.ntiles, 64 * 16, (16 * bins + 16) * sizeof(uint32_t), stream>>>(
